@@ -1,0 +1,111 @@
+// gst_common.h -- layouts shared by the host engine and the gfx950 kernels.
+//
+// State layout (one query, "slot"): T[((v * ND) + d) * W + p] is the packed
+// 64-bit state (cost << 32 | pred) of the DP entry T(v, p, d) -- vertex-major,
+// so the W * ND costs of one vertex form one contiguous, coalesced segment.
+// ND = D + 1 for the hop-bounded DP and 1 for the unbounded one.  The reference
+// keeps 28 B / 40 B AoS nodes per state (code/GPUGST/include/DPQ.cuh:51-58,
+// code/D-GPUGST/include/DPQ.cuh:51-58) and writes cost and predecessor with two
+// separate, racing stores; here both live in one word updated by one atomicMin.
+#pragma once
+#include <stdint.h>
+
+namespace gst {
+
+constexpr uint32_t kInf = 0x3FFFFFFFu;  // cost sentinel, saturating arithmetic below 2^31
+constexpr uint64_t kInfState = (uint64_t(kInf) << 32) | 0xFFFFFFFFull;
+constexpr int kMaxK = 6;                // W = 2^k <= 64: one wave lane per subset
+constexpr int kMaxW = 1 << kMaxK;
+constexpr int kMaxND = 16;              // D <= 15
+constexpr int kMaxSlots = 1024;         // concurrently admitted queries per engine
+
+// pred word: bits 31..30 type; grow: bits 29..0 = index of the arc v->u in the
+// row of the state's own vertex v (so u and w(u,v) are one load away during
+// reconstruction); merge: bits 0..15 = sub-mask p1, 16..22 = d1, 23..29 = d2.
+constexpr uint32_t kPredSeed = 0u;
+constexpr uint32_t kPredGrow = 1u << 30;
+constexpr uint32_t kPredMerge = 2u << 30;
+constexpr uint32_t kArcMask = (1u << 30) - 1;
+
+// block shape and work-item granularities of the engine kernel
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / 64;
+constexpr uint32_t kFillWords = 16384;  // u64 words per FILL item
+constexpr uint32_t kMergeTV = 64;       // vertices per MERGE / FINAL item
+constexpr uint32_t kDistTV = 256;       // vertices per DIST item
+constexpr uint32_t kClearPer = 1024;    // mask clears per CLEAR item
+constexpr int kStageWords = 1152;       // per-wave LDS staging (u64): 64*ND rows or the W*k*k route table
+
+enum Phase : int32_t {
+  PH_FREE = 0,
+  PH_FILL = 1,   // initialise T, masks, distance table
+  PH_SEED = 2,   // T(v, {i}, 0) = 0 for members of query group i
+  PH_ROUND = 3,  // one frontier round of grow (label-correcting) at level L
+  PH_DIST = 4,   // group->vertex distance table + inter-group distances + route table
+  PH_MERGE = 5,  // level-L merge pass over every vertex segment
+  PH_FINAL = 6,  // full-set merge + argmin
+  PH_RECON = 7,  // predecessor chase -> tree edges
+  PH_DONE = 8
+};
+
+struct GraphDev {
+  const int64_t* rowptr;  // V + 1
+  const int32_t* col;     // n_arcs
+  const int32_t* wt;      // n_arcs
+  const int32_t* rev;     // n_arcs: index of the reverse arc
+  const int64_t* gptr;    // n_groups + 1
+  const int32_t* gmem;    // members
+  int64_t V;
+  int64_t n_arcs;
+};
+
+struct EdgeDev {
+  int32_t u, v, w, parent;
+};
+
+struct alignas(64) Slot {
+  // ---- query (set at admission) ----
+  int32_t k, W, ND, D;  // D < 0: unbounded (ND == 1)
+  int32_t qg[kMaxK];
+  int32_t edge_cap;
+  int32_t chunk;        // arcs per ROUND item
+  uint64_t nT;          // V * ND * W
+  uint64_t* T;
+  uint64_t* mask;       // 3 * V (triple-buffered dirty sub-masks)
+  uint32_t* C;          // V * k group->vertex distances (padded to even)
+  uint32_t* list;       // 3 * V frontier vertices
+  uint32_t* loff;       // 3 * V arc offsets of frontier entries
+  uint32_t* R;          // W * k * k route table
+  EdgeDev* edges;       // edge_cap
+  uint32_t* rq;         // 3 * (edge_cap + 2 * kMaxK) reconstruction queue
+  // ---- engine state ----
+  int32_t phase, level;
+  uint32_t step, in_step;
+  uint32_t n_work, n_items;
+  uint64_t app[3];      // (count << 32) | arcs of the three frontier buffers
+  uint32_t best;        // upper bound on the optimum (cost of a real walk-tree)
+  uint32_t status;
+  uint64_t ans;         // (cost << 32) | root vertex
+  uint32_t vv[kMaxK * kMaxK];
+  int32_t out_cost, out_root, out_nedges, out_d;
+  unsigned long long ctr[6];
+};
+
+struct alignas(64) Ctrl {
+  uint32_t arrive;
+  uint32_t gen;
+  uint32_t item_next;
+  uint32_t item_total;
+  uint32_t n_active;
+  uint32_t exit_flag;
+  uint32_t abort_flag;
+  uint32_t finished;
+  uint32_t steps;
+  uint32_t exit_after;
+  uint32_t max_steps;
+  uint32_t nslots;
+  uint32_t active[kMaxSlots];
+  uint32_t prefix[kMaxSlots + 1];
+};
+
+}  // namespace gst

@@ -1,0 +1,978 @@
+// gst_device.hip -- the persistent DPBF engine kernel for gfx950 (CDNA4, wave64).
+//
+// One launch runs many queries ("slots") at once.  The grid loops over
+// global steps; between steps the last block to arrive at a software grid
+// barrier advances every slot's state machine and publishes the next step's
+// work items (prefix over slots); waves then pull items with one atomic each.
+//
+// Per query the DP is evaluated level by level in |p| (Dreyfus-Wagner order):
+//   level 1  : SEED + frontier rounds  -> T(v,{i},d) = hop-bounded / plain
+//              group->vertex distances (the reference's dis0_init_1 SSSP,
+//              code/GPUGST/src/DPQ.cu:241-271, and the singleton states)
+//   DIST     : compact distance table C[v][i], inter-group distances vv,
+//              route table R (host loops of code/GPUGST/src/DPQ.cu:409-483
+//              moved on the device)
+//   level L  : MERGE pass (every vertex segment staged in LDS; all sub-mask
+//              splits of every |p| = L subset, hop pairs d1 + d2 <= D, new
+//              depth max(d1,d2) -- code/D-GPUGST/src/DPQ.cu:100-124) followed by
+//              frontier rounds of grow (code/GPUGST/src/DPQ.cu:113-136,
+//              code/D-GPUGST/src/DPQ.cu:80-98) with lower-bound pruning
+//   FINAL    : full-set merge, argmin over (v, d) (DPQ.cu:558-575)
+//   RECON    : predecessor chase into tree edges (the reference's commented
+//              out reconstruction, DPQ.cu:577-606, and CPUHOP.h:234-259)
+// Merges only read levels < L, which are final, so every state is exact when
+// its level ends; grow within a level is a label-correcting closure.
+//
+// Pruning: a candidate for T(u, p, d) is dropped when cand + lb(u, ~p) > best,
+// best being the cost of a real walk-tree (star or complement merge) and lb a
+// proven lower bound on the cheapest tree rooted at u covering ~p (one-label
+// bound and tour bound, GPUGST+ one_label_lb_1, code/GPUGST+/src/
+// GSTnonHop.cu:88-138).  Using '>' keeps every state of an optimal witness.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "gst_common.h"
+
+namespace gst {
+
+struct WaveLDS {
+  uint32_t wv[64];
+  uint32_t woff[64];
+  uint32_t wrs[64];
+  uint32_t pad[64];
+  uint64_t wm[64];
+  uint64_t stage[kStageWords];
+};
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t x) {
+  const int lane = lane_id();
+  uint32_t incl = x;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t y = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += y;
+  }
+  return incl - x;
+}
+
+__device__ __forceinline__ uint64_t wave_min_u64(uint64_t x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint64_t y = (uint64_t)__shfl_xor((unsigned long long)x, o, 64);
+    x = y < x ? y : x;
+  }
+  return x;
+}
+
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    uint32_t y = __shfl_xor(x, o, 64);
+    x = y < x ? y : x;
+  }
+  return x;
+}
+
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += (uint64_t)__shfl_xor((unsigned long long)x, o, 64);
+  return x;
+}
+
+__device__ __forceinline__ uint64_t sidx(uint32_t v, uint32_t p, uint32_t d, int ND, int W) {
+  return ((uint64_t)v * (uint64_t)ND + d) * (uint64_t)W + p;
+}
+
+__device__ __forceinline__ uint32_t sat(uint32_t a) { return a < kInf ? a : kInf; }
+
+// Lower bound on the cost of the cheapest tree containing u that covers the
+// groups in x (x non-empty): max( max_{i in x} c_i(u),
+//   ceil( min_{i != j in x} (c_i(u) + R[x][i][j] + c_j(u)) / 2 ) ).
+// The second term halves a closed walk from u through every group of x; R is
+// the cheapest group-to-group route visiting all of x from i to j.
+__device__ __forceinline__ uint32_t lower_bound(const uint32_t* __restrict__ Crow, int k,
+                                                uint32_t x, const uint32_t* __restrict__ R) {
+  uint32_t c[kMaxK];
+  uint32_t lb1 = 0;
+#pragma unroll
+  for (int i = 0; i < kMaxK; i++) {
+    c[i] = (i < k) ? Crow[i] : kInf;
+    if (i < k && ((x >> i) & 1u)) lb1 = c[i] > lb1 ? c[i] : lb1;
+  }
+  if (lb1 >= kInf) return kInf;
+  if (__popc(x) >= 2) {
+    const uint32_t* Rx = R + (size_t)x * (uint32_t)(k * k);
+    uint32_t tour = 3u * kInf;
+#pragma unroll
+    for (int i = 0; i < kMaxK; i++) {
+      if (i >= k || !((x >> i) & 1u)) continue;
+#pragma unroll
+      for (int j = 0; j < kMaxK; j++) {
+        if (j >= k || j == i || !((x >> j) & 1u)) continue;
+        uint32_t t = c[i] + Rx[i * k + j] + c[j];
+        tour = t < tour ? t : tour;
+      }
+    }
+    uint32_t lb2 = (tour + 1u) >> 1;
+    lb1 = lb2 > lb1 ? lb2 : lb1;
+  }
+  return sat(lb1);
+}
+
+// Wave-aggregated append of vertex u (when flag) to frontier buffer `buf`:
+// one 64-bit atomicAdd of (count << 32 | degree sum) per wave gives every
+// appended vertex both its list position and its arc offset, so the next
+// round can split its arcs into equal chunks without a separate scan.
+__device__ __forceinline__ void wave_append(bool flag, uint32_t u, Slot* sp, int buf,
+                                            const GraphDev& g, uint64_t V) {
+  const uint64_t ball = __ballot(flag);
+  if (ball == 0) return;
+  const int lane = lane_id();
+  uint32_t deg = 0;
+  if (flag) deg = (uint32_t)(g.rowptr[u + 1] - g.rowptr[u]);
+  const uint32_t excl = wave_excl_scan(deg);
+  const uint32_t total = __shfl(excl + deg, 63, 64);
+  const int leader = __ffsll((long long)ball) - 1;
+  unsigned long long old = 0;
+  if (lane == leader) {
+    old = atomicAdd((unsigned long long*)&sp->app[buf],
+                    ((unsigned long long)__popcll(ball) << 32) | (unsigned long long)total);
+  }
+  old = __shfl(old, leader, 64);
+  if (flag) {
+    const uint32_t rank = __popcll(ball & __lanemask_lt());
+    const uint32_t pos = (uint32_t)(old >> 32) + rank;
+    sp->list[(uint64_t)buf * V + pos] = u;
+    sp->loff[(uint64_t)buf * V + pos] = (uint32_t)old + excl;
+  }
+}
+
+// ---------------------------------------------------------------- items ----
+
+__device__ void item_fill(Slot* sp, uint32_t idx, uint64_t V) {
+  const uint64_t nT = sp->nT;
+  const uint64_t nM = 3 * V;
+  const uint64_t nC = ((uint64_t)V * sp->k + 1) / 2;
+  const uint64_t total = nT + nM + nC;
+  const uint64_t base = (uint64_t)idx * kFillWords;
+  const uint64_t end = base + kFillWords < total ? base + kFillWords : total;
+  uint64_t* T = sp->T;
+  uint64_t* M = sp->mask;
+  uint64_t* C64 = (uint64_t*)sp->C;
+  const uint64_t cinf = ((uint64_t)kInf << 32) | kInf;
+  for (uint64_t t = base + lane_id(); t < end; t += 64) {
+    if (t < nT)
+      T[t] = kInfState;
+    else if (t < nT + nM)
+      M[t - nT] = 0;
+    else
+      C64[t - nT - nM] = cinf;
+  }
+}
+
+__device__ void item_seed(const GraphDev& g, Slot* sp, uint32_t i) {
+  const int W = sp->W, ND = sp->ND;
+  const int nxt = (sp->step + 1) % 3;
+  const uint64_t V = (uint64_t)g.V;
+  const int32_t grp = sp->qg[i];
+  const int64_t beg = g.gptr[grp], end = g.gptr[grp + 1];
+  uint64_t* mn = sp->mask + (uint64_t)nxt * V;
+  for (int64_t base = beg; base < end; base += 64) {
+    const int64_t t = base + lane_id();
+    bool app = false;
+    uint32_t v = 0;
+    if (t < end) {
+      v = (uint32_t)g.gmem[t];
+      sp->T[sidx(v, 1u << i, 0, ND, W)] = (uint64_t)kPredSeed;  // cost 0
+      unsigned long long old = atomicOr((unsigned long long*)&mn[v], 1ull << (1u << i));
+      app = (old == 0);
+    }
+    wave_append(app, v, sp, nxt, g, V);
+  }
+}
+
+__device__ void item_clear(Slot* sp, uint32_t idx, uint64_t V) {
+  const int buf = (sp->step + 2) % 3;
+  const uint32_t n = (uint32_t)(sp->app[buf] >> 32);
+  const uint32_t* lst = sp->list + (uint64_t)buf * V;
+  uint64_t* m = sp->mask + (uint64_t)buf * V;
+  const uint32_t base = idx * kClearPer;
+  const uint32_t end = base + kClearPer < n ? base + kClearPer : n;
+  for (uint32_t t = base + lane_id(); t < end; t += 64) m[lst[t]] = 0;
+}
+
+// One chunk of arcs of the current frontier: grow T(v,p,d) -> T(u,p,d') for
+// every arc v->u in the chunk and every dirty subset p of v.
+__device__ void item_round(const GraphDev& g, Slot* sp, uint32_t chunk, WaveLDS* L) {
+  const int lane = lane_id();
+  const uint64_t V = (uint64_t)g.V;
+  const int k = sp->k, W = sp->W, ND = sp->ND, D = sp->D;
+  const bool hop = D >= 0;
+  const int dmax = hop ? D : 1;  // source depths that may grow
+  const int level = sp->level;
+  const uint32_t full = (uint32_t)W - 1;
+  const uint32_t r = sp->step;
+  const int cur = r % 3, nxt = (r + 1) % 3;
+  const uint64_t appc = sp->app[cur];
+  const uint32_t n = (uint32_t)(appc >> 32);
+  const uint32_t arcs_total = (uint32_t)appc;
+  const uint32_t CH = (uint32_t)sp->chunk;
+  const uint32_t a0 = chunk * CH;
+  const uint32_t a1 = a0 + CH < arcs_total ? a0 + CH : arcs_total;
+  const uint32_t* lst = sp->list + (uint64_t)cur * V;
+  const uint32_t* lof = sp->loff + (uint64_t)cur * V;
+  const uint64_t* mcur = sp->mask + (uint64_t)cur * V;
+  uint64_t* mnxt = sp->mask + (uint64_t)nxt * V;
+  uint64_t* T = sp->T;
+  const uint32_t* C = sp->C;
+  const uint32_t* R = sp->R;
+
+  // last frontier entry whose arc range starts at or before a0
+  uint32_t lo = 0, hi = n;
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (lof[mid] <= a0) lo = mid; else hi = mid;
+  }
+  uint64_t n_relax = 0, n_impr = 0;
+  uint32_t a = a0;
+  for (uint32_t i = lo; a < a1; i += 64) {
+    const uint32_t idx = i + lane;
+    const bool valid = idx < n;
+    const uint32_t v = valid ? lst[idx] : 0u;
+    L->wv[lane] = v;
+    L->woff[lane] = valid ? lof[idx] : arcs_total;
+    L->wrs[lane] = valid ? (uint32_t)g.rowptr[v] : 0u;
+    L->wm[lane] = valid ? mcur[v] : 0ull;
+    const uint32_t wend = (i + 64 < n) ? lof[i + 64] : arcs_total;
+    wave_sync();
+    const uint32_t hia = a1 < wend ? a1 : wend;
+    for (uint32_t base = a; base < hia; base += 64) {
+      const uint32_t pos = base + lane;
+      bool app = false;
+      uint32_t u = 0;
+      if (pos < hia) {
+        int o = 0;
+#pragma unroll
+        for (int s = 32; s > 0; s >>= 1)
+          if (L->woff[o + s] <= pos) o += s;
+        const uint32_t vs = L->wv[o];
+        const uint64_t m = L->wm[o];
+        const uint32_t e = L->wrs[o] + (pos - L->woff[o]);
+        u = (uint32_t)g.col[e];
+        const uint32_t w = (uint32_t)g.wt[e];
+        uint32_t revarc = 0xFFFFFFFFu;
+        uint64_t improved = 0;
+        uint64_t mm = m;
+        while (mm) {
+          const uint32_t p = (uint32_t)(__ffsll((long long)mm) - 1);
+          mm &= mm - 1;
+          const uint32_t x = full ^ p;
+          uint32_t lbx = 0xFFFFFFFFu;
+          for (int d = 0; d < dmax; d++) {
+            const uint32_t sc = (uint32_t)(T[sidx(vs, p, d, ND, W)] >> 32);
+            if (sc >= kInf) continue;
+            const uint32_t cand = sc + w;
+            if (cand >= kInf) continue;
+            n_relax++;
+            const int td = hop ? d + 1 : 0;
+            unsigned long long* tp = (unsigned long long*)&T[sidx(u, p, td, ND, W)];
+            const uint32_t tc = (uint32_t)(*tp >> 32);
+            if (cand >= tc) continue;
+            if (level >= 2) {
+              if (lbx == 0xFFFFFFFFu) lbx = lower_bound(C + (uint64_t)u * k, k, x, R);
+              const uint32_t bst = __hip_atomic_load(&sp->best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              if (cand + lbx > bst) continue;
+            }
+            if (revarc == 0xFFFFFFFFu) revarc = (uint32_t)g.rev[e];
+            const unsigned long long nv = ((unsigned long long)cand << 32) | (kPredGrow | revarc);
+            const unsigned long long old = atomicMin(tp, nv);
+            if ((uint32_t)(old >> 32) > cand) {
+              improved |= 1ull << p;
+              n_impr++;
+              if (__popc(x) <= level) {
+                // a complete walk-tree: this state plus the complement state at u
+                uint32_t cbest = kInf;
+                const int d2max = hop ? (D - td) : 0;
+                for (int d2 = 0; d2 <= d2max; d2++) {
+                  const uint32_t c2 = (uint32_t)(T[sidx(u, x, d2, ND, W)] >> 32);
+                  cbest = c2 < cbest ? c2 : cbest;
+                }
+                if (cbest < kInf) {
+                  const uint32_t ub = sat(cand + cbest);
+                  if (ub < __hip_atomic_load(&sp->best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                    atomicMin(&sp->best, ub);
+                }
+              }
+            }
+          }
+        }
+        if (improved) {
+          const unsigned long long om = atomicOr((unsigned long long*)&mnxt[u], improved);
+          app = (om == 0);
+        }
+      }
+      wave_append(app, u, sp, nxt, g, V);
+    }
+    a = hia;
+    wave_sync();
+  }
+  n_relax = wave_sum_u64(n_relax);
+  n_impr = wave_sum_u64(n_impr);
+  if (lane == 0) {
+    atomicAdd(&sp->ctr[3], (unsigned long long)n_relax);
+    atomicAdd(&sp->ctr[4], (unsigned long long)n_impr);
+  }
+}
+
+// Distance table, inter-group distances and route table after level 1.
+__device__ void item_dist(const GraphDev& g, Slot* sp, uint32_t tile) {
+  const uint64_t V = (uint64_t)g.V;
+  const int k = sp->k, W = sp->W, ND = sp->ND;
+  const bool hop = sp->D >= 0;
+  const uint64_t v0 = (uint64_t)tile * kDistTV;
+  uint32_t star = kInf;
+  for (uint64_t v = v0 + lane_id(); v < v0 + kDistTV && v < V; v += 64) {
+    uint32_t s = 0;
+    for (int i = 0; i < k; i++) {
+      uint32_t c = kInf;
+      for (int d = 0; d < ND; d++) {
+        const uint32_t t = (uint32_t)(sp->T[sidx((uint32_t)v, 1u << i, d, ND, W)] >> 32);
+        c = t < c ? t : c;
+      }
+      sp->C[v * k + i] = c;
+      s = sat(s + c);
+    }
+    star = s < star ? s : star;
+  }
+  if (!hop) {
+    // star: shortest paths from one vertex to every group form a connected
+    // cover, so their sum bounds the optimum from above
+    star = wave_min_u32(star);
+    if (lane_id() == 0 && star < kInf) atomicMin(&sp->best, star);
+  }
+}
+
+__device__ void item_vvr(const GraphDev& g, Slot* sp, WaveLDS* L) {
+  const int lane = lane_id();
+  const int k = sp->k, W = sp->W, ND = sp->ND;
+  uint32_t* vvl = (uint32_t*)L->wv;  // k*k <= 36 words
+  for (int i = 0; i < k; i++) {
+    const int32_t grp = sp->qg[i];
+    const int64_t beg = g.gptr[grp], end = g.gptr[grp + 1];
+    uint32_t loc[kMaxK];
+#pragma unroll
+    for (int j = 0; j < kMaxK; j++) loc[j] = kInf;
+    for (int64_t t = beg + lane; t < end; t += 64) {
+      const uint32_t v = (uint32_t)g.gmem[t];
+#pragma unroll
+      for (int j = 0; j < kMaxK; j++) {
+        if (j >= k) continue;
+        for (int d = 0; d < ND; d++) {
+          const uint32_t c = (uint32_t)(sp->T[sidx(v, 1u << j, d, ND, W)] >> 32);
+          loc[j] = c < loc[j] ? c : loc[j];
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kMaxK; j++) {
+      uint32_t m = wave_min_u32(loc[j]);
+      if (lane == 0 && j < k) vvl[i * k + j] = (i == j) ? 0u : m;
+    }
+  }
+  wave_sync();
+  // route table R[x][i][j]: cheapest sequence of inter-group hops that starts
+  // in group i, visits every group of x, and ends in group j (i != j)
+  uint32_t* Rl = (uint32_t*)L->stage;
+  const int kk = k * k;
+  const int nR = W * kk;
+  for (int t = lane; t < nR; t += 64) Rl[t] = kInf;
+  wave_sync();
+  for (int c = 2; c <= k; c++) {
+    for (int t = lane; t < nR; t += 64) {
+      const uint32_t x = (uint32_t)(t / kk);
+      const int i = (t / k) % k, j = t % k;
+      if (__popc(x) != c || i == j || !((x >> i) & 1u) || !((x >> j) & 1u)) continue;
+      uint32_t best;
+      if (c == 2) {
+        best = vvl[i * k + j];
+      } else {
+        const uint32_t y = x ^ (1u << j);
+        best = kInf;
+        for (int m = 0; m < k; m++) {
+          if (m == i || !((y >> m) & 1u)) continue;
+          const uint32_t cand = sat(Rl[y * kk + i * k + m] + vvl[m * k + j]);
+          best = cand < best ? cand : best;
+        }
+      }
+      Rl[t] = best;
+    }
+    wave_sync();
+  }
+  for (int t = lane; t < nR; t += 64) sp->R[t] = Rl[t];
+  if (lane < kk) sp->vv[lane] = vvl[lane];
+}
+
+// Level-L merge: every |p| = L subset of every vertex from the final lower
+// levels.  One wave lane per subset p (W <= 64), 64 / W vertices per pass,
+// their W * ND segment staged in LDS with one coalesced load.
+__device__ void item_merge(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* L) {
+  const int lane = lane_id();
+  const uint64_t V = (uint64_t)g.V;
+  const int k = sp->k, W = sp->W, ND = sp->ND, D = sp->D;
+  const bool hop = D >= 0;
+  const int Dlim = hop ? D : 0;
+  const int level = sp->level;
+  const uint32_t full = (uint32_t)W - 1;
+  const int nxt = (sp->step + 1) % 3;
+  uint64_t* mnxt = sp->mask + (uint64_t)nxt * V;
+  const int rowlen = W * ND;
+  const int VPW = 64 / W;
+  const int j = lane / W;
+  const uint32_t p = (uint32_t)(lane % W);
+  const uint64_t v0 = (uint64_t)tile * kMergeTV;
+  const uint64_t v1 = v0 + kMergeTV < V ? v0 + kMergeTV : V;
+  const uint32_t bst0 = __hip_atomic_load(&sp->best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  uint32_t ubmin = kInf;
+  uint64_t n_eval = 0;
+  for (uint64_t vb = v0; vb < v1; vb += VPW) {
+    const int nv = (int)((v1 - vb) < (uint64_t)VPW ? (v1 - vb) : (uint64_t)VPW);
+    const uint64_t* src = sp->T + vb * (uint64_t)rowlen;
+    for (int t = lane; t < nv * rowlen; t += 64) L->stage[t] = src[t];
+    wave_sync();
+    const uint32_t v = (uint32_t)(vb + j);
+    const bool act = (j < nv) && p != 0 && __popc(p) == level;
+    bool wrote = false;
+    if (act) {
+      const uint64_t* row = L->stage + j * rowlen;
+      const uint32_t x = full ^ p;
+      const uint32_t low = p & (0u - p);
+      uint32_t lbx = 0xFFFFFFFFu;
+      uint32_t csum = 0;
+      if (!hop) {
+        for (int i = 0; i < k; i++)
+          if ((x >> i) & 1u) csum = sat(csum + sp->C[(uint64_t)v * k + i]);
+      }
+      for (int nd = 0; nd < ND; nd++) {
+        uint64_t bestpk = row[nd * W + p];
+        for (uint32_t sub = (p - 1) & p; sub; sub = (sub - 1) & p) {
+          if (!(sub & low)) continue;
+          const uint32_t sub2 = p ^ sub;
+          // pairs (d1, d2) with max(d1, d2) == nd and d1 + d2 <= Dlim
+          const int lim = Dlim - nd;
+          if (lim < 0) continue;
+          const int dm = nd < lim ? nd : lim;
+          for (int d2 = 0; d2 <= dm; d2++) {  // d1 = nd
+            const uint64_t a = row[nd * W + sub], b = row[d2 * W + sub2];
+            const uint32_t c = sat((uint32_t)(a >> 32) + (uint32_t)(b >> 32));
+            n_eval++;
+            if (c >= kInf) continue;
+            const uint64_t pk = ((uint64_t)c << 32) | (kPredMerge | sub | ((uint32_t)nd << 16) | ((uint32_t)d2 << 23));
+            bestpk = pk < bestpk ? pk : bestpk;
+          }
+          const int dm1 = (nd - 1) < lim ? (nd - 1) : lim;
+          for (int d1 = 0; d1 <= dm1; d1++) {  // d2 = nd, d1 < nd
+            const uint64_t a = row[d1 * W + sub], b = row[nd * W + sub2];
+            const uint32_t c = sat((uint32_t)(a >> 32) + (uint32_t)(b >> 32));
+            n_eval++;
+            if (c >= kInf) continue;
+            const uint64_t pk = ((uint64_t)c << 32) | (kPredMerge | sub | ((uint32_t)d1 << 16) | ((uint32_t)nd << 23));
+            bestpk = pk < bestpk ? pk : bestpk;
+          }
+        }
+        const uint32_t c = (uint32_t)(bestpk >> 32);
+        if (c >= kInf) continue;
+        if (lbx == 0xFFFFFFFFu) lbx = lower_bound(sp->C + (uint64_t)v * k, k, x, sp->R);
+        if (c + lbx > bst0) continue;
+        sp->T[sidx(v, p, nd, ND, W)] = bestpk;
+        wrote = true;
+        // upper bounds from complete walk-trees at v
+        if (!hop) {
+          const uint32_t ub = sat(c + csum);
+          ubmin = ub < ubmin ? ub : ubmin;
+        }
+        if (__popc(x) < level) {
+          const int d2m = Dlim - nd;
+          for (int d2 = 0; d2 <= d2m; d2++) {
+            const uint32_t ub = sat(c + (uint32_t)(row[d2 * W + x] >> 32));
+            ubmin = ub < ubmin ? ub : ubmin;
+          }
+        }
+      }
+    }
+    const uint64_t ball = __ballot(wrote);
+    bool app = false;
+    if (j < nv && p == 0) {
+      const uint64_t segmask = (W == 64) ? ~0ull : ((1ull << W) - 1);
+      const uint64_t m = (ball >> (j * W)) & segmask;
+      if (m) {
+        mnxt[v] = m;
+        app = true;
+      }
+    }
+    wave_append(app, v, sp, nxt, g, V);
+    wave_sync();
+  }
+  ubmin = wave_min_u32(ubmin);
+  n_eval = wave_sum_u64(n_eval);
+  if (lane == 0) {
+    if (ubmin < kInf) atomicMin(&sp->best, ubmin);
+    atomicAdd(&sp->ctr[5], (unsigned long long)n_eval);
+  }
+}
+
+// Full-set merge: W/2 lanes per vertex, one lane per split (sub contains bit 0).
+__device__ void item_final(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* L) {
+  const int lane = lane_id();
+  const uint64_t V = (uint64_t)g.V;
+  const int W = sp->W, ND = sp->ND, D = sp->D;
+  const bool hop = D >= 0;
+  const int Dlim = hop ? D : 0;
+  const uint32_t full = (uint32_t)W - 1;
+  const int rowlen = W * ND;
+  const int half = W / 2;  // W >= 2
+  int VPF = 64 / half;
+  const int cap = kStageWords / rowlen;
+  if (VPF > cap) VPF = cap;
+  const int j = lane / half;
+  const int s = lane % half;
+  const uint32_t sub = 2u * (uint32_t)s + 1u;
+  const uint32_t sub2 = full ^ sub;
+  const uint64_t v0 = (uint64_t)tile * kMergeTV;
+  const uint64_t v1 = v0 + kMergeTV < V ? v0 + kMergeTV : V;
+  uint64_t ansmin = ~0ull;
+  uint64_t n_eval = 0;
+  for (uint64_t vb = v0; vb < v1; vb += VPF) {
+    const int nv = (int)((v1 - vb) < (uint64_t)VPF ? (v1 - vb) : (uint64_t)VPF);
+    const uint64_t* src = sp->T + vb * (uint64_t)rowlen;
+    for (int t = lane; t < nv * rowlen; t += 64) L->stage[t] = src[t];
+    wave_sync();
+    const bool vact = j < nv;
+    const uint32_t v = (uint32_t)(vb + j);
+    const uint64_t* row = L->stage + (vact ? j : 0) * rowlen;
+    const bool sact = vact && sub != full;
+    for (int nd = 0; nd < ND; nd++) {
+      uint64_t bestpk = (vact && s == 0) ? row[nd * W + full] : kInfState;
+      if (sact) {
+        const int lim = Dlim - nd;
+        if (lim >= 0) {
+          const int dm = nd < lim ? nd : lim;
+          for (int d2 = 0; d2 <= dm; d2++) {
+            const uint64_t a = row[nd * W + sub], b = row[d2 * W + sub2];
+            const uint32_t c = sat((uint32_t)(a >> 32) + (uint32_t)(b >> 32));
+            n_eval++;
+            if (c >= kInf) continue;
+            const uint64_t pk = ((uint64_t)c << 32) | (kPredMerge | sub | ((uint32_t)nd << 16) | ((uint32_t)d2 << 23));
+            bestpk = pk < bestpk ? pk : bestpk;
+          }
+          const int dm1 = (nd - 1) < lim ? (nd - 1) : lim;
+          for (int d1 = 0; d1 <= dm1; d1++) {
+            const uint64_t a = row[d1 * W + sub], b = row[nd * W + sub2];
+            const uint32_t c = sat((uint32_t)(a >> 32) + (uint32_t)(b >> 32));
+            n_eval++;
+            if (c >= kInf) continue;
+            const uint64_t pk = ((uint64_t)c << 32) | (kPredMerge | sub | ((uint32_t)d1 << 16) | ((uint32_t)nd << 23));
+            bestpk = pk < bestpk ? pk : bestpk;
+          }
+        }
+      }
+      // min over the half lanes of this vertex (segments are aligned powers of two)
+      for (int o = half / 2; o > 0; o >>= 1) {
+        const uint64_t y = (uint64_t)__shfl_xor((unsigned long long)bestpk, o, 64);
+        bestpk = y < bestpk ? y : bestpk;
+      }
+      if (vact && s == 0 && (uint32_t)(bestpk >> 32) < kInf) {
+        sp->T[sidx(v, full, nd, ND, W)] = bestpk;
+        const uint64_t a = (bestpk & 0xFFFFFFFF00000000ull) | v;
+        ansmin = a < ansmin ? a : ansmin;
+      }
+    }
+    wave_sync();
+  }
+  ansmin = wave_min_u64(ansmin);
+  n_eval = wave_sum_u64(n_eval);
+  if (lane == 0) {
+    if (ansmin != ~0ull) atomicMin((unsigned long long*)&sp->ans, (unsigned long long)ansmin);
+    atomicAdd(&sp->ctr[5], (unsigned long long)n_eval);
+  }
+}
+
+// Predecessor chase from the optimal root: breadth-first over the witness,
+// one lane per state; edges get their parent edge index so that a tree-like
+// walk (hop DP) keeps its shape.
+__device__ void item_recon(const GraphDev& g, Slot* sp) {
+  const int lane = lane_id();
+  const int W = sp->W, ND = sp->ND, D = sp->D;
+  const bool hop = D >= 0;
+  const uint32_t full = (uint32_t)W - 1;
+  const uint64_t ans = sp->ans;
+  const uint32_t cost = (uint32_t)(ans >> 32);
+  if (cost >= kInf) {
+    if (lane == 0) {
+      sp->out_cost = (int32_t)kInf;
+      sp->out_root = -1;
+      sp->out_nedges = 0;
+      sp->out_d = -1;
+    }
+    return;
+  }
+  const uint32_t root = (uint32_t)ans;
+  // smallest depth holding the optimum at the root
+  uint32_t cd = kInf;
+  if (lane < ND) cd = (uint32_t)(sp->T[sidx(root, full, lane, ND, W)] >> 32);
+  const uint64_t hit = __ballot(lane < ND && cd == cost);
+  const int dstar = hit ? (__ffsll((long long)hit) - 1) : 0;
+  const uint32_t qcap = (uint32_t)sp->edge_cap + 2 * kMaxK;
+  uint32_t* qv = sp->rq;
+  uint32_t* qpd = sp->rq + qcap;
+  int32_t* qat = (int32_t*)(sp->rq + 2 * qcap);
+  if (lane == 0) {
+    qv[0] = root;
+    qpd[0] = full | ((uint32_t)dstar << 16);
+    qat[0] = -1;
+  }
+  __threadfence();
+  wave_sync();
+  uint32_t head = 0, tail = 1, ne = 0;
+  bool overflow = false;
+  while (head < tail) {
+    const uint32_t cnt = (tail - head) < 64u ? (tail - head) : 64u;
+    const uint32_t t = head + lane;
+    const bool act = (uint32_t)lane < cnt;
+    uint32_t v = 0, p = 0, d = 0, pred = 0;
+    int32_t att = -1;
+    if (act) {
+      v = qv[t];
+      const uint32_t pd = qpd[t];
+      p = pd & 0xFFFFu;
+      d = pd >> 16;
+      att = qat[t];
+      pred = (uint32_t)sp->T[sidx(v, p, d, ND, W)];
+    }
+    const uint32_t type = pred >> 30;
+    const bool isg = act && type == 1u;
+    const bool ism = act && type == 2u;
+    const uint64_t gb = __ballot(isg);
+    const uint32_t eidx = ne + __popcll(gb & __lanemask_lt());
+    uint32_t c1v = 0, c1pd = 0, c2v = 0, c2pd = 0;
+    int32_t c1a = att;
+    if (isg) {
+      const uint32_t ea = pred & kArcMask;
+      const uint32_t u = (uint32_t)g.col[ea];
+      const int32_t w = g.wt[ea];
+      if (eidx < (uint32_t)sp->edge_cap) {
+        EdgeDev ed;
+        ed.u = (int32_t)v;
+        ed.v = (int32_t)u;
+        ed.w = w;
+        ed.parent = att;
+        sp->edges[eidx] = ed;
+      }
+      c1v = u;
+      c1pd = p | ((hop ? d - 1 : 0u) << 16);
+      c1a = (int32_t)eidx;
+    } else if (ism) {
+      const uint32_t s1 = pred & 0xFFFFu;
+      const uint32_t d1 = (pred >> 16) & 0x7Fu, d2 = (pred >> 23) & 0x7Fu;
+      c1v = v;
+      c1pd = s1 | (d1 << 16);
+      c2v = v;
+      c2pd = (p ^ s1) | (d2 << 16);
+    }
+    const uint32_t nch = isg ? 1u : (ism ? 2u : 0u);
+    const uint32_t off = wave_excl_scan(nch);
+    const uint32_t tot = __shfl(off + nch, 63, 64);
+    if (tail + tot > qcap) {
+      overflow = true;
+      break;
+    }
+    if (nch >= 1) {
+      qv[tail + off] = c1v;
+      qpd[tail + off] = c1pd;
+      qat[tail + off] = c1a;
+    }
+    if (nch == 2) {
+      qv[tail + off + 1] = c2v;
+      qpd[tail + off + 1] = c2pd;
+      qat[tail + off + 1] = att;
+    }
+    ne += __popcll(gb);
+    head += cnt;
+    tail += tot;
+    __threadfence();
+    wave_sync();
+  }
+  if (lane == 0) {
+    sp->out_cost = (int32_t)cost;
+    sp->out_root = (int32_t)root;
+    sp->out_nedges = (int32_t)ne;
+    sp->out_d = dstar;
+    if (overflow || ne > (uint32_t)sp->edge_cap) sp->status = 1;  // edge capacity exceeded
+  }
+}
+
+// -------------------------------------------------------- state machine ----
+
+__device__ uint32_t ceil_div(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+__device__ void advance_level(Slot* s) {
+  if (s->level == 1) {
+    s->phase = PH_DIST;
+  } else if (s->level + 1 < s->k) {
+    s->phase = PH_MERGE;
+    s->level++;
+  } else {
+    s->phase = PH_FINAL;
+  }
+}
+
+__device__ void complete_step(Slot* s, Ctrl* ctrl) {
+  const uint32_t r = s->step;
+  switch (s->phase) {
+    case PH_FILL:
+      s->phase = PH_SEED;
+      break;
+    case PH_SEED:
+      if (s->k == 1) {
+        s->phase = PH_FINAL;
+      } else {
+        s->phase = PH_ROUND;
+        s->level = 1;
+      }
+      break;
+    case PH_ROUND:
+      s->ctr[1] += (unsigned long long)(s->app[r % 3] >> 32);
+      s->ctr[2] += (unsigned long long)(uint32_t)s->app[r % 3];
+      if ((s->app[(r + 1) % 3] >> 32) == 0) advance_level(s);
+      break;
+    case PH_DIST:
+      if (s->k == 2) {
+        s->phase = PH_FINAL;
+      } else {
+        s->phase = PH_MERGE;
+        s->level = 2;
+      }
+      break;
+    case PH_MERGE:
+      if ((s->app[(r + 1) % 3] >> 32) == 0) advance_level(s);
+      else s->phase = PH_ROUND;
+      break;
+    case PH_FINAL:
+      s->phase = PH_RECON;
+      break;
+    case PH_RECON:
+      s->phase = PH_DONE;
+      atomicAdd(&ctrl->finished, 1u);
+      break;
+    default:
+      break;
+  }
+  s->app[(r + 2) % 3] = 0;
+  s->step = r + 1;
+  s->ctr[0]++;
+}
+
+__device__ uint32_t plan_step(Slot* s, uint64_t V) {
+  const uint32_t r = s->step;
+  uint32_t work = 0;
+  switch (s->phase) {
+    case PH_FILL: {
+      const uint64_t total = s->nT + 3 * V + ((uint64_t)V * s->k + 1) / 2;
+      work = ceil_div(total, kFillWords);
+      break;
+    }
+    case PH_SEED: work = (uint32_t)s->k; break;
+    case PH_ROUND: work = ceil_div((uint32_t)s->app[r % 3], (uint32_t)s->chunk); break;
+    case PH_DIST: work = ceil_div(V, kDistTV) + 1; break;
+    case PH_MERGE:
+    case PH_FINAL: work = ceil_div(V, kMergeTV); break;
+    case PH_RECON: work = 1; break;
+    default: work = 0; break;
+  }
+  const uint32_t nclr = ceil_div((uint32_t)(s->app[(r + 2) % 3] >> 32), kClearPer);
+  s->n_work = work;
+  s->n_items = work + nclr;
+  return s->n_items;
+}
+
+// Run by the last block to reach the grid barrier.
+__device__ void phase_b(Slot* slots, Ctrl* ctrl, uint64_t V, uint32_t* sh_items, uint32_t* sh_scan) {
+  const int tid = threadIdx.x;
+  const int nslots = (int)ctrl->nslots;
+  for (int i = tid; i < nslots; i += kBlock) {
+    Slot* s = &slots[i];
+    if (s->in_step) {
+      complete_step(s, ctrl);
+      s->in_step = 0;
+    }
+  }
+  __threadfence();
+  __syncthreads();
+  if (tid == 0) {
+    const uint32_t steps = ++ctrl->steps;  // only this block touches it now
+    const uint32_t fin = __hip_atomic_load(&ctrl->finished, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    sh_scan[0] = (fin >= ctrl->exit_after || steps >= ctrl->max_steps) ? 1u : 0u;
+  }
+  __syncthreads();
+  const bool leave = sh_scan[0] != 0;
+  __syncthreads();
+  for (int i = tid; i < nslots; i += kBlock) {
+    Slot* s = &slots[i];
+    uint32_t items = 0;
+    if (!leave && s->phase != PH_FREE && s->phase != PH_DONE) {
+      items = plan_step(s, V);
+      s->in_step = items ? 1u : 0u;
+    }
+    sh_items[i] = items;
+  }
+  __syncthreads();
+  // exclusive scan over slots with work, compacting them into the active list
+  const int per = (nslots + kBlock - 1) / kBlock;
+  const int b0 = tid * per;
+  const int b1 = (b0 + per) < nslots ? (b0 + per) : nslots;
+  uint32_t cnt = 0, act = 0;
+  for (int i = b0; i < b1; i++) {
+    cnt += sh_items[i];
+    act += sh_items[i] ? 1u : 0u;
+  }
+  sh_scan[tid] = cnt;
+  sh_scan[kBlock + tid] = act;
+  __syncthreads();
+  if (tid == 0) {
+    uint32_t c = 0, a = 0;
+    for (int t = 0; t < kBlock; t++) {
+      const uint32_t x = sh_scan[t], y = sh_scan[kBlock + t];
+      sh_scan[t] = c;
+      sh_scan[kBlock + t] = a;
+      c += x;
+      a += y;
+    }
+    ctrl->item_total = c;
+    ctrl->n_active = a;
+    ctrl->prefix[a] = c;
+    ctrl->item_next = 0;
+    ctrl->exit_flag = (leave || a == 0) ? 1u : 0u;
+  }
+  __syncthreads();
+  uint32_t c = sh_scan[tid], a = sh_scan[kBlock + tid];
+  for (int i = b0; i < b1; i++) {
+    if (sh_items[i]) {
+      ctrl->active[a] = (uint32_t)i;
+      ctrl->prefix[a] = c;
+      a++;
+      c += sh_items[i];
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void gst_engine_kernel(GraphDev g, Slot* slots, Ctrl* ctrl,
+                                                            uint64_t timeout_ticks) {
+  __shared__ WaveLDS lds[kWaves];
+  __shared__ uint32_t sh_items[kMaxSlots];
+  __shared__ uint32_t sh_scan[2 * kBlock];
+  __shared__ uint32_t sh_flag;
+  const int wid = threadIdx.x >> 6;
+  const int lane = lane_id();
+  WaveLDS* L = &lds[wid];
+  const uint64_t V = (uint64_t)g.V;
+  uint32_t my_gen = __hip_atomic_load(&ctrl->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+
+  for (;;) {
+    // ---------------- grid barrier; the last arriving block runs phase B
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      __threadfence();
+      const uint32_t a = atomicAdd(&ctrl->arrive, 1u);
+      sh_flag = (a == gridDim.x - 1) ? 1u : 0u;
+    }
+    __syncthreads();
+    if (sh_flag) {
+      __threadfence();
+      phase_b(slots, ctrl, V, sh_items, sh_scan);
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        ctrl->arrive = 0;
+        __threadfence();
+        __hip_atomic_store(&ctrl->gen, my_gen + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    } else if (threadIdx.x == 0) {
+      const uint64_t t0 = wall_clock64();
+      while (__hip_atomic_load(&ctrl->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == my_gen) {
+        __builtin_amdgcn_s_sleep(1);
+        if (__hip_atomic_load(&ctrl->abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+        if (wall_clock64() - t0 > timeout_ticks) {
+          atomicExch(&ctrl->abort_flag, 1u);
+          break;
+        }
+      }
+    }
+    my_gen++;
+    __syncthreads();
+    __threadfence();
+    if (__hip_atomic_load(&ctrl->abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+    if (ctrl->exit_flag) break;
+
+    // ---------------- phase A: pull work items
+    const uint32_t total = ctrl->item_total;
+    const uint32_t nact = ctrl->n_active;
+    for (;;) {
+      uint32_t it = 0;
+      if (lane == 0) it = atomicAdd(&ctrl->item_next, 1u);
+      it = __shfl(it, 0, 64);
+      if (it >= total) break;
+      uint32_t lo = 0, hi = nact;  // last active slot whose prefix <= it
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (ctrl->prefix[mid] <= it) lo = mid; else hi = mid;
+      }
+      Slot* sp = &slots[ctrl->active[lo]];
+      const uint32_t local = it - ctrl->prefix[lo];
+      const uint32_t nwork = sp->n_work;
+      if (local >= nwork) {
+        item_clear(sp, local - nwork, V);
+        continue;
+      }
+      switch (sp->phase) {
+        case PH_FILL: item_fill(sp, local, V); break;
+        case PH_SEED: item_seed(g, sp, local); break;
+        case PH_ROUND: item_round(g, sp, local, L); break;
+        case PH_DIST:
+          if (local + 1 == nwork) item_vvr(g, sp, L);
+          else item_dist(g, sp, local);
+          break;
+        case PH_MERGE: item_merge(g, sp, local, L); break;
+        case PH_FINAL: item_final(g, sp, local, L); break;
+        case PH_RECON: item_recon(g, sp); break;
+        default: break;
+      }
+    }
+  }
+}
+
+}  // namespace gst
+
+extern "C" hipError_t gst_launch_engine(const gst::GraphDev* g, gst::Slot* slots, gst::Ctrl* ctrl,
+                                        uint64_t timeout_ticks, int grid, int cooperative,
+                                        hipStream_t stream) {
+  gst::GraphDev gd = *g;
+  if (cooperative) {
+    void* args[] = {(void*)&gd, (void*)&slots, (void*)&ctrl, (void*)&timeout_ticks};
+    return hipLaunchCooperativeKernel((const void*)gst::gst_engine_kernel, dim3(grid), dim3(gst::kBlock),
+                                      args, 0, stream);
+  }
+  hipLaunchKernelGGL(gst::gst_engine_kernel, dim3(grid), dim3(gst::kBlock), 0, stream, gd, slots, ctrl,
+                     timeout_ticks);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t gst_engine_occupancy(int* blocks_per_cu) {
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, (const void*)gst::gst_engine_kernel,
+                                                       gst::kBlock, 0);
+}

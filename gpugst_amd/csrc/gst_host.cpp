@@ -1,0 +1,947 @@
+// gst_host.cpp -- host side of libgst: graph residency, the slot engine that
+// drives the persistent kernel, memory admission, batch and server paths, and
+// the C ABI declared in include/gst.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "gst.h"
+#include "gst_common.h"
+
+extern "C" hipError_t gst_launch_engine(const gst::GraphDev* g, gst::Slot* slots, gst::Ctrl* ctrl,
+                                        uint64_t timeout_ticks, int grid, int cooperative,
+                                        hipStream_t stream);
+extern "C" hipError_t gst_engine_occupancy(int* blocks_per_cu);
+
+namespace {
+
+thread_local std::string g_last_error;
+
+gst_status fail(gst_status st, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return st;
+}
+
+double now_ms() {
+  using namespace std::chrono;
+  return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+#define HIP_TRY(expr)                                                                          \
+  do {                                                                                         \
+    hipError_t e_ = (expr);                                                                    \
+    if (e_ != hipSuccess)                                                                      \
+      return fail(GST_DEVICE_ERROR, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                  __FILE__, __LINE__);                                                         \
+  } while (0)
+
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+// First-fit allocator over one device region, with coalescing of free runs.
+class Arena {
+ public:
+  void reset(size_t size) {
+    free_.clear();
+    size_ = size;
+    if (size) free_[0] = size;
+  }
+  bool alloc(size_t len, size_t* off) {
+    len = align256(len);
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second >= len) {
+        *off = it->first;
+        const size_t rest = it->second - len, noff = it->first + len;
+        free_.erase(it);
+        if (rest) free_[noff] = rest;
+        return true;
+      }
+    }
+    return false;
+  }
+  void release(size_t off, size_t len) {
+    len = align256(len);
+    auto it = free_.emplace(off, len).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+  size_t largest() const {
+    size_t m = 0;
+    for (auto& kv : free_) m = std::max(m, kv.second);
+    return m;
+  }
+  size_t size() const { return size_; }
+
+ private:
+  std::map<size_t, size_t> free_;
+  size_t size_ = 0;
+};
+
+struct Options {
+  int64_t max_slots = 256;
+  int64_t budget_mb = 0;  // 0: most of the free device memory
+  int64_t blocks_per_cu = 0;  // 0: occupancy maximum
+  int64_t chunk = 512;
+  int64_t edge_cap = 65536;
+  int64_t timeout_ms = 10000;
+  int64_t cooperative = 1;
+};
+
+}  // namespace
+
+struct gst_graph {
+  int device = 0;
+  int num_cu = 0;
+  int occ = 1;
+  int64_t wall_khz = 100000;
+  int64_t V = 0, n_arcs = 0;
+  int64_t n_groups = 0;
+  std::vector<int64_t> h_gptr;
+  int64_t* d_rowptr = nullptr;
+  int32_t* d_col = nullptr;
+  int32_t* d_wt = nullptr;
+  int32_t* d_rev = nullptr;
+  int64_t* d_gptr = nullptr;
+  int32_t* d_gmem = nullptr;
+  Options opt;
+  std::mutex dev_mu;  // one persistent engine kernel at a time
+  std::unique_ptr<class Engine> plain;
+  std::unique_ptr<class Engine> batch;
+  gst::GraphDev dev() const {
+    gst::GraphDev g;
+    g.rowptr = d_rowptr;
+    g.col = d_col;
+    g.wt = d_wt;
+    g.rev = d_rev;
+    g.gptr = d_gptr;
+    g.gmem = d_gmem;
+    g.V = V;
+    g.n_arcs = n_arcs;
+    return g;
+  }
+};
+
+namespace {
+
+struct QueryReq {
+  std::vector<int32_t> groups;
+  int32_t D = -1;
+  uint64_t tag = 0;
+  double t_submit = 0;
+};
+
+struct Layout {
+  size_t T, mask, C, list, loff, R, edges, rq, total;
+  int edge_cap;
+};
+
+Layout layout_for(const gst_graph* g, int k, int D) {
+  const size_t V = (size_t)g->V;
+  const size_t W = size_t(1) << k;
+  const size_t ND = D < 0 ? 1 : (size_t)D + 1;
+  Layout L;
+  int64_t cap = std::min<int64_t>((int64_t)(V * ND), g->opt.edge_cap);
+  L.edge_cap = (int)std::max<int64_t>(cap, 64);
+  size_t off = 0;
+  L.T = off;    off += align256(V * ND * W * 8);
+  L.mask = off; off += align256(3 * V * 8);
+  L.C = off;    off += align256(((V * k + 1) / 2) * 8);
+  L.list = off; off += align256(3 * V * 4);
+  L.loff = off; off += align256(3 * V * 4);
+  L.R = off;    off += align256(W * k * k * 4);
+  L.edges = off; off += align256((size_t)L.edge_cap * sizeof(gst::EdgeDev));
+  L.rq = off;   off += align256(3 * ((size_t)L.edge_cap + 2 * gst::kMaxK) * 4);
+  L.total = off;
+  return L;
+}
+
+gst_status validate_query(const gst_graph* g, const int32_t* groups, int32_t k, int32_t D) {
+  if (!g) return fail(GST_INVALID_ARG, "null graph");
+  if (g->n_groups <= 0) return fail(GST_INVALID_ARG, "no groups set (call gst_groups_set)");
+  if (k < 1 || k > GST_MAX_K) return fail(GST_INVALID_ARG, "k=%d outside [1,%d]", k, GST_MAX_K);
+  if (D < GST_UNBOUNDED || D > GST_MAX_D)
+    return fail(GST_INVALID_ARG, "D=%d outside [-1,%d]", D, GST_MAX_D);
+  if (!groups) return fail(GST_INVALID_ARG, "null group id list");
+  for (int i = 0; i < k; i++)
+    if (groups[i] < 0 || groups[i] >= g->n_groups)
+      return fail(GST_INVALID_ARG, "group id %d outside [0,%lld)", groups[i], (long long)g->n_groups);
+  return GST_OK;
+}
+
+void fill_infeasible(gst_result* r) {
+  memset(r, 0, sizeof(*r));
+  r->cost = GST_INFEASIBLE;
+  r->root = -1;
+  r->status = GST_OK;
+}
+
+struct SlotHost {
+  bool used = false;
+  size_t off = 0, len = 0;
+  uint64_t tag = 0;
+  double t_submit = 0, t_admit = 0;
+};
+
+}  // namespace
+
+class Engine {
+ public:
+  Engine(gst_graph* g, int max_slots) : g_(g), max_slots_(std::min(max_slots, gst::kMaxSlots)) {}
+  ~Engine() {
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    if (arena_mem_) (void)hipFree(arena_mem_);
+    if (d_slots_) (void)hipFree(d_slots_);
+    if (d_ctrl_) (void)hipFree(d_ctrl_);
+    if (h_slots_) (void)hipHostFree(h_slots_);
+    if (h_ctrl_) (void)hipHostFree(h_ctrl_);
+    if (stream_) (void)hipStreamDestroy(stream_);
+  }
+
+  gst_status init(size_t arena_bytes) {
+    HIP_TRY(hipSetDevice(g_->device));
+    HIP_TRY(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    HIP_TRY(hipMalloc(&d_slots_, sizeof(gst::Slot) * max_slots_));
+    HIP_TRY(hipMalloc(&d_ctrl_, sizeof(gst::Ctrl)));
+    HIP_TRY(hipHostMalloc(&h_slots_, sizeof(gst::Slot) * max_slots_, hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc(&h_ctrl_, sizeof(gst::Ctrl), hipHostMallocDefault));
+    memset(h_slots_, 0, sizeof(gst::Slot) * max_slots_);
+    memset(h_ctrl_, 0, sizeof(gst::Ctrl));
+    HIP_TRY(hipMemset(d_ctrl_, 0, sizeof(gst::Ctrl)));
+    sh_.assign(max_slots_, SlotHost());
+    return resize_arena(arena_bytes);
+  }
+
+  gst_status resize_arena(size_t bytes) {
+    if (active_ > 0) return fail(GST_INVALID_ARG, "arena resize with active queries");
+    if (arena_mem_) {
+      HIP_TRY(hipFree(arena_mem_));
+      arena_mem_ = nullptr;
+    }
+    bytes = align256(bytes);
+    if (bytes) {
+      hipError_t e = hipMalloc((void**)&arena_mem_, bytes);
+      if (e != hipSuccess) {
+        arena_mem_ = nullptr;
+        arena_.reset(0);
+        return fail(GST_OUT_OF_MEMORY, "hipMalloc(%zu) for the state arena failed: %s", bytes,
+                    hipGetErrorString(e));
+      }
+    }
+    arena_.reset(bytes);
+    return GST_OK;
+  }
+
+  size_t arena_size() const { return arena_.size(); }
+  int active() const { return active_; }
+  bool has_free_slot() const { return active_ < max_slots_; }
+
+  // Returns the slot index, -1 when memory or slots are exhausted now.
+  int admit(const QueryReq& q) {
+    if (active_ >= max_slots_) return -1;
+    const int k = (int)q.groups.size();
+    const Layout L = layout_for(g_, k, q.D);
+    size_t off = 0;
+    if (!arena_.alloc(L.total, &off)) return -1;
+    int s = -1;
+    for (int i = 0; i < max_slots_; i++)
+      if (!sh_[i].used) {
+        s = i;
+        break;
+      }
+    SlotHost& h = sh_[s];
+    h.used = true;
+    h.off = off;
+    h.len = L.total;
+    h.tag = q.tag;
+    h.t_submit = q.t_submit;
+    h.t_admit = now_ms();
+    gst::Slot& d = h_slots_[s];
+    memset(&d, 0, sizeof(d));
+    d.k = k;
+    d.W = 1 << k;
+    d.ND = q.D < 0 ? 1 : q.D + 1;
+    d.D = q.D;
+    for (int i = 0; i < k; i++) d.qg[i] = q.groups[i];
+    d.edge_cap = L.edge_cap;
+    d.chunk = (int32_t)g_->opt.chunk;
+    d.nT = (uint64_t)g_->V * d.ND * d.W;
+    char* base = arena_mem_ + off;
+    d.T = (uint64_t*)(base + L.T);
+    d.mask = (uint64_t*)(base + L.mask);
+    d.C = (uint32_t*)(base + L.C);
+    d.list = (uint32_t*)(base + L.list);
+    d.loff = (uint32_t*)(base + L.loff);
+    d.R = (uint32_t*)(base + L.R);
+    d.edges = (gst::EdgeDev*)(base + L.edges);
+    d.rq = (uint32_t*)(base + L.rq);
+    d.phase = gst::PH_FILL;
+    d.level = 0;
+    d.step = 0;
+    d.in_step = 0;
+    d.best = gst::kInf;
+    d.ans = ~0ull;
+    d.out_cost = (int32_t)gst::kInf;
+    d.out_root = -1;
+    nslots_used_ = std::max(nslots_used_, s + 1);
+    active_++;
+    return s;
+  }
+
+  // Runs the engine kernel until `exit_after` queries finish (or max_steps
+  // global steps pass); returns the finished slot indices.
+  gst_status run_epoch(uint32_t exit_after, uint32_t max_steps, std::vector<int>* finished) {
+    finished->clear();
+    if (nslots_used_ == 0) return GST_OK;
+    const size_t slot_bytes = sizeof(gst::Slot) * nslots_used_;
+    HIP_TRY(hipMemcpyAsync(d_slots_, h_slots_, slot_bytes, hipMemcpyHostToDevice, stream_));
+    gst::Ctrl& c = *h_ctrl_;
+    c.arrive = 0;
+    c.item_next = 0;
+    c.item_total = 0;
+    c.n_active = 0;
+    c.exit_flag = 0;
+    c.abort_flag = 0;
+    c.finished = 0;
+    c.steps = 0;
+    c.exit_after = std::max(1u, exit_after);
+    c.max_steps = max_steps;
+    c.nslots = (uint32_t)nslots_used_;
+    // gen keeps the value read back after the previous epoch
+    const size_t hdr = offsetof(gst::Ctrl, active);
+    HIP_TRY(hipMemcpyAsync(d_ctrl_, h_ctrl_, hdr, hipMemcpyHostToDevice, stream_));
+    const uint64_t ticks = (uint64_t)g_->wall_khz * (uint64_t)g_->opt.timeout_ms;
+    const gst::GraphDev gd = g_->dev();
+    int grid = grid_size();
+    hipError_t le = gst_launch_engine(&gd, d_slots_, d_ctrl_, ticks, grid, (int)g_->opt.cooperative, stream_);
+    if (le != hipSuccess && g_->opt.cooperative) {
+      (void)hipGetLastError();
+      g_->opt.cooperative = 0;
+      le = gst_launch_engine(&gd, d_slots_, d_ctrl_, ticks, grid, 0, stream_);
+    }
+    if (le != hipSuccess) return fail(GST_DEVICE_ERROR, "engine launch failed: %s", hipGetErrorString(le));
+    HIP_TRY(hipMemcpyAsync(h_slots_, d_slots_, slot_bytes, hipMemcpyDeviceToHost, stream_));
+    HIP_TRY(hipMemcpyAsync(h_ctrl_, d_ctrl_, hdr, hipMemcpyDeviceToHost, stream_));
+    HIP_TRY(hipStreamSynchronize(stream_));
+    last_steps_ = h_ctrl_->steps;
+    if (h_ctrl_->abort_flag)
+      return fail(GST_DEVICE_ERROR, "engine grid barrier timed out (grid %d blocks)", grid);
+    for (int i = 0; i < nslots_used_; i++)
+      if (sh_[i].used && h_slots_[i].phase == gst::PH_DONE) finished->push_back(i);
+    return GST_OK;
+  }
+
+  // Copies the result of a finished slot out and releases its memory.
+  gst_status harvest(int s, gst_result* out, uint64_t* tag) {
+    const gst::Slot& d = h_slots_[s];
+    SlotHost& h = sh_[s];
+    memset(out, 0, sizeof(*out));
+    const double t = now_ms();
+    out->latency_ms = t - h.t_submit;
+    out->device_ms = t - h.t_admit;
+    out->cost = d.out_cost;
+    out->root = d.out_root;
+    out->status = GST_OK;
+    out->counters.steps = (int64_t)d.ctr[0];
+    out->counters.frontier_entries = (int64_t)d.ctr[1];
+    out->counters.arcs = (int64_t)d.ctr[2];
+    out->counters.relaxations = (int64_t)d.ctr[3];
+    out->counters.improvements = (int64_t)d.ctr[4];
+    out->counters.merge_evals = (int64_t)d.ctr[5];
+    gst_status st = GST_OK;
+    if (d.status != 0) {
+      out->status = GST_DEVICE_ERROR;
+      st = fail(GST_DEVICE_ERROR, "tree of the query exceeds the edge capacity %d", d.edge_cap);
+      out->n_edges = 0;
+    } else if (d.out_cost >= (int32_t)gst::kInf) {
+      out->cost = GST_INFEASIBLE;
+      out->root = -1;
+      out->n_edges = 0;
+    } else {
+      out->n_edges = d.out_nedges;
+      if (d.out_nedges > 0) {
+        out->edges = (gst_edge*)malloc(sizeof(gst_edge) * (size_t)d.out_nedges);
+        if (!out->edges) return fail(GST_OUT_OF_MEMORY, "host edge buffer");
+        hipError_t e = hipMemcpyAsync(out->edges, d.edges, sizeof(gst_edge) * (size_t)d.out_nedges,
+                                      hipMemcpyDeviceToHost, stream_);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+        if (e != hipSuccess) {
+          out->status = GST_DEVICE_ERROR;
+          st = fail(GST_DEVICE_ERROR, "edge copy failed: %s", hipGetErrorString(e));
+        }
+      }
+    }
+    *tag = h.tag;
+    arena_.release(h.off, h.len);
+    h.used = false;
+    memset(&h_slots_[s], 0, sizeof(gst::Slot));  // PH_FREE
+    active_--;
+    while (nslots_used_ > 0 && !sh_[nslots_used_ - 1].used) nslots_used_--;
+    return st;
+  }
+
+  uint32_t last_steps() const { return last_steps_; }
+  size_t largest_free() const { return arena_.largest(); }
+
+  int grid_size() const {
+    int bpc = g_->occ;
+    if (g_->opt.blocks_per_cu > 0) bpc = std::min<int>((int)g_->opt.blocks_per_cu, g_->occ);
+    return std::max(1, bpc) * g_->num_cu;
+  }
+
+ private:
+  gst_graph* g_;
+  int max_slots_;
+  hipStream_t stream_ = nullptr;
+  gst::Slot* d_slots_ = nullptr;
+  gst::Slot* h_slots_ = nullptr;
+  gst::Ctrl* d_ctrl_ = nullptr;
+  gst::Ctrl* h_ctrl_ = nullptr;
+  char* arena_mem_ = nullptr;
+  Arena arena_;
+  std::vector<SlotHost> sh_;
+  int nslots_used_ = 0;
+  int active_ = 0;
+  uint32_t last_steps_ = 0;
+};
+
+namespace {
+
+// Parallel loop over [0, n) on up to hardware_concurrency threads.
+template <class F>
+void parallel_for(int64_t n, F f) {
+  int nt = (int)std::min<int64_t>(std::max(1u, std::thread::hardware_concurrency()), 16);
+  if (n < 100000) nt = 1;
+  std::vector<std::thread> th;
+  const int64_t per = (n + nt - 1) / nt;
+  for (int t = 0; t < nt; t++) {
+    const int64_t b = t * per, e = std::min(n, b + per);
+    if (b >= e) break;
+    th.emplace_back([=]() { f(b, e); });
+  }
+  for (auto& x : th) x.join();
+}
+
+gst_status engine_for(gst_graph* g, bool batch, size_t need, Engine** out) {
+  std::unique_ptr<Engine>& e = batch ? g->batch : g->plain;
+  if (!e) {
+    int slots = batch ? (int)g->opt.max_slots : 1;
+    std::unique_ptr<Engine> ne(new Engine(g, slots));
+    size_t bytes = need;
+    if (batch) {
+      size_t fr = 0, tot = 0;
+      if (hipMemGetInfo(&fr, &tot) != hipSuccess) return fail(GST_DEVICE_ERROR, "hipMemGetInfo failed");
+      bytes = g->opt.budget_mb > 0 ? (size_t)g->opt.budget_mb << 20 : (size_t)(fr * 0.85);
+      bytes = std::min(bytes, (size_t)(fr * 0.95));
+    }
+    gst_status st = ne->init(bytes);
+    if (st != GST_OK) return st;
+    e = std::move(ne);
+  } else if (!batch && e->arena_size() < need) {
+    gst_status st = e->resize_arena(need);
+    if (st != GST_OK) return st;
+  }
+  *out = e.get();
+  return GST_OK;
+}
+
+// Drives an engine over a list of queries: admits while memory and slots
+// allow, runs epochs, harvests finished slots, refills.
+gst_status run_queries(gst_graph* g, Engine* eng, std::vector<QueryReq>& qs, gst_result* out) {
+  std::deque<size_t> pending;
+  for (size_t i = 0; i < qs.size(); i++) pending.push_back(i);
+  std::vector<int> fin;
+  gst_status first_err = GST_OK;
+  while (!pending.empty() || eng->active() > 0) {
+    while (!pending.empty()) {
+      QueryReq& q = qs[pending.front()];
+      q.tag = pending.front();
+      if (eng->admit(q) < 0) {
+        if (eng->active() == 0)
+          return fail(GST_OUT_OF_MEMORY, "query of k=%zu D=%d does not fit the state arena",
+                      q.groups.size(), q.D);
+        break;
+      }
+      pending.pop_front();
+    }
+    const int act = eng->active();
+    const uint32_t exit_after = pending.empty() ? (uint32_t)act : std::max(1u, (uint32_t)act / 8);
+    gst_status st = eng->run_epoch(exit_after, 0xFFFFFFFFu, &fin);
+    if (st != GST_OK) return st;
+    for (int s : fin) {
+      uint64_t tag = 0;
+      gst_result r;
+      gst_status hs = eng->harvest(s, &r, &tag);
+      out[tag] = r;
+      if (hs != GST_OK && first_err == GST_OK) first_err = hs;
+    }
+  }
+  return first_err;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------- C ABI
+
+extern "C" {
+
+const char* gst_last_error(void) { return g_last_error.c_str(); }
+const char* gst_version(void) { return "gpugst-amd 0.1 (gfx950)"; }
+
+gst_status gst_graph_create(const int64_t* beg_pos, int64_t V, const int32_t* adj, const int32_t* w,
+                            int64_t n_arcs, gst_graph** out) {
+  if (!out) return fail(GST_INVALID_ARG, "null output handle");
+  *out = nullptr;
+  if (!beg_pos || V < 1 || V >= (int64_t(1) << 31) - 1) return fail(GST_INVALID_ARG, "bad V=%lld", (long long)V);
+  if (n_arcs < 0 || n_arcs >= (int64_t(1) << 30)) return fail(GST_INVALID_ARG, "bad n_arcs=%lld", (long long)n_arcs);
+  if (n_arcs > 0 && (!adj || !w)) return fail(GST_INVALID_ARG, "null adjacency or weights");
+  if (beg_pos[0] != 0 || beg_pos[V] != n_arcs)
+    return fail(GST_INVALID_ARG, "beg_pos must start at 0 and end at n_arcs");
+  for (int64_t v = 0; v < V; v++)
+    if (beg_pos[v + 1] < beg_pos[v]) return fail(GST_INVALID_ARG, "beg_pos decreases at %lld", (long long)v);
+  std::atomic<int64_t> bad_col{-1}, bad_w{-1};
+  parallel_for(n_arcs, [&](int64_t b, int64_t e) {
+    for (int64_t i = b; i < e; i++) {
+      if (adj[i] < 0 || adj[i] >= V) bad_col = i;
+      if (w[i] < 1 || w[i] > (1 << 20)) bad_w = i;
+    }
+  });
+  if (bad_col >= 0) return fail(GST_INVALID_ARG, "adjacency entry %lld out of range", (long long)bad_col.load());
+  if (bad_w >= 0) return fail(GST_INVALID_ARG, "weight of arc %lld outside [1, 2^20]", (long long)bad_w.load());
+
+  // rows sorted by (neighbour, weight), so the reverse of every arc is found
+  // by binary search
+  std::vector<int32_t> col(adj, adj + n_arcs), wt(w, w + n_arcs);
+  parallel_for(V, [&](int64_t b, int64_t e) {
+    std::vector<std::pair<int32_t, int32_t>> tmp;
+    for (int64_t v = b; v < e; v++) {
+      const int64_t r0 = beg_pos[v], r1 = beg_pos[v + 1];
+      bool sorted = true;
+      for (int64_t i = r0 + 1; i < r1 && sorted; i++)
+        if (col[i] < col[i - 1] || (col[i] == col[i - 1] && wt[i] < wt[i - 1])) sorted = false;
+      if (sorted) continue;
+      tmp.clear();
+      for (int64_t i = r0; i < r1; i++) tmp.emplace_back(col[i], wt[i]);
+      std::sort(tmp.begin(), tmp.end());
+      for (int64_t i = r0; i < r1; i++) {
+        col[i] = tmp[i - r0].first;
+        wt[i] = tmp[i - r0].second;
+      }
+    }
+  });
+  std::vector<int32_t> rev(n_arcs);
+  std::atomic<int64_t> asym{-1};
+  parallel_for(V, [&](int64_t b, int64_t e) {
+    for (int64_t u = b; u < e; u++) {
+      for (int64_t i = beg_pos[u]; i < beg_pos[u + 1]; i++) {
+        const int32_t v = col[i], ww = wt[i];
+        const int32_t* lo = col.data() + beg_pos[v];
+        const int32_t* hi = col.data() + beg_pos[v + 1];
+        const int32_t* it = std::lower_bound(lo, hi, (int32_t)u);
+        int64_t j = it - col.data();
+        while (j < beg_pos[v + 1] && col[j] == u && wt[j] < ww) j++;
+        if (j >= beg_pos[v + 1] || col[j] != u || wt[j] != ww) {
+          asym = i;
+          continue;
+        }
+        rev[i] = (int32_t)j;
+      }
+    }
+  });
+  if (asym >= 0)
+    return fail(GST_INVALID_ARG, "arc %lld has no reverse arc of equal weight (graph must be undirected)",
+                (long long)asym.load());
+
+  std::unique_ptr<gst_graph> g(new gst_graph());
+  HIP_TRY(hipGetDevice(&g->device));
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, g->device));
+  g->num_cu = prop.multiProcessorCount;
+  int wr = 0;
+  if (hipDeviceGetAttribute(&wr, hipDeviceAttributeWallClockRate, g->device) == hipSuccess && wr > 0)
+    g->wall_khz = wr;
+  int occ = 1;
+  if (gst_engine_occupancy(&occ) == hipSuccess && occ > 0) g->occ = occ;
+  g->V = V;
+  g->n_arcs = n_arcs;
+  HIP_TRY(hipMalloc(&g->d_rowptr, sizeof(int64_t) * (V + 1)));
+  HIP_TRY(hipMalloc(&g->d_col, sizeof(int32_t) * std::max<int64_t>(n_arcs, 1)));
+  HIP_TRY(hipMalloc(&g->d_wt, sizeof(int32_t) * std::max<int64_t>(n_arcs, 1)));
+  HIP_TRY(hipMalloc(&g->d_rev, sizeof(int32_t) * std::max<int64_t>(n_arcs, 1)));
+  HIP_TRY(hipMemcpy(g->d_rowptr, beg_pos, sizeof(int64_t) * (V + 1), hipMemcpyHostToDevice));
+  if (n_arcs) {
+    HIP_TRY(hipMemcpy(g->d_col, col.data(), sizeof(int32_t) * n_arcs, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(g->d_wt, wt.data(), sizeof(int32_t) * n_arcs, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(g->d_rev, rev.data(), sizeof(int32_t) * n_arcs, hipMemcpyHostToDevice));
+  }
+  *out = g.release();
+  return GST_OK;
+}
+
+gst_status gst_graph_destroy(gst_graph* g) {
+  if (!g) return GST_OK;
+  {
+    std::lock_guard<std::mutex> lk(g->dev_mu);
+    g->plain.reset();
+    g->batch.reset();
+  }
+  (void)hipFree(g->d_rowptr);
+  (void)hipFree(g->d_col);
+  (void)hipFree(g->d_wt);
+  (void)hipFree(g->d_rev);
+  if (g->d_gptr) (void)hipFree(g->d_gptr);
+  if (g->d_gmem) (void)hipFree(g->d_gmem);
+  delete g;
+  return GST_OK;
+}
+
+gst_status gst_groups_set(gst_graph* g, const int64_t* grp_ptr, const int32_t* members, int64_t n_groups) {
+  if (!g || !grp_ptr || n_groups < 1) return fail(GST_INVALID_ARG, "bad group arguments");
+  if (grp_ptr[0] != 0) return fail(GST_INVALID_ARG, "grp_ptr must start at 0");
+  for (int64_t i = 0; i < n_groups; i++)
+    if (grp_ptr[i + 1] < grp_ptr[i]) return fail(GST_INVALID_ARG, "grp_ptr decreases at %lld", (long long)i);
+  const int64_t nm = grp_ptr[n_groups];
+  if (nm > 0 && !members) return fail(GST_INVALID_ARG, "null members");
+  for (int64_t i = 0; i < nm; i++)
+    if (members[i] < 0 || members[i] >= g->V)
+      return fail(GST_INVALID_ARG, "member %lld = %d out of range", (long long)i, members[i]);
+  std::lock_guard<std::mutex> lk(g->dev_mu);
+  if (g->d_gptr) (void)hipFree(g->d_gptr);
+  if (g->d_gmem) (void)hipFree(g->d_gmem);
+  g->d_gptr = nullptr;
+  g->d_gmem = nullptr;
+  HIP_TRY(hipMalloc(&g->d_gptr, sizeof(int64_t) * (n_groups + 1)));
+  HIP_TRY(hipMalloc(&g->d_gmem, sizeof(int32_t) * std::max<int64_t>(nm, 1)));
+  HIP_TRY(hipMemcpy(g->d_gptr, grp_ptr, sizeof(int64_t) * (n_groups + 1), hipMemcpyHostToDevice));
+  if (nm) HIP_TRY(hipMemcpy(g->d_gmem, members, sizeof(int32_t) * nm, hipMemcpyHostToDevice));
+  g->h_gptr.assign(grp_ptr, grp_ptr + n_groups + 1);
+  g->n_groups = n_groups;
+  return GST_OK;
+}
+
+static int64_t* option_ref(gst_graph* g, const std::string& k) {
+  if (k == "max_slots") return &g->opt.max_slots;
+  if (k == "budget_mb") return &g->opt.budget_mb;
+  if (k == "blocks_per_cu") return &g->opt.blocks_per_cu;
+  if (k == "chunk") return &g->opt.chunk;
+  if (k == "edge_cap") return &g->opt.edge_cap;
+  if (k == "timeout_ms") return &g->opt.timeout_ms;
+  if (k == "cooperative") return &g->opt.cooperative;
+  return nullptr;
+}
+
+gst_status gst_set_option(gst_graph* g, const char* key, int64_t value) {
+  if (!g || !key) return fail(GST_INVALID_ARG, "null argument");
+  int64_t* r = option_ref(g, key);
+  if (!r) return fail(GST_INVALID_ARG, "unknown option '%s'", key);
+  const std::string k(key);
+  if (k == "max_slots" && (value < 1 || value > gst::kMaxSlots))
+    return fail(GST_INVALID_ARG, "max_slots outside [1,%d]", gst::kMaxSlots);
+  if (k == "chunk" && (value < 64 || value > (1 << 20))) return fail(GST_INVALID_ARG, "chunk outside [64, 2^20]");
+  if (k == "edge_cap" && (value < 64 || value > (1 << 26))) return fail(GST_INVALID_ARG, "edge_cap outside [64, 2^26]");
+  if ((k == "budget_mb" || k == "blocks_per_cu" || k == "timeout_ms") && value < 0)
+    return fail(GST_INVALID_ARG, "negative value");
+  std::lock_guard<std::mutex> lk(g->dev_mu);
+  *r = value;
+  if (k == "max_slots" || k == "budget_mb") g->batch.reset();  // rebuilt on next use
+  return GST_OK;
+}
+
+gst_status gst_get_option(gst_graph* g, const char* key, int64_t* value) {
+  if (!g || !key || !value) return fail(GST_INVALID_ARG, "null argument");
+  const std::string k(key);
+  if (k == "num_cu") { *value = g->num_cu; return GST_OK; }
+  if (k == "occupancy") { *value = g->occ; return GST_OK; }
+  if (k == "grid") { *value = (int64_t)std::max<int64_t>(1, g->opt.blocks_per_cu > 0 ? std::min<int64_t>(g->opt.blocks_per_cu, g->occ) : g->occ) * g->num_cu; return GST_OK; }
+  int64_t* r = option_ref(g, key);
+  if (!r) return fail(GST_INVALID_ARG, "unknown option '%s'", key);
+  *value = *r;
+  return GST_OK;
+}
+
+gst_status gst_query_bytes(gst_graph* g, int32_t k, int32_t D, int64_t* bytes) {
+  if (!g || !bytes || k < 1 || k > GST_MAX_K || D < -1 || D > GST_MAX_D) return fail(GST_INVALID_ARG, "bad arguments");
+  *bytes = (int64_t)layout_for(g, k, D).total;
+  return GST_OK;
+}
+
+static bool has_empty_group(const gst_graph* g, const int32_t* ids, int k) {
+  for (int i = 0; i < k; i++)
+    if (g->h_gptr[ids[i] + 1] == g->h_gptr[ids[i]]) return true;
+  return false;
+}
+
+gst_status gst_solve(gst_graph* g, const int32_t* group_ids, int32_t k, int32_t D, gst_result* out) {
+  if (!out) return fail(GST_INVALID_ARG, "null result");
+  memset(out, 0, sizeof(*out));
+  gst_status st = validate_query(g, group_ids, k, D);
+  if (st != GST_OK) return st;
+  if (has_empty_group(g, group_ids, k)) {
+    fill_infeasible(out);
+    return GST_OK;
+  }
+  std::lock_guard<std::mutex> lk(g->dev_mu);
+  Engine* eng = nullptr;
+  st = engine_for(g, false, layout_for(g, k, D).total, &eng);
+  if (st != GST_OK) return st;
+  std::vector<QueryReq> qs(1);
+  qs[0].groups.assign(group_ids, group_ids + k);
+  qs[0].D = D;
+  qs[0].t_submit = now_ms();
+  return run_queries(g, eng, qs, out);
+}
+
+gst_status gst_solve_batch(gst_graph* g, int64_t n, const gst_query* q, gst_result* out) {
+  if (!g || n < 0 || (n > 0 && (!q || !out))) return fail(GST_INVALID_ARG, "bad batch arguments");
+  std::vector<QueryReq> qs;
+  std::vector<int64_t> map;
+  const double t0 = now_ms();
+  for (int64_t i = 0; i < n; i++) {
+    memset(&out[i], 0, sizeof(gst_result));
+    gst_status st = validate_query(g, q[i].group_ids, q[i].k, q[i].D);
+    if (st != GST_OK) return st;
+    if (has_empty_group(g, q[i].group_ids, q[i].k)) {
+      fill_infeasible(&out[i]);
+      continue;
+    }
+    QueryReq r;
+    r.groups.assign(q[i].group_ids, q[i].group_ids + q[i].k);
+    r.D = q[i].D;
+    r.t_submit = t0;
+    qs.push_back(std::move(r));
+    map.push_back(i);
+  }
+  if (qs.empty()) return GST_OK;
+  std::lock_guard<std::mutex> lk(g->dev_mu);
+  Engine* eng = nullptr;
+  gst_status st = engine_for(g, true, 0, &eng);
+  if (st != GST_OK) return st;
+  std::vector<gst_result> tmp(qs.size());
+  st = run_queries(g, eng, qs, tmp.data());
+  for (size_t i = 0; i < qs.size(); i++) out[map[i]] = tmp[i];
+  return st;
+}
+
+void gst_result_free(gst_result* r) {
+  if (!r) return;
+  free(r->edges);
+  r->edges = nullptr;
+  r->n_edges = 0;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ server
+
+struct gst_server {
+  gst_graph* g = nullptr;
+  std::unique_ptr<Engine> eng;
+  std::thread worker;
+  std::mutex mu;
+  std::condition_variable cv_new, cv_done;
+  std::deque<std::pair<uint64_t, QueryReq>> pending;
+  std::unordered_map<uint64_t, gst_result> done;
+  std::unordered_map<uint64_t, int> known;  // 0 queued, 1 admitted, 2 done
+  uint64_t next_ticket = 1;
+  bool stop = false;
+  gst_status error = GST_OK;
+  std::string error_msg;
+  size_t max_pending = 1 << 20;
+  uint32_t epoch_steps = 256;
+};
+
+namespace {
+
+void server_loop(gst_server* s) {
+  std::vector<int> fin;
+  for (;;) {
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      s->cv_new.wait(lk, [&] { return s->stop || !s->pending.empty() || s->eng->active() > 0; });
+      if (s->stop && s->pending.empty() && s->eng->active() == 0) break;
+      while (!s->pending.empty()) {
+        auto& pr = s->pending.front();
+        pr.second.tag = pr.first;
+        if (s->eng->admit(pr.second) < 0) {
+          if (s->eng->active() == 0) {
+            // can never fit: answer with an error result
+            gst_result r;
+            memset(&r, 0, sizeof(r));
+            r.status = GST_OUT_OF_MEMORY;
+            r.cost = GST_INFEASIBLE;
+            r.root = -1;
+            s->done[pr.first] = r;
+            s->known[pr.first] = 2;
+            s->pending.pop_front();
+            s->cv_done.notify_all();
+            continue;
+          }
+          break;
+        }
+        s->known[pr.first] = 1;
+        s->pending.pop_front();
+      }
+    }
+    if (s->eng->active() == 0) continue;
+    uint32_t exit_after;
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      exit_after = s->pending.empty() ? std::max(1, s->eng->active() / 8) : 1u;
+    }
+    gst_status st;
+    {
+      std::lock_guard<std::mutex> dk(s->g->dev_mu);
+      st = s->eng->run_epoch(exit_after, s->epoch_steps, &fin);
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (st != GST_OK) {
+      s->error = st;
+      s->error_msg = g_last_error;
+      s->stop = true;
+      s->cv_done.notify_all();
+      break;
+    }
+    for (int slot : fin) {
+      uint64_t tag = 0;
+      gst_result r;
+      s->eng->harvest(slot, &r, &tag);
+      s->done[tag] = r;
+      s->known[tag] = 2;
+    }
+    if (!fin.empty()) s->cv_done.notify_all();
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+gst_status gst_server_create(gst_graph* g, int64_t budget_bytes, int32_t max_slots, gst_server** out) {
+  if (!g || !out) return fail(GST_INVALID_ARG, "null argument");
+  if (g->n_groups <= 0) return fail(GST_INVALID_ARG, "no groups set");
+  std::unique_ptr<gst_server> s(new gst_server());
+  s->g = g;
+  int slots = max_slots > 0 ? std::min(max_slots, gst::kMaxSlots) : (int)g->opt.max_slots;
+  s->eng.reset(new Engine(g, slots));
+  size_t bytes = (size_t)budget_bytes;
+  if (budget_bytes <= 0) {
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) return fail(GST_DEVICE_ERROR, "hipMemGetInfo failed");
+    bytes = (size_t)(fr * 0.85);
+  }
+  {
+    std::lock_guard<std::mutex> dk(g->dev_mu);
+    gst_status st = s->eng->init(bytes);
+    if (st != GST_OK) return st;
+  }
+  gst_server* sp = s.get();
+  s->worker = std::thread(server_loop, sp);
+  *out = s.release();
+  return GST_OK;
+}
+
+gst_status gst_submit(gst_server* s, const gst_query* q, uint64_t* ticket) {
+  if (!s || !q || !ticket) return fail(GST_INVALID_ARG, "null argument");
+  gst_status st = validate_query(s->g, q->group_ids, q->k, q->D);
+  if (st != GST_OK) return st;
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (s->error != GST_OK) return fail(s->error, "server stopped: %s", s->error_msg.c_str());
+  if (s->stop) return fail(GST_CANCELLED, "server is shutting down");
+  if (s->pending.size() >= s->max_pending) return fail(GST_QUEUE_FULL, "server queue full");
+  const uint64_t t = s->next_ticket++;
+  *ticket = t;
+  if (has_empty_group(s->g, q->group_ids, q->k)) {
+    gst_result r;
+    fill_infeasible(&r);
+    s->done[t] = r;
+    s->known[t] = 2;
+    s->cv_done.notify_all();
+    return GST_OK;
+  }
+  QueryReq r;
+  r.groups.assign(q->group_ids, q->group_ids + q->k);
+  r.D = q->D;
+  r.t_submit = now_ms();
+  s->pending.emplace_back(t, std::move(r));
+  s->known[t] = 0;
+  s->cv_new.notify_one();
+  return GST_OK;
+}
+
+gst_status gst_poll(gst_server* s, uint64_t ticket, gst_result* out) {
+  if (!s || !out) return fail(GST_INVALID_ARG, "null argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->done.find(ticket);
+  if (it != s->done.end()) {
+    *out = it->second;
+    s->done.erase(it);
+    s->known.erase(ticket);
+    return GST_OK;
+  }
+  if (s->known.count(ticket)) {
+    if (s->error != GST_OK) return fail(s->error, "server stopped: %s", s->error_msg.c_str());
+    return GST_NOT_READY;
+  }
+  return fail(GST_INVALID_ARG, "unknown ticket %llu", (unsigned long long)ticket);
+}
+
+gst_status gst_wait(gst_server* s, uint64_t ticket, gst_result* out) {
+  if (!s || !out) return fail(GST_INVALID_ARG, "null argument");
+  std::unique_lock<std::mutex> lk(s->mu);
+  if (!s->known.count(ticket)) return fail(GST_INVALID_ARG, "unknown ticket %llu", (unsigned long long)ticket);
+  s->cv_done.wait(lk, [&] { return s->done.count(ticket) || s->error != GST_OK; });
+  auto it = s->done.find(ticket);
+  if (it == s->done.end()) return fail(s->error, "server stopped: %s", s->error_msg.c_str());
+  *out = it->second;
+  s->done.erase(it);
+  s->known.erase(ticket);
+  return GST_OK;
+}
+
+gst_status gst_server_destroy(gst_server* s) {
+  if (!s) return GST_OK;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+    s->pending.clear();
+  }
+  s->cv_new.notify_all();
+  if (s->worker.joinable()) s->worker.join();
+  for (auto& kv : s->done) gst_result_free(&kv.second);
+  {
+    std::lock_guard<std::mutex> dk(s->g->dev_mu);
+    s->eng.reset();
+  }
+  delete s;
+  return GST_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,154 @@
+/*
+ * gst.h -- C ABI of the MI355X group Steiner tree engine (libgst.so).
+ *
+ * Drop-in boundary for the reference's DPBF query path.  The reference has
+ * no library boundary; its per-query entry points are C++ functions taking
+ * host graph containers by reference:
+ *
+ *   DPBF_GPU(node**, node*, CSR_graph&, std::vector<int>& query_groups,
+ *            graph_v_of_v_idealID& group_graph, graph_v_of_v_idealID&,
+ *            int& real_cost, non_overlapped_group_sets, double& rt, int& RAM,
+ *            records&)                        code/GPUGST/include/DPQ.cuh:63
+ *   DPBF_gpu(CSR_graph&, std::vector<int>&, graph_v_of_v_idealID&,
+ *            graph_v_of_v_idealID&, int D, double*, int& real_cost, int& RAM,
+ *            records&)                        code/D-GPUGST/include/DPQ.cuh:63
+ *   main() of GPUGST+ / D-GPUGST+ driving balanced_push per query
+ *                                             code/GPUGST+/src/GSTnonHop.cu:161-510
+ *                                             code/D-GPUGST+/src/GPUHop.cu:95-268
+ *
+ * Every function returns a gst_status.  An infeasible query is not an error:
+ * it is reported with cost == GST_INFEASIBLE (the reference returns 0 or
+ * 100000 instead, code/GPUGST/src/DPQ.cu:571-575, code/GPUGST+/src/
+ * GSTnonHop.cu:435-440).  No C++ types cross this boundary.
+ *
+ * Threading: a graph handle is immutable after gst_groups_set and may be
+ * shared by threads; device work from one process is serialised on the
+ * device (one persistent engine kernel at a time), so concurrency comes from
+ * gst_solve_batch and the server, which run many queries inside one kernel.
+ */
+#ifndef GST_H_
+#define GST_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  GST_OK = 0,
+  GST_INVALID_ARG = 1,
+  GST_OUT_OF_MEMORY = 2,
+  GST_DEVICE_ERROR = 3,
+  GST_CANCELLED = 4,
+  GST_NOT_READY = 5,
+  GST_QUEUE_FULL = 6
+} gst_status;
+
+/* cost sentinel of an infeasible query (no tree covers every group) */
+#define GST_INFEASIBLE 0x3FFFFFFF
+/* D value meaning "no diameter bound" */
+#define GST_UNBOUNDED (-1)
+/* largest supported number of query groups and hop bound */
+#define GST_MAX_K 6
+#define GST_MAX_D 15
+
+typedef struct gst_graph gst_graph;
+typedef struct gst_server gst_server;
+
+/* One tree edge.  Edges are listed so that each edge's parent copy is known:
+ * `parent` is the index of the edge whose child end is this edge's parent
+ * end (u), or -1 when u is the root.  For the hop-bounded DP the optimum is
+ * a tree-like walk; a vertex may then appear in two branches, and this
+ * structure still identifies the walk's tree shape. */
+typedef struct {
+  int32_t u;      /* parent vertex (closer to the root) */
+  int32_t v;      /* child vertex */
+  int32_t w;      /* edge weight */
+  int32_t parent; /* index of the parent edge, -1 if u is the root */
+} gst_edge;
+
+/* Per-query device counters.  The reference's `records{process_queue_num,
+ * counts}` (code/GPUGST/include/DPQ.cuh:59-62) map to frontier_entries and
+ * states_written. */
+typedef struct {
+  int64_t steps;            /* engine steps (rounds + merges + phases) */
+  int64_t frontier_entries; /* frontier vertices processed */
+  int64_t arcs;             /* CSR arcs scanned */
+  int64_t relaxations;      /* (arc, subset, hop) candidates evaluated */
+  int64_t improvements;     /* successful packed atomicMin updates */
+  int64_t merge_evals;      /* (split, hop pair) merge candidates evaluated */
+} gst_counters;
+
+typedef struct {
+  int32_t cost;     /* optimal cost, or GST_INFEASIBLE */
+  int32_t root;     /* root vertex of the returned tree, -1 if infeasible */
+  int32_t n_edges;  /* number of records in edges */
+  int32_t status;   /* gst_status of this query */
+  gst_edge *edges;  /* owned by the library until gst_result_free */
+  gst_counters counters;
+  double latency_ms; /* submit -> result available (batch and server) */
+  double device_ms;  /* admission -> completion on the device */
+} gst_result;
+
+typedef struct {
+  const int32_t *group_ids; /* k distinct-or-not group ids (indices into the groups set) */
+  int32_t k;                /* 1 .. GST_MAX_K */
+  int32_t D;                /* GST_UNBOUNDED or 0 .. GST_MAX_D */
+} gst_query;
+
+/* Graph: CSR with int64 row pointers beg_pos[V+1] and int32 adjacency and
+ * weights of n_arcs arcs; undirected, both arcs present with equal weights
+ * (code/GPUGST+/include/graph.hpp:12-118 loads the same three arrays from
+ * X_beg_pos.bin / X_csr.bin / X_weight.bin).  Weights must be in [1, 2^20].
+ * The library copies the arrays to the device and keeps them resident. */
+gst_status gst_graph_create(const int64_t *beg_pos, int64_t V,
+                            const int32_t *adj, const int32_t *w,
+                            int64_t n_arcs, gst_graph **out);
+gst_status gst_graph_destroy(gst_graph *g);
+
+/* Groups: group g's members are members[grp_ptr[g] .. grp_ptr[g+1]), as read
+ * from the .g file line "g{g+1}:" (code/GPUGST/include/
+ * graph_hash_of_mixed_weighted_read_for_GSTP.h:190-239).  Replaces any
+ * earlier set. */
+gst_status gst_groups_set(gst_graph *g, const int64_t *grp_ptr,
+                          const int32_t *members, int64_t n_groups);
+
+/* One query, one call: the plain one-query-at-a-time path (the role of
+ * DPBF_GPU / DPBF_gpu).  D = GST_UNBOUNDED for the unbounded DP. */
+gst_status gst_solve(gst_graph *g, const int32_t *group_ids, int32_t k,
+                     int32_t D, gst_result *out);
+
+/* Many queries through one persistent engine kernel with memory-based
+ * admission (continuous batching).  out[i] answers q[i]. */
+gst_status gst_solve_batch(gst_graph *g, int64_t n, const gst_query *q,
+                           gst_result *out);
+
+void gst_result_free(gst_result *r);
+
+/* Tunables: "max_slots", "budget_mb", "blocks_per_cu", "chunk",
+ * "edge_cap", "timeout_ms".  Unknown keys return GST_INVALID_ARG. */
+gst_status gst_set_option(gst_graph *g, const char *key, int64_t value);
+gst_status gst_get_option(gst_graph *g, const char *key, int64_t *value);
+
+/* Device bytes one query of (k, D) needs while admitted. */
+gst_status gst_query_bytes(gst_graph *g, int32_t k, int32_t D, int64_t *bytes);
+
+/* Server: a background thread keeps the graph resident and admits submitted
+ * queries into the running engine as memory and slots free up. */
+gst_status gst_server_create(gst_graph *g, int64_t budget_bytes,
+                             int32_t max_slots, gst_server **out);
+gst_status gst_submit(gst_server *s, const gst_query *q, uint64_t *ticket);
+/* GST_NOT_READY while pending; on GST_OK *out owns the result. */
+gst_status gst_poll(gst_server *s, uint64_t ticket, gst_result *out);
+gst_status gst_wait(gst_server *s, uint64_t ticket, gst_result *out);
+gst_status gst_server_destroy(gst_server *s);
+
+/* Thread-local text of the last error. */
+const char *gst_last_error(void);
+const char *gst_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GST_H_ */

@@ -1,0 +1,136 @@
+"""GPU parity: the HIP engine against the oracle and the golden fixtures.
+
+Bar: costs bit-exact (integer DP); every returned tree validated (edges exist
+with their weights, edge sum == cost, every query group covered, hop diameter
+<= D).  Ties may pick different trees.
+"""
+import json
+import os
+import random
+
+import numpy as np
+import pytest
+
+import oracle as O
+from gpugst_amd import INFEASIBLE, Graph, data
+from gpugst_amd.validate import validate_tree
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _graph_from(rp, col, wt, groups):
+    g = Graph(rp, col, wt)
+    gptr = np.zeros(len(groups) + 1, np.int64)
+    for i, m in enumerate(groups):
+        gptr[i + 1] = gptr[i] + len(m)
+    gmem = np.concatenate([np.asarray(m, np.int32) for m in groups]) if groups else np.zeros(0, np.int32)
+    g.set_groups(gptr, gmem)
+    return g
+
+
+def _check(rp, col, wt, groups, res, expect, D=-1):
+    assert res.status == 0
+    assert res.cost == expect, (res.cost, expect)
+    if expect == INFEASIBLE:
+        assert res.root == -1 and res.edges.shape[0] == 0
+        return
+    v = validate_tree(rp, col, wt, groups, res.cost, res.root, res.edges, D=D)
+    assert v["ok"], v
+
+
+def test_tiny_golden(golden_dir):
+    cases = json.load(open(os.path.join(golden_dir, "tiny.json")))
+    for ci, c in enumerate(cases):
+        _say("tiny case", ci, "V", c["V"], "k", len(c["groups"]))
+        rp, col, wt = data.csr_from_edges(c["V"], c["edges"])
+        g = _graph_from(rp, col, wt, c["groups"])
+        ids = list(range(len(c["groups"])))
+        _check(rp, col, wt, c["groups"], g.solve(ids), c["cost"])
+        for D, hc in c["hop"].items():
+            _check(rp, col, wt, c["groups"], g.solve(ids, D=int(D)), hc, D=int(D))
+        g.close()
+
+
+def test_reference_instance(golden_dir):
+    z = np.load(os.path.join(golden_dir, "simple_iterative_tests.npz"))
+    gl = [z["gmem"][z["gptr"][i]:z["gptr"][i + 1]] for i in range(len(z["gptr"]) - 1)]
+    g = Graph(z["rowptr"], z["col"], z["wt"])
+    g.set_groups(z["gptr"], z["gmem"])
+    for D, cost in zip(z["D"], z["cost"]):
+        _say("reference instance D", int(D))
+        _check(z["rowptr"], z["col"], z["wt"], gl, g.solve([0, 1, 2, 3], D=int(D)), int(cost), D=int(D))
+    g.close()
+
+
+def test_c1_all_queries(golden_dir):
+    ref = json.load(open(os.path.join(golden_dir, "c1_costs.json")))
+    inst = data.make_config(1)
+    g = Graph(inst.rowptr, inst.col, inst.wt)
+    g.set_groups(inst.gptr, inst.gmem)
+    for qi, q in enumerate(inst.queries):
+        _say("C1 query", qi, list(q))
+        r = g.solve(q)
+        _check(inst.rowptr, inst.col, inst.wt, inst.query_groups(qi), r, ref["costs"][qi])
+    g.close()
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_random_medium_vs_oracle(seed):
+    rng = random.Random(1000 + seed)
+    for it in range(6):
+        V = rng.randint(50, 3000)
+        m = rng.randint(V, 6 * V)
+        u, v = data.chung_lu_edges(V, m, 2.3, seed * 100 + it)
+        rp, col, wt, _ = data.weighted_csr(V, u, v, 1, rng.choice([4, 100, 1000]), seed * 100 + it + 7)
+        G = 40
+        gptr, gmem = data.make_groups(V, G, rng.randint(1, 8), 50, seed * 100 + it + 9)
+        g = Graph(rp, col, wt)
+        g.set_groups(gptr, gmem)
+        for qi in range(5):
+            k = rng.randint(1, 6)
+            q = rng.sample(range(G), k)
+            groups = [gmem[gptr[x]:gptr[x + 1]] for x in q]
+            for D in (-1, rng.randint(0, 5)):
+                _say("medium", seed, it, "V", V, "k", k, "D", D)
+                expect = O.dpbf(rp, col, wt, groups, D=D)[0]
+                _check(rp, col, wt, groups, g.solve(q, D=D), expect, D=D)
+        g.close()
+
+
+def test_infeasible_and_edge_cases():
+    # two components; empty group; vertex in all groups; k = 1
+    el = [(0, 1, 3), (1, 2, 4), (3, 4, 5)]
+    rp, col, wt = data.csr_from_edges(6, el)
+    groups = [[0], [2], [4], [], [1, 3], [5]]
+    g = _graph_from(rp, col, wt, groups)
+    _check(rp, col, wt, [[0], [2]], g.solve([0, 1]), 7)
+    _check(rp, col, wt, [[0], [4]], g.solve([0, 2]), INFEASIBLE)
+    _check(rp, col, wt, [[0], []], g.solve([0, 3]), INFEASIBLE)
+    _check(rp, col, wt, [[1, 3]], g.solve([4]), 0)
+    _check(rp, col, wt, [[5]], g.solve([5]), 0)          # isolated vertex, k = 1
+    _check(rp, col, wt, [[5], [0]], g.solve([5, 0]), INFEASIBLE)
+    _check(rp, col, wt, [[0], [2]], g.solve([0, 1], D=1), INFEASIBLE)
+    _check(rp, col, wt, [[0], [2]], g.solve([0, 1], D=2), 7, D=2)
+    _check(rp, col, wt, [[0], [0]], g.solve([0, 0]), 0)  # repeated group id
+    g.close()
+
+
+def test_batch_matches_plain():
+    inst = data.make_config(2, scale=0.02, nq=6)
+    g = Graph(inst.rowptr, inst.col, inst.wt)
+    g.set_groups(inst.gptr, inst.gmem)
+    qs = inst.queries
+    Ds = [-1 if i % 3 else 2 for i in range(len(qs))]
+    plain = [g.solve(q, D=d) for q, d in zip(qs, Ds)]
+    _say("batch of", len(qs))
+    batch = g.solve_batch(qs, Ds)
+    for i, (a, b) in enumerate(zip(plain, batch)):
+        groups = inst.query_groups(i)
+        expect = O.dpbf(inst.rowptr, inst.col, inst.wt, groups, D=Ds[i])[0]
+        _check(inst.rowptr, inst.col, inst.wt, groups, a, expect, D=Ds[i])
+        _check(inst.rowptr, inst.col, inst.wt, groups, b, expect, D=Ds[i])
+    g.close()
