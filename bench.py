@@ -1,0 +1,211 @@
+"""Benchmark: optimal-GST queries/sec on one MI355X, batched vs plain.
+
+Workload (one "step" = one batch of B queries through the persistent engine,
+inputs resident in HBM): the seeded R-MAT graph of 1.1M vertices / 3.0M
+undirected edges (a=.57, b=c=.19, weights U[1,100]) with 500k geometric groups
+(mean 6), queries of k = 5 distinct group ids, no diameter bound.  The plain
+path answers the same queries one gst_solve call at a time.
+
+Prints one JSON line.  ``value`` is the batched throughput; the plain
+throughput, their ratio, latency percentiles, cost parity, the roofline of the
+engine kernel and the CPU oracle baseline ride along.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "optimal-GST queries/sec + p50/p99 latency, batched vs plain, 1 MI355X"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def pct(x, q):
+    return float(np.percentile(np.asarray(x, dtype=np.float64), q)) if len(x) else None
+
+
+def algorithmic_bytes(res_list, V, k, ND=1):
+    """Bytes the DP has to move for these queries, from the per-query device
+    counters: per frontier entry 24 B (list id, arc offset, dirty mask, row
+    pointer pair), per scanned arc 8 B (column + weight), per relaxation
+    candidate 16 B (source and target state), per improvement 8 B (the packed
+    atomic); per query 8 B per state to initialise, 8 B per state per merge
+    pass (k - 1 passes read every vertex segment) and the level-1 distance
+    table (4 B per vertex and group)."""
+    W = 1 << k
+    total = 0
+    for r in res_list:
+        c = r.counters
+        total += (24 * c["frontier_entries"] + 8 * c["arcs"] + 16 * c["relaxations"]
+                  + 8 * c["improvements"])
+        total += 8 * V * W * ND * (1 + max(k - 1, 0)) + 4 * V * k
+    return total
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=256, help="queries per step")
+    ap.add_argument("--plain", type=int, default=32, help="queries answered one at a time")
+    ap.add_argument("--k", type=int, default=5)
+    ap.add_argument("--D", type=int, default=-1)
+    ap.add_argument("--scale", type=float, default=1.0, help="shrink the graph (tests only)")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU oracle sample budget")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--max-slots", type=int, default=0)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
+              "queries are independent and would shard by replicating the graph per GPU.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from gpugst_amd import Graph, data, validate_tree
+
+    t0 = time.time()
+    inst = data.make_config(3, scale=args.scale, nq=1)
+    V = inst.V
+    n_batched = (args.warmup + args.steps) * args.batch
+    nq = max(n_batched, args.plain)
+    queries = data.make_queries(nq, args.k, inst.meta["eligible"], 3000 + 4 + 10 * args.k + 1)
+    log(f"graph V={V} arcs={inst.col.shape[0]} groups={inst.gptr.shape[0]-1} "
+        f"queries={nq} k={args.k} D={args.D} gen {time.time()-t0:.1f}s")
+
+    g = Graph(inst.rowptr, inst.col, inst.wt)
+    g.set_groups(inst.gptr, inst.gmem)
+    if args.max_slots:
+        g.set_option("max_slots", args.max_slots)
+    Ds = [args.D] * nq
+
+    # ---------------- plain: one query per call
+    g.solve(queries[0], D=args.D)  # warm-up (allocates the single-query arena)
+    g.set_option("reset_stats", 0)
+    plain_res, plain_lat = [], []
+    tp = time.perf_counter()
+    for i in range(args.plain):
+        t = time.perf_counter()
+        plain_res.append(g.solve(queries[i], D=args.D))
+        plain_lat.append((time.perf_counter() - t) * 1e3)
+    plain_s = time.perf_counter() - tp
+    plain_qps = args.plain / plain_s
+    log(f"plain: {args.plain} queries {plain_s:.3f}s -> {plain_qps:.1f} q/s "
+        f"p50 {pct(plain_lat, 50):.2f} ms p99 {pct(plain_lat, 99):.2f} ms")
+
+    # ---------------- batched: warmup + timed steps
+    bres = []
+    for w in range(args.warmup):
+        qs = queries[w * args.batch:(w + 1) * args.batch]
+        bres += g.solve_batch(list(qs), Ds[:len(qs)])
+        log(f"warmup step {w} done")
+    g.set_option("reset_stats", 0)
+    timed = []
+    step_ms = []
+    tb = time.perf_counter()
+    for s in range(args.steps):
+        a = (args.warmup + s) * args.batch
+        qs = queries[a:a + args.batch]
+        t = time.perf_counter()
+        r = g.solve_batch(list(qs), Ds[:len(qs)])
+        step_ms.append((time.perf_counter() - t) * 1e3)
+        timed += r
+        log(f"step {s}: {step_ms[-1]:.1f} ms")
+    batch_s = time.perf_counter() - tb
+    batch_qps = len(timed) / batch_s
+    kern_ns = g.get_option("batch_kernel_ns")
+    launches = g.get_option("batch_launches")
+    log(f"batched: {len(timed)} queries {batch_s:.3f}s -> {batch_qps:.1f} q/s; "
+        f"kernel {kern_ns/1e6:.1f} ms in {launches} launches")
+
+    # ---------------- parity: plain vs batched on shared queries, trees valid
+    all_b = bres + timed
+    mism = 0
+    for i in range(min(args.plain, len(all_b))):
+        if plain_res[i].cost != all_b[i].cost:
+            mism += 1
+    bad_trees = 0
+    for i, r in enumerate(all_b[:64]):
+        if r.feasible:
+            v = validate_tree(inst.rowptr, inst.col, inst.wt,
+                              [inst.group(int(x)) for x in queries[i]], r.cost, r.root, r.edges, D=args.D)
+            bad_trees += 0 if v["ok"] else 1
+    infeas = sum(1 for r in timed if not r.feasible)
+
+    # ---------------- CPU oracle baseline on a bounded sample (also parity)
+    cpu = None
+    oracle_mism = 0
+    oracle_checked = 0
+    if not args.no_cpu:
+        import oracle
+        oracle.lib()
+        tc = time.perf_counter()
+        n = 0
+        while n < len(all_b) and (time.perf_counter() - tc) < args.cpu_seconds:
+            c = oracle.dpbf(inst.rowptr, inst.col, inst.wt,
+                            [inst.group(int(x)) for x in queries[n]], D=args.D)[0]
+            oracle_mism += int(c != all_b[n].cost)
+            n += 1
+        cs = time.perf_counter() - tc
+        oracle_checked = n
+        cpu = {"value": n / cs, "unit": "queries/s", "cores": 1, "kind": "port",
+               "sample": f"first {n} of the same k={args.k} queries, exact unpruned DPBF "
+                         f"(oracle/gst_oracle.c), single thread, {cs:.1f} s"}
+        log(f"cpu oracle: {n} queries in {cs:.1f}s, mismatches {oracle_mism}")
+
+    # ---------------- roofline of the engine kernel over the timed steps
+    alg = algorithmic_bytes(timed, V, args.k, 1 if args.D < 0 else args.D + 1)
+    achieved = alg / (kern_ns * 1e-9) / 1e9 if kern_ns else 0.0
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "gst_engine_kernel", "algorithmic_bytes_per_launch": int(alg / max(launches, 1)),
+            "avg_launch_ms": round(kern_ns / 1e6 / max(launches, 1), 3)}
+
+    lat = [r.latency_ms for r in timed]
+    dev = [r.device_ms for r in timed]
+    out = {
+        "metric": METRIC,
+        "value": round(batch_qps, 2),
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(batch_s * 1e3 / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": f"rmat-1.1M k={args.k} {'unbounded' if args.D < 0 else f'D={args.D}'} "
+                               f"batched, {args.batch} queries/step",
+                   "V": V, "arcs": int(inst.col.shape[0]), "groups": int(inst.gptr.shape[0] - 1),
+                   "k": args.k, "D": args.D, "batch": args.batch, "scale": args.scale},
+        "plain_qps": round(plain_qps, 2),
+        "batched_vs_plain": round(batch_qps / plain_qps, 2) if plain_qps else None,
+        "latency_ms": {"plain_p50": pct(plain_lat, 50), "plain_p99": pct(plain_lat, 99),
+                       "batched_device_p50": pct(dev, 50), "batched_device_p99": pct(dev, 99),
+                       "batched_submit_p50": pct(lat, 50), "batched_submit_p99": pct(lat, 99)},
+        "parity": {"plain_vs_batched_mismatches": mism, "invalid_trees": bad_trees,
+                   "oracle_checked": oracle_checked, "oracle_mismatches": oracle_mism,
+                   "infeasible": infeas},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    g.close()
+
+
+if __name__ == "__main__":
+    main()

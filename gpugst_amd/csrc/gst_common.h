@@ -1,8 +1,10 @@
 // gst_common.h -- layouts shared by the host engine and the gfx950 kernels.
 //
 // State layout (one query, "slot"): T[((v * ND) + d) * W + p] is the packed
-// 64-bit state (cost << 32 | pred) of the DP entry T(v, p, d) -- vertex-major,
-// so the W * ND costs of one vertex form one contiguous, coalesced segment.
+// 64-bit state (cost << 32 | pred) of the DP entry T(v, p, d) for |p| >= 2 --
+// vertex-major, so the W * ND costs of one vertex form one contiguous,
+// coalesced segment.  Singleton states |p| = 1 (the group->vertex distances,
+// needed at every vertex) live in the compact S1[((v * ND) + d) * k + i].
 // ND = D + 1 for the hop-bounded DP and 1 for the unbounded one.  The reference
 // keeps 28 B / 40 B AoS nodes per state (code/GPUGST/include/DPQ.cuh:51-58,
 // code/D-GPUGST/include/DPQ.cuh:51-58) and writes cost and predecessor with two
@@ -45,7 +47,8 @@ enum Phase : int32_t {
   PH_MERGE = 5,  // level-L merge pass over every vertex segment
   PH_FINAL = 6,  // full-set merge + argmin
   PH_RECON = 7,  // predecessor chase -> tree edges
-  PH_DONE = 8
+  PH_DONE = 8,
+  PH_LIVE = 9    // live-vertex list + initialisation of their T segments
 };
 
 struct GraphDev {
@@ -70,7 +73,9 @@ struct alignas(64) Slot {
   int32_t edge_cap;
   int32_t chunk;        // arcs per ROUND item
   uint64_t nT;          // V * ND * W
-  uint64_t* T;
+  uint64_t* T;          // states with |p| >= 2; only live vertices' segments are touched
+  uint64_t* S1;         // V * ND * k singleton states T(v, {i}, d) = hop-bounded distances
+  uint32_t* live;       // V: vertices that can lie on a tree cheaper than the bound
   uint64_t* mask;       // 3 * V (triple-buffered dirty sub-masks)
   uint32_t* C;          // V * k group->vertex distances (padded to even)
   uint32_t* list;       // 3 * V frontier vertices
@@ -85,6 +90,8 @@ struct alignas(64) Slot {
   uint64_t app[3];      // (count << 32) | arcs of the three frontier buffers
   uint32_t best;        // upper bound on the optimum (cost of a real walk-tree)
   uint32_t status;
+  uint32_t nlive;
+  uint32_t pad1;
   uint64_t ans;         // (cost << 32) | root vertex
   uint32_t vv[kMaxK * kMaxK];
   int32_t out_cost, out_root, out_nedges, out_d;

@@ -87,11 +87,29 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t x) {
   return x;
 }
 
-__device__ __forceinline__ uint64_t sidx(uint32_t v, uint32_t p, uint32_t d, int ND, int W) {
-  return ((uint64_t)v * (uint64_t)ND + d) * (uint64_t)W + p;
-}
-
 __device__ __forceinline__ uint32_t sat(uint32_t a) { return a < kInf ? a : kInf; }
+
+// Address of state (v, p, d): singletons in the compact S1 table, every other
+// subset in the vertex-major T segments.
+struct States {
+  uint64_t* T;
+  uint64_t* S1;
+  int k, ND, W;
+  __device__ __forceinline__ uint64_t* at(uint32_t v, uint32_t p, uint32_t d) const {
+    if ((p & (p - 1u)) == 0u) return S1 + ((uint64_t)v * ND + d) * (uint64_t)k + (uint32_t)(__ffs(p) - 1);
+    return T + ((uint64_t)v * ND + d) * (uint64_t)W + p;
+  }
+};
+
+__device__ __forceinline__ States states_of(const Slot* sp) {
+  States st;
+  st.T = sp->T;
+  st.S1 = sp->S1;
+  st.k = sp->k;
+  st.ND = sp->ND;
+  st.W = sp->W;
+  return st;
+}
 
 // Lower bound on the cost of the cheapest tree containing u that covers the
 // groups in x (x non-empty): max( max_{i in x} c_i(u),
@@ -158,13 +176,13 @@ __device__ __forceinline__ void wave_append(bool flag, uint32_t u, Slot* sp, int
 // ---------------------------------------------------------------- items ----
 
 __device__ void item_fill(Slot* sp, uint32_t idx, uint64_t V) {
-  const uint64_t nT = sp->nT;
+  const uint64_t nT = V * (uint64_t)sp->ND * (uint64_t)sp->k;  // S1 words
   const uint64_t nM = 3 * V;
   const uint64_t nC = ((uint64_t)V * sp->k + 1) / 2;
   const uint64_t total = nT + nM + nC;
   const uint64_t base = (uint64_t)idx * kFillWords;
   const uint64_t end = base + kFillWords < total ? base + kFillWords : total;
-  uint64_t* T = sp->T;
+  uint64_t* T = sp->S1;
   uint64_t* M = sp->mask;
   uint64_t* C64 = (uint64_t*)sp->C;
   const uint64_t cinf = ((uint64_t)kInf << 32) | kInf;
@@ -185,18 +203,27 @@ __device__ void item_seed(const GraphDev& g, Slot* sp, uint32_t i) {
   const int32_t grp = sp->qg[i];
   const int64_t beg = g.gptr[grp], end = g.gptr[grp + 1];
   uint64_t* mn = sp->mask + (uint64_t)nxt * V;
+  const int k = sp->k;
+  uint32_t vmin = 0xFFFFFFFFu;
   for (int64_t base = beg; base < end; base += 64) {
     const int64_t t = base + lane_id();
     bool app = false;
     uint32_t v = 0;
     if (t < end) {
       v = (uint32_t)g.gmem[t];
-      sp->T[sidx(v, 1u << i, 0, ND, W)] = (uint64_t)kPredSeed;  // cost 0
+      sp->S1[((uint64_t)v * ND) * k + i] = (uint64_t)kPredSeed;  // cost 0, d = 0
       unsigned long long old = atomicOr((unsigned long long*)&mn[v], 1ull << (1u << i));
       app = (old == 0);
+      vmin = v < vmin ? v : vmin;
     }
     wave_append(app, v, sp, nxt, g, V);
   }
+  if (k == 1) {
+    // a member alone is an optimal tree; the smallest member id is the root
+    vmin = wave_min_u32(vmin);
+    if (lane_id() == 0 && vmin != 0xFFFFFFFFu) atomicMin((unsigned long long*)&sp->ans, (unsigned long long)vmin);
+  }
+  (void)W;
 }
 
 __device__ void item_clear(Slot* sp, uint32_t idx, uint64_t V) {
@@ -214,7 +241,7 @@ __device__ void item_clear(Slot* sp, uint32_t idx, uint64_t V) {
 __device__ void item_round(const GraphDev& g, Slot* sp, uint32_t chunk, WaveLDS* L) {
   const int lane = lane_id();
   const uint64_t V = (uint64_t)g.V;
-  const int k = sp->k, W = sp->W, ND = sp->ND, D = sp->D;
+  const int k = sp->k, W = sp->W, D = sp->D;
   const bool hop = D >= 0;
   const int dmax = hop ? D : 1;  // source depths that may grow
   const int level = sp->level;
@@ -231,7 +258,7 @@ __device__ void item_round(const GraphDev& g, Slot* sp, uint32_t chunk, WaveLDS*
   const uint32_t* lof = sp->loff + (uint64_t)cur * V;
   const uint64_t* mcur = sp->mask + (uint64_t)cur * V;
   uint64_t* mnxt = sp->mask + (uint64_t)nxt * V;
-  uint64_t* T = sp->T;
+  const States S = states_of(sp);
   const uint32_t* C = sp->C;
   const uint32_t* R = sp->R;
 
@@ -277,20 +304,22 @@ __device__ void item_round(const GraphDev& g, Slot* sp, uint32_t chunk, WaveLDS*
           const uint32_t x = full ^ p;
           uint32_t lbx = 0xFFFFFFFFu;
           for (int d = 0; d < dmax; d++) {
-            const uint32_t sc = (uint32_t)(T[sidx(vs, p, d, ND, W)] >> 32);
+            const uint32_t sc = (uint32_t)(*S.at(vs, p, d) >> 32);
             if (sc >= kInf) continue;
             const uint32_t cand = sc + w;
             if (cand >= kInf) continue;
             n_relax++;
             const int td = hop ? d + 1 : 0;
-            unsigned long long* tp = (unsigned long long*)&T[sidx(u, p, td, ND, W)];
-            const uint32_t tc = (uint32_t)(*tp >> 32);
-            if (cand >= tc) continue;
             if (level >= 2) {
+              // bound first: a target that fails it is never read, so only the
+              // segments of live vertices are ever touched
               if (lbx == 0xFFFFFFFFu) lbx = lower_bound(C + (uint64_t)u * k, k, x, R);
               const uint32_t bst = __hip_atomic_load(&sp->best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
               if (cand + lbx > bst) continue;
             }
+            unsigned long long* tp = (unsigned long long*)S.at(u, p, td);
+            const uint32_t tc = (uint32_t)(*tp >> 32);
+            if (cand >= tc) continue;
             if (revarc == 0xFFFFFFFFu) revarc = (uint32_t)g.rev[e];
             const unsigned long long nv = ((unsigned long long)cand << 32) | (kPredGrow | revarc);
             const unsigned long long old = atomicMin(tp, nv);
@@ -302,7 +331,7 @@ __device__ void item_round(const GraphDev& g, Slot* sp, uint32_t chunk, WaveLDS*
                 uint32_t cbest = kInf;
                 const int d2max = hop ? (D - td) : 0;
                 for (int d2 = 0; d2 <= d2max; d2++) {
-                  const uint32_t c2 = (uint32_t)(T[sidx(u, x, d2, ND, W)] >> 32);
+                  const uint32_t c2 = (uint32_t)(*S.at(u, x, d2) >> 32);
                   cbest = c2 < cbest ? c2 : cbest;
                 }
                 if (cbest < kInf) {
@@ -335,16 +364,17 @@ __device__ void item_round(const GraphDev& g, Slot* sp, uint32_t chunk, WaveLDS*
 // Distance table, inter-group distances and route table after level 1.
 __device__ void item_dist(const GraphDev& g, Slot* sp, uint32_t tile) {
   const uint64_t V = (uint64_t)g.V;
-  const int k = sp->k, W = sp->W, ND = sp->ND;
+  const int k = sp->k, ND = sp->ND;
   const bool hop = sp->D >= 0;
   const uint64_t v0 = (uint64_t)tile * kDistTV;
   uint32_t star = kInf;
   for (uint64_t v = v0 + lane_id(); v < v0 + kDistTV && v < V; v += 64) {
     uint32_t s = 0;
+    const uint64_t* row = sp->S1 + v * (uint64_t)ND * k;
     for (int i = 0; i < k; i++) {
       uint32_t c = kInf;
       for (int d = 0; d < ND; d++) {
-        const uint32_t t = (uint32_t)(sp->T[sidx((uint32_t)v, 1u << i, d, ND, W)] >> 32);
+        const uint32_t t = (uint32_t)(row[d * k + i] >> 32);
         c = t < c ? t : c;
       }
       sp->C[v * k + i] = c;
@@ -376,7 +406,7 @@ __device__ void item_vvr(const GraphDev& g, Slot* sp, WaveLDS* L) {
       for (int j = 0; j < kMaxK; j++) {
         if (j >= k) continue;
         for (int d = 0; d < ND; d++) {
-          const uint32_t c = (uint32_t)(sp->T[sidx(v, 1u << j, d, ND, W)] >> 32);
+          const uint32_t c = (uint32_t)(sp->S1[((uint64_t)v * ND + d) * k + j] >> 32);
           loc[j] = c < loc[j] ? c : loc[j];
         }
       }
@@ -420,9 +450,72 @@ __device__ void item_vvr(const GraphDev& g, Slot* sp, WaveLDS* L) {
   if (lane < kk) sp->vv[lane] = vvl[lane];
 }
 
-// Level-L merge: every |p| = L subset of every vertex from the final lower
-// levels.  One wave lane per subset p (W <= 64), 64 / W vertices per pass,
-// their W * ND segment staged in LDS with one coalesced load.
+// Stages the W * ND segments of nv live vertices into LDS (row-major like T);
+// singleton entries are taken from S1, the unused p = 0 column is INF.
+__device__ __forceinline__ void stage_rows(const Slot* sp, const uint32_t* ids, int nv, int rowlen,
+                                           int W, int ND, int k, uint64_t* stage) {
+  for (int t = lane_id(); t < nv * rowlen; t += 64) {
+    const int jj = t / rowlen, w = t - jj * rowlen;
+    const uint32_t v = ids[jj];
+    const int d = w / W;
+    const uint32_t p = (uint32_t)(w - d * W);
+    uint64_t val;
+    if (p == 0)
+      val = kInfState;
+    else if ((p & (p - 1u)) == 0u)
+      val = sp->S1[((uint64_t)v * ND + d) * (uint64_t)k + (uint32_t)(__ffs(p) - 1)];
+    else
+      val = sp->T[(uint64_t)v * rowlen + w];
+    stage[t] = val;
+  }
+}
+
+// Live list after level 1: a vertex whose distance to some query group exceeds
+// the current upper bound cannot lie on a tree cheaper than that bound, so
+// it never holds a useful state of level >= 2.  Live vertices get their T
+// segments initialised here (and only they are ever read or written later).
+__device__ void item_live(Slot* sp, uint32_t tile, uint64_t V, WaveLDS* L) {
+  const int lane = lane_id();
+  const int k = sp->k, W = sp->W, ND = sp->ND;
+  const int rowlen = W * ND;
+  const uint32_t bst = __hip_atomic_load(&sp->best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const uint64_t v0 = (uint64_t)tile * kDistTV;
+  const uint64_t v1 = v0 + kDistTV < V ? v0 + kDistTV : V;
+  for (uint64_t vb = v0; vb < v1; vb += 64) {
+    const uint64_t v = vb + lane;
+    bool lv = false;
+    if (v < v1) {
+      uint32_t m = 0;
+      for (int i = 0; i < k; i++) {
+        const uint32_t c = sp->C[v * k + i];
+        m = c > m ? c : m;
+      }
+      lv = (m < kInf) && (m <= bst);
+    }
+    const uint64_t ball = __ballot(lv);
+    if (ball == 0) continue;
+    const int n = __popcll(ball);
+    const uint32_t rank = __popcll(ball & __lanemask_lt());
+    uint32_t base = 0;
+    const int leader = __ffsll((long long)ball) - 1;
+    if (lane == leader) base = atomicAdd(&sp->nlive, (uint32_t)n);
+    base = __shfl(base, leader, 64);
+    if (lv) {
+      sp->live[base + rank] = (uint32_t)v;
+      L->wv[rank] = (uint32_t)v;
+    }
+    wave_sync();
+    for (int t = lane; t < n * rowlen; t += 64) {
+      const int jj = t / rowlen;
+      sp->T[(uint64_t)L->wv[jj] * rowlen + (t - jj * rowlen)] = kInfState;
+    }
+    wave_sync();
+  }
+}
+
+// Level-L merge: every |p| = L subset of every live vertex from the final
+// lower levels.  One wave lane per subset p (W <= 64), 64 / W vertices per
+// pass, their W * ND segments staged in LDS.
 __device__ void item_merge(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* L) {
   const int lane = lane_id();
   const uint64_t V = (uint64_t)g.V;
@@ -437,17 +530,17 @@ __device__ void item_merge(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* 
   const int VPW = 64 / W;
   const int j = lane / W;
   const uint32_t p = (uint32_t)(lane % W);
+  const uint64_t nl = sp->nlive;
   const uint64_t v0 = (uint64_t)tile * kMergeTV;
-  const uint64_t v1 = v0 + kMergeTV < V ? v0 + kMergeTV : V;
+  const uint64_t v1 = v0 + kMergeTV < nl ? v0 + kMergeTV : nl;
   const uint32_t bst0 = __hip_atomic_load(&sp->best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   uint32_t ubmin = kInf;
   uint64_t n_eval = 0;
   for (uint64_t vb = v0; vb < v1; vb += VPW) {
     const int nv = (int)((v1 - vb) < (uint64_t)VPW ? (v1 - vb) : (uint64_t)VPW);
-    const uint64_t* src = sp->T + vb * (uint64_t)rowlen;
-    for (int t = lane; t < nv * rowlen; t += 64) L->stage[t] = src[t];
+    stage_rows(sp, sp->live + vb, nv, rowlen, W, ND, k, L->stage);
     wave_sync();
-    const uint32_t v = (uint32_t)(vb + j);
+    const uint32_t v = (j < nv) ? sp->live[vb + j] : 0u;
     const bool act = (j < nv) && p != 0 && __popc(p) == level;
     bool wrote = false;
     if (act) {
@@ -491,7 +584,7 @@ __device__ void item_merge(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* 
         if (c >= kInf) continue;
         if (lbx == 0xFFFFFFFFu) lbx = lower_bound(sp->C + (uint64_t)v * k, k, x, sp->R);
         if (c + lbx > bst0) continue;
-        sp->T[sidx(v, p, nd, ND, W)] = bestpk;
+        sp->T[((uint64_t)v * ND + nd) * W + p] = bestpk;
         wrote = true;
         // upper bounds from complete walk-trees at v
         if (!hop) {
@@ -531,8 +624,7 @@ __device__ void item_merge(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* 
 // Full-set merge: W/2 lanes per vertex, one lane per split (sub contains bit 0).
 __device__ void item_final(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* L) {
   const int lane = lane_id();
-  const uint64_t V = (uint64_t)g.V;
-  const int W = sp->W, ND = sp->ND, D = sp->D;
+  const int k = sp->k, W = sp->W, ND = sp->ND, D = sp->D;
   const bool hop = D >= 0;
   const int Dlim = hop ? D : 0;
   const uint32_t full = (uint32_t)W - 1;
@@ -545,17 +637,17 @@ __device__ void item_final(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* 
   const int s = lane % half;
   const uint32_t sub = 2u * (uint32_t)s + 1u;
   const uint32_t sub2 = full ^ sub;
+  const uint64_t nl = sp->nlive;
   const uint64_t v0 = (uint64_t)tile * kMergeTV;
-  const uint64_t v1 = v0 + kMergeTV < V ? v0 + kMergeTV : V;
+  const uint64_t v1 = v0 + kMergeTV < nl ? v0 + kMergeTV : nl;
   uint64_t ansmin = ~0ull;
   uint64_t n_eval = 0;
   for (uint64_t vb = v0; vb < v1; vb += VPF) {
     const int nv = (int)((v1 - vb) < (uint64_t)VPF ? (v1 - vb) : (uint64_t)VPF);
-    const uint64_t* src = sp->T + vb * (uint64_t)rowlen;
-    for (int t = lane; t < nv * rowlen; t += 64) L->stage[t] = src[t];
+    stage_rows(sp, sp->live + vb, nv, rowlen, W, ND, k, L->stage);
     wave_sync();
     const bool vact = j < nv;
-    const uint32_t v = (uint32_t)(vb + j);
+    const uint32_t v = vact ? sp->live[vb + j] : 0u;
     const uint64_t* row = L->stage + (vact ? j : 0) * rowlen;
     const bool sact = vact && sub != full;
     for (int nd = 0; nd < ND; nd++) {
@@ -589,7 +681,7 @@ __device__ void item_final(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* 
         bestpk = y < bestpk ? y : bestpk;
       }
       if (vact && s == 0 && (uint32_t)(bestpk >> 32) < kInf) {
-        sp->T[sidx(v, full, nd, ND, W)] = bestpk;
+        sp->T[((uint64_t)v * ND + nd) * W + full] = bestpk;
         const uint64_t a = (bestpk & 0xFFFFFFFF00000000ull) | v;
         ansmin = a < ansmin ? a : ansmin;
       }
@@ -610,6 +702,7 @@ __device__ void item_final(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* 
 __device__ void item_recon(const GraphDev& g, Slot* sp) {
   const int lane = lane_id();
   const int W = sp->W, ND = sp->ND, D = sp->D;
+  const States S = states_of(sp);
   const bool hop = D >= 0;
   const uint32_t full = (uint32_t)W - 1;
   const uint64_t ans = sp->ans;
@@ -626,7 +719,7 @@ __device__ void item_recon(const GraphDev& g, Slot* sp) {
   const uint32_t root = (uint32_t)ans;
   // smallest depth holding the optimum at the root
   uint32_t cd = kInf;
-  if (lane < ND) cd = (uint32_t)(sp->T[sidx(root, full, lane, ND, W)] >> 32);
+  if (lane < ND) cd = (uint32_t)(*S.at(root, full, lane) >> 32);
   const uint64_t hit = __ballot(lane < ND && cd == cost);
   const int dstar = hit ? (__ffsll((long long)hit) - 1) : 0;
   const uint32_t qcap = (uint32_t)sp->edge_cap + 2 * kMaxK;
@@ -654,7 +747,7 @@ __device__ void item_recon(const GraphDev& g, Slot* sp) {
       p = pd & 0xFFFFu;
       d = pd >> 16;
       att = qat[t];
-      pred = (uint32_t)sp->T[sidx(v, p, d, ND, W)];
+      pred = (uint32_t)*S.at(v, p, d);
     }
     const uint32_t type = pred >> 30;
     const bool isg = act && type == 1u;
@@ -741,7 +834,7 @@ __device__ void complete_step(Slot* s, Ctrl* ctrl) {
       break;
     case PH_SEED:
       if (s->k == 1) {
-        s->phase = PH_FINAL;
+        s->phase = PH_RECON;  // the answer was set while seeding
       } else {
         s->phase = PH_ROUND;
         s->level = 1;
@@ -753,6 +846,9 @@ __device__ void complete_step(Slot* s, Ctrl* ctrl) {
       if ((s->app[(r + 1) % 3] >> 32) == 0) advance_level(s);
       break;
     case PH_DIST:
+      s->phase = PH_LIVE;
+      break;
+    case PH_LIVE:
       if (s->k == 2) {
         s->phase = PH_FINAL;
       } else {
@@ -784,18 +880,21 @@ __device__ uint32_t plan_step(Slot* s, uint64_t V) {
   uint32_t work = 0;
   switch (s->phase) {
     case PH_FILL: {
-      const uint64_t total = s->nT + 3 * V + ((uint64_t)V * s->k + 1) / 2;
+      const uint64_t total = V * (uint64_t)s->ND * s->k + 3 * V + ((uint64_t)V * s->k + 1) / 2;
       work = ceil_div(total, kFillWords);
       break;
     }
     case PH_SEED: work = (uint32_t)s->k; break;
     case PH_ROUND: work = ceil_div((uint32_t)s->app[r % 3], (uint32_t)s->chunk); break;
     case PH_DIST: work = ceil_div(V, kDistTV) + 1; break;
+    case PH_LIVE: work = ceil_div(V, kDistTV); break;
     case PH_MERGE:
-    case PH_FINAL: work = ceil_div(V, kMergeTV); break;
+    case PH_FINAL: work = ceil_div(s->nlive, kMergeTV); break;
     case PH_RECON: work = 1; break;
-    default: work = 0; break;
+    default: return (s->n_items = s->n_work = 0);
   }
+  // every active step runs at least one (possibly empty) item so that it completes
+  if (work == 0) work = 1;
   const uint32_t nclr = ceil_div((uint32_t)(s->app[(r + 2) % 3] >> 32), kClearPer);
   s->n_work = work;
   s->n_items = work + nclr;
@@ -947,6 +1046,7 @@ __global__ __launch_bounds__(kBlock) void gst_engine_kernel(GraphDev g, Slot* sl
           if (local + 1 == nwork) item_vvr(g, sp, L);
           else item_dist(g, sp, local);
           break;
+        case PH_LIVE: item_live(sp, local, V, L); break;
         case PH_MERGE: item_merge(g, sp, local, L); break;
         case PH_FINAL: item_final(g, sp, local, L); break;
         case PH_RECON: item_recon(g, sp); break;

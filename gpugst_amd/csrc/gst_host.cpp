@@ -114,6 +114,7 @@ struct Options {
   int64_t edge_cap = 65536;
   int64_t timeout_ms = 10000;
   int64_t cooperative = 1;
+  int64_t graph = 0;  // replay each epoch as a captured hipGraph
 };
 
 }  // namespace
@@ -160,7 +161,7 @@ struct QueryReq {
 };
 
 struct Layout {
-  size_t T, mask, C, list, loff, R, edges, rq, total;
+  size_t T, S1, live, mask, C, list, loff, R, edges, rq, total;
   int edge_cap;
 };
 
@@ -173,6 +174,8 @@ Layout layout_for(const gst_graph* g, int k, int D) {
   L.edge_cap = (int)std::max<int64_t>(cap, 64);
   size_t off = 0;
   L.T = off;    off += align256(V * ND * W * 8);
+  L.S1 = off;   off += align256(V * ND * k * 8);
+  L.live = off; off += align256(V * 4);
   L.mask = off; off += align256(3 * V * 8);
   L.C = off;    off += align256(((V * k + 1) / 2) * 8);
   L.list = off; off += align256(3 * V * 4);
@@ -223,12 +226,17 @@ class Engine {
     if (d_ctrl_) (void)hipFree(d_ctrl_);
     if (h_slots_) (void)hipHostFree(h_slots_);
     if (h_ctrl_) (void)hipHostFree(h_ctrl_);
+    drop_graph();
+    if (ev0_) (void)hipEventDestroy(ev0_);
+    if (ev1_) (void)hipEventDestroy(ev1_);
     if (stream_) (void)hipStreamDestroy(stream_);
   }
 
   gst_status init(size_t arena_bytes) {
     HIP_TRY(hipSetDevice(g_->device));
     HIP_TRY(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreate(&ev0_));
+    HIP_TRY(hipEventCreate(&ev1_));
     HIP_TRY(hipMalloc(&d_slots_, sizeof(gst::Slot) * max_slots_));
     HIP_TRY(hipMalloc(&d_ctrl_, sizeof(gst::Ctrl)));
     HIP_TRY(hipHostMalloc(&h_slots_, sizeof(gst::Slot) * max_slots_, hipHostMallocDefault));
@@ -296,6 +304,8 @@ class Engine {
     d.nT = (uint64_t)g_->V * d.ND * d.W;
     char* base = arena_mem_ + off;
     d.T = (uint64_t*)(base + L.T);
+    d.S1 = (uint64_t*)(base + L.S1);
+    d.live = (uint32_t*)(base + L.live);
     d.mask = (uint64_t*)(base + L.mask);
     d.C = (uint32_t*)(base + L.C);
     d.list = (uint32_t*)(base + L.list);
@@ -322,7 +332,7 @@ class Engine {
     finished->clear();
     if (nslots_used_ == 0) return GST_OK;
     const size_t slot_bytes = sizeof(gst::Slot) * nslots_used_;
-    HIP_TRY(hipMemcpyAsync(d_slots_, h_slots_, slot_bytes, hipMemcpyHostToDevice, stream_));
+    if (!g_->opt.graph) HIP_TRY(hipMemcpyAsync(d_slots_, h_slots_, slot_bytes, hipMemcpyHostToDevice, stream_));
     gst::Ctrl& c = *h_ctrl_;
     c.arrive = 0;
     c.item_next = 0;
@@ -334,29 +344,82 @@ class Engine {
     c.steps = 0;
     c.exit_after = std::max(1u, exit_after);
     c.max_steps = max_steps;
-    c.nslots = (uint32_t)nslots_used_;
+    c.nslots = (uint32_t)(g_->opt.graph ? max_slots_ : nslots_used_);
     // gen keeps the value read back after the previous epoch
     const size_t hdr = offsetof(gst::Ctrl, active);
-    HIP_TRY(hipMemcpyAsync(d_ctrl_, h_ctrl_, hdr, hipMemcpyHostToDevice, stream_));
     const uint64_t ticks = (uint64_t)g_->wall_khz * (uint64_t)g_->opt.timeout_ms;
-    const gst::GraphDev gd = g_->dev();
-    int grid = grid_size();
-    hipError_t le = gst_launch_engine(&gd, d_slots_, d_ctrl_, ticks, grid, (int)g_->opt.cooperative, stream_);
-    if (le != hipSuccess && g_->opt.cooperative) {
-      (void)hipGetLastError();
-      g_->opt.cooperative = 0;
-      le = gst_launch_engine(&gd, d_slots_, d_ctrl_, ticks, grid, 0, stream_);
+    const int grid = grid_size();
+    if (g_->opt.graph) {
+      // the whole epoch (table upload, engine kernel, readback) replays as one
+      // captured hipGraph; re-captured only when the launch shape changes
+      if (gexec_ && (graph_grid_ != grid || graph_ticks_ != ticks)) drop_graph();
+      if (!gexec_) {
+        gst_status st = capture_graph(grid, ticks);
+        if (st != GST_OK) return st;
+      }
+      HIP_TRY(hipGraphLaunch(gexec_, stream_));
+    } else {
+      HIP_TRY(hipMemcpyAsync(d_ctrl_, h_ctrl_, hdr, hipMemcpyHostToDevice, stream_));
+      gst_status st = enqueue_kernel(grid, ticks, (int)g_->opt.cooperative);
+      if (st != GST_OK) return st;
+      HIP_TRY(hipMemcpyAsync(h_slots_, d_slots_, slot_bytes, hipMemcpyDeviceToHost, stream_));
+      HIP_TRY(hipMemcpyAsync(h_ctrl_, d_ctrl_, hdr, hipMemcpyDeviceToHost, stream_));
     }
-    if (le != hipSuccess) return fail(GST_DEVICE_ERROR, "engine launch failed: %s", hipGetErrorString(le));
-    HIP_TRY(hipMemcpyAsync(h_slots_, d_slots_, slot_bytes, hipMemcpyDeviceToHost, stream_));
-    HIP_TRY(hipMemcpyAsync(h_ctrl_, d_ctrl_, hdr, hipMemcpyDeviceToHost, stream_));
     HIP_TRY(hipStreamSynchronize(stream_));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, ev0_, ev1_));
+    kernel_ns_ += (int64_t)((double)ms * 1e6);
+    launches_++;
+    steps_total_ += h_ctrl_->steps;
     last_steps_ = h_ctrl_->steps;
     if (h_ctrl_->abort_flag)
       return fail(GST_DEVICE_ERROR, "engine grid barrier timed out (grid %d blocks)", grid);
     for (int i = 0; i < nslots_used_; i++)
       if (sh_[i].used && h_slots_[i].phase == gst::PH_DONE) finished->push_back(i);
     return GST_OK;
+  }
+
+  gst_status enqueue_kernel(int grid, uint64_t ticks, int cooperative) {
+    const gst::GraphDev gd = g_->dev();
+    HIP_TRY(hipEventRecord(ev0_, stream_));
+    hipError_t le = gst_launch_engine(&gd, d_slots_, d_ctrl_, ticks, grid, cooperative, stream_);
+    if (le != hipSuccess && cooperative) {
+      (void)hipGetLastError();
+      g_->opt.cooperative = 0;
+      le = gst_launch_engine(&gd, d_slots_, d_ctrl_, ticks, grid, 0, stream_);
+    }
+    if (le != hipSuccess) return fail(GST_DEVICE_ERROR, "engine launch failed: %s", hipGetErrorString(le));
+    HIP_TRY(hipEventRecord(ev1_, stream_));
+    return GST_OK;
+  }
+
+  gst_status capture_graph(int grid, uint64_t ticks) {
+    const size_t all = sizeof(gst::Slot) * max_slots_;
+    const size_t hdr = offsetof(gst::Ctrl, active);
+    HIP_TRY(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    (void)hipMemcpyAsync(d_slots_, h_slots_, all, hipMemcpyHostToDevice, stream_);
+    (void)hipMemcpyAsync(d_ctrl_, h_ctrl_, hdr, hipMemcpyHostToDevice, stream_);
+    gst_status st = enqueue_kernel(grid, ticks, 0);
+    (void)hipMemcpyAsync(h_slots_, d_slots_, all, hipMemcpyDeviceToHost, stream_);
+    (void)hipMemcpyAsync(h_ctrl_, d_ctrl_, hdr, hipMemcpyDeviceToHost, stream_);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(stream_, &graph);
+    if (st != GST_OK) return st;
+    if (e != hipSuccess) return fail(GST_DEVICE_ERROR, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&gexec_, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+      gexec_ = nullptr;
+      return fail(GST_DEVICE_ERROR, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+    graph_grid_ = grid;
+    graph_ticks_ = ticks;
+    return GST_OK;
+  }
+
+  void drop_graph() {
+    if (gexec_) (void)hipGraphExecDestroy(gexec_);
+    gexec_ = nullptr;
   }
 
   // Copies the result of a finished slot out and releases its memory.
@@ -409,6 +472,10 @@ class Engine {
   }
 
   uint32_t last_steps() const { return last_steps_; }
+  int64_t kernel_ns() const { return kernel_ns_; }
+  int64_t launches() const { return launches_; }
+  int64_t steps_total() const { return steps_total_; }
+  void reset_stats() { kernel_ns_ = launches_ = steps_total_ = 0; }
   size_t largest_free() const { return arena_.largest(); }
 
   int grid_size() const {
@@ -431,6 +498,11 @@ class Engine {
   int nslots_used_ = 0;
   int active_ = 0;
   uint32_t last_steps_ = 0;
+  hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+  hipGraphExec_t gexec_ = nullptr;
+  int graph_grid_ = 0;
+  uint64_t graph_ticks_ = 0;
+  int64_t kernel_ns_ = 0, launches_ = 0, steps_total_ = 0;
 };
 
 namespace {
@@ -655,11 +727,18 @@ static int64_t* option_ref(gst_graph* g, const std::string& k) {
   if (k == "edge_cap") return &g->opt.edge_cap;
   if (k == "timeout_ms") return &g->opt.timeout_ms;
   if (k == "cooperative") return &g->opt.cooperative;
+  if (k == "graph") return &g->opt.graph;
   return nullptr;
 }
 
 gst_status gst_set_option(gst_graph* g, const char* key, int64_t value) {
   if (!g || !key) return fail(GST_INVALID_ARG, "null argument");
+  if (std::string(key) == "reset_stats") {
+    std::lock_guard<std::mutex> lk(g->dev_mu);
+    if (g->plain) g->plain->reset_stats();
+    if (g->batch) g->batch->reset_stats();
+    return GST_OK;
+  }
   int64_t* r = option_ref(g, key);
   if (!r) return fail(GST_INVALID_ARG, "unknown option '%s'", key);
   const std::string k(key);
@@ -679,6 +758,13 @@ gst_status gst_get_option(gst_graph* g, const char* key, int64_t* value) {
   if (!g || !key || !value) return fail(GST_INVALID_ARG, "null argument");
   const std::string k(key);
   if (k == "num_cu") { *value = g->num_cu; return GST_OK; }
+  if (k.rfind("plain_", 0) == 0 || k.rfind("batch_", 0) == 0) {
+    Engine* e = (k[0] == 'p') ? g->plain.get() : g->batch.get();
+    const std::string f = k.substr(6);
+    if (f == "kernel_ns") { *value = e ? e->kernel_ns() : 0; return GST_OK; }
+    if (f == "launches") { *value = e ? e->launches() : 0; return GST_OK; }
+    if (f == "steps") { *value = e ? e->steps_total() : 0; return GST_OK; }
+  }
   if (k == "occupancy") { *value = g->occ; return GST_OK; }
   if (k == "grid") { *value = (int64_t)std::max<int64_t>(1, g->opt.blocks_per_cu > 0 ? std::min<int64_t>(g->opt.blocks_per_cu, g->occ) : g->occ) * g->num_cu; return GST_OK; }
   int64_t* r = option_ref(g, key);

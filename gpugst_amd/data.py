@@ -196,7 +196,8 @@ CONFIGS = {
     3: dict(name="rmat-1.1M-k5-D4", gen="rmat", V=1_100_000, m=3_000_000, scale=21, w=(1, 100),
             lcc=False, G=500_000, gmean=6, gmax=1_000, queries=[(5, 300)], D=4),
     4: dict(name="rmat-1.1M-kmix-batched", gen="rmat", V=1_100_000, m=3_000_000, scale=21, w=(1, 100),
-            lcc=False, G=500_000, gmean=6, gmax=1_000, queries="mix", nq=4096, kmix=(3, 6), D=-1),
+            lcc=False, G=500_000, gmean=6, gmax=1_000, queries="mix", nq=4096, kmix=(3, 6), D=-1,
+            graph_of=3),
     5: dict(name="chunglu-3M-serving", gen="chung_lu", V=3_000_000, m=117_000_000, gamma=2.0, w=(1, 1000),
             lcc=False, G=2_000_000, gmean=10, gmax=5_000, queries="mix", nq=2000, kmix=(3, 6), D=-1,
             hop_frac=0.2, hop_D=3),
@@ -211,22 +212,24 @@ def make_config(c: int, scale: float = 1.0, nq: Optional[int] = None) -> Instanc
     m = max(V, int(cfg["m"] * scale))
     G = max(8, int(cfg["G"] * scale))
     base = 1000 * c
+    gbase = 1000 * cfg.get("graph_of", c)  # configs sharing one graph share its seeds
     if cfg["gen"] == "chung_lu":
-        u, v = chung_lu_edges(V, m, cfg["gamma"], base + 1)
+        u, v = chung_lu_edges(V, m, cfg["gamma"], gbase + 1)
     else:
         sc = int(cfg["scale"])
         while (1 << (sc - 1)) >= V and sc > 1:
             sc -= 1
-        u, v = rmat_edges(V, m, sc, seed=base + 1)
-    rowptr, col, wt, (su, sv, sw) = weighted_csr(V, u, v, cfg["w"][0], cfg["w"][1], base + 2)
+        u, v = rmat_edges(V, m, sc, seed=gbase + 1)
+    rowptr, col, wt, (su, sv, sw) = weighted_csr(V, u, v, cfg["w"][0], cfg["w"][1], gbase + 2)
     if cfg["lcc"]:
         V, rowptr, col, wt = restrict_to_lcc(V, su, sv, sw, rowptr, col)
-    gptr, gmem = make_groups(V, G, cfg["gmean"], cfg["gmax"], base + 3)
+    gptr, gmem = make_groups(V, G, cfg["gmean"], cfg["gmax"], gbase + 3)
     comp, lcc, _ = components(rowptr, col)
     elig = eligible_groups(gptr, gmem, comp == lcc)
     inst = Instance(rowptr, col, wt, gptr, gmem,
                     meta=dict(config=c, name=cfg["name"], V=V, arcs=int(col.shape[0]),
                               groups=G, scale=scale))
+    inst.meta["eligible"] = elig
     if cfg["queries"] == "mix":
         n = nq if nq is not None else cfg["nq"]
         ks = uniform_ints(n, cfg["kmix"][0], cfg["kmix"][1], base + 4)

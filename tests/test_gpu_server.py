@@ -1,0 +1,81 @@
+"""Serving: continuous batching inside one persistent engine.
+
+Requests that arrive while others run are admitted as slots and memory free
+up; each result must be the same alone and in the mix (cost bit-exact, tree
+valid), whatever else shares the device.
+"""
+import threading
+import time
+
+import numpy as np
+import pytest
+
+import oracle as O
+from gpugst_amd import Graph, Server, data
+from gpugst_amd.validate import validate_tree
+
+pytestmark = pytest.mark.gpu
+
+
+def _instance():
+    inst = data.make_config(2, scale=0.02, nq=1)
+    rng = np.random.default_rng(5)
+    elig = inst.meta["eligible"]
+    qs, Ds = [], []
+    for i in range(24):
+        k = int(rng.integers(2, 7))
+        qs.append(data.make_queries(1, k, elig, 777 + i)[0])
+        Ds.append(-1 if i % 4 else 3)
+    return inst, qs, Ds
+
+
+def test_mix_equals_alone():
+    inst, qs, Ds = _instance()
+    g = Graph(inst.rowptr, inst.col, inst.wt)
+    g.set_groups(inst.gptr, inst.gmem)
+    alone = [g.solve(q, D=d) for q, d in zip(qs, Ds)]
+    # a small arena forces admission to wait for memory
+    per = max(g.query_bytes(len(q), d) for q, d in zip(qs, Ds))
+    srv = Server(g, budget_bytes=4 * per, max_slots=8)
+    tickets = [None] * len(qs)
+
+    def submitter():
+        for i, (q, d) in enumerate(zip(qs, Ds)):
+            tickets[i] = srv.submit(q, D=d)
+            time.sleep(0.002 * (i % 3))
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    th.join()
+    mixed = [srv.wait(t) for t in tickets]
+    srv.close()
+    for i, (a, b) in enumerate(zip(alone, mixed)):
+        print("query", i, "k", len(qs[i]), "D", Ds[i], "alone", a.cost, "mix", b.cost, flush=True)
+        assert a.cost == b.cost
+        if i % 6 == 0:
+            groups = inst.query_groups(0)
+            groups = [inst.group(int(x)) for x in qs[i]]
+            assert O.dpbf(inst.rowptr, inst.col, inst.wt, groups, D=Ds[i])[0] == a.cost
+        if b.feasible:
+            groups = [inst.group(int(x)) for x in qs[i]]
+            v = validate_tree(inst.rowptr, inst.col, inst.wt, groups, b.cost, b.root, b.edges, D=Ds[i])
+            assert v["ok"], v
+    g.close()
+
+
+def test_poll_and_unknown_ticket():
+    inst, qs, Ds = _instance()
+    g = Graph(inst.rowptr, inst.col, inst.wt)
+    g.set_groups(inst.gptr, inst.gmem)
+    srv = Server(g, max_slots=4)
+    t = srv.submit(qs[0], D=-1)
+    r = None
+    t0 = time.time()
+    while r is None and time.time() - t0 < 60:
+        r = srv.poll(t)
+        time.sleep(0.001)
+    assert r is not None and r.cost == g.solve(qs[0]).cost
+    with pytest.raises(Exception):
+        srv.poll(123456789)
+    srv.close()
+    g.close()
