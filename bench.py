@@ -154,15 +154,20 @@ def main():
         tc = time.perf_counter()
         n = 0
         while n < len(all_b) and (time.perf_counter() - tc) < args.cpu_seconds:
-            c = oracle.dpbf(inst.rowptr, inst.col, inst.wt,
-                            [inst.group(int(x)) for x in queries[n]], D=args.D)[0]
+            groups = [inst.group(int(x)) for x in queries[n]]
+            if args.D < 0:
+                c = oracle.dpbf_pruned(inst.rowptr, inst.col, inst.wt, groups)[0]
+            else:
+                c = oracle.dpbf(inst.rowptr, inst.col, inst.wt, groups, D=args.D)[0]
             oracle_mism += int(c != all_b[n].cost)
             n += 1
         cs = time.perf_counter() - tc
         oracle_checked = n
+        algo = ("exact pruned best-first DPBF (orc_dpbf_pruned, the exact-mode PrunedDP++ "
+                "restatement)" if args.D < 0 else "exact hop-bounded DP (orc_dpbf, HOP_cpu restatement)")
         cpu = {"value": n / cs, "unit": "queries/s", "cores": 1, "kind": "port",
-               "sample": f"first {n} of the same k={args.k} queries, exact unpruned DPBF "
-                         f"(oracle/gst_oracle.c), single thread, {cs:.1f} s"}
+               "sample": f"first {n} of the same k={args.k} queries, {algo} in oracle/gst_oracle.c, "
+                         f"single thread, {cs:.1f} s"}
         log(f"cpu oracle: {n} queries in {cs:.1f}s, mismatches {oracle_mism}")
 
     # ---------------- roofline of the engine kernel over the timed steps

@@ -76,6 +76,7 @@ struct alignas(64) Slot {
   uint64_t* T;          // states with |p| >= 2; only live vertices' segments are touched
   uint64_t* S1;         // V * ND * k singleton states T(v, {i}, d) = hop-bounded distances
   uint32_t* live;       // V: vertices that can lie on a tree cheaper than the bound
+  uint8_t* lvf;         // V: live flag per vertex
   uint64_t* mask;       // 3 * V (triple-buffered dirty sub-masks)
   uint32_t* C;          // V * k group->vertex distances (padded to even)
   uint32_t* list;       // 3 * V frontier vertices

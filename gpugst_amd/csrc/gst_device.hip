@@ -179,20 +179,24 @@ __device__ void item_fill(Slot* sp, uint32_t idx, uint64_t V) {
   const uint64_t nT = V * (uint64_t)sp->ND * (uint64_t)sp->k;  // S1 words
   const uint64_t nM = 3 * V;
   const uint64_t nC = ((uint64_t)V * sp->k + 1) / 2;
-  const uint64_t total = nT + nM + nC;
+  const uint64_t nF = (V + 7) / 8;                              // live flags
+  const uint64_t total = nT + nM + nC + nF;
   const uint64_t base = (uint64_t)idx * kFillWords;
   const uint64_t end = base + kFillWords < total ? base + kFillWords : total;
   uint64_t* T = sp->S1;
   uint64_t* M = sp->mask;
   uint64_t* C64 = (uint64_t*)sp->C;
+  uint64_t* F64 = (uint64_t*)sp->lvf;
   const uint64_t cinf = ((uint64_t)kInf << 32) | kInf;
   for (uint64_t t = base + lane_id(); t < end; t += 64) {
     if (t < nT)
       T[t] = kInfState;
     else if (t < nT + nM)
       M[t - nT] = 0;
-    else
+    else if (t < nT + nM + nC)
       C64[t - nT - nM] = cinf;
+    else
+      F64[t - nT - nM - nC] = 0;
   }
 }
 
@@ -297,7 +301,8 @@ __device__ void item_round(const GraphDev& g, Slot* sp, uint32_t chunk, WaveLDS*
         const uint32_t w = (uint32_t)g.wt[e];
         uint32_t revarc = 0xFFFFFFFFu;
         uint64_t improved = 0;
-        uint64_t mm = m;
+        // above level 1 only live targets can hold useful states
+        uint64_t mm = (level >= 2 && !sp->lvf[u]) ? 0ull : m;
         while (mm) {
           const uint32_t p = (uint32_t)(__ffsll((long long)mm) - 1);
           mm &= mm - 1;
@@ -470,10 +475,11 @@ __device__ __forceinline__ void stage_rows(const Slot* sp, const uint32_t* ids, 
   }
 }
 
-// Live list after level 1: a vertex whose distance to some query group exceeds
-// the current upper bound cannot lie on a tree cheaper than that bound, so
-// it never holds a useful state of level >= 2.  Live vertices get their T
-// segments initialised here (and only they are ever read or written later).
+// Live list after level 1: a vertex whose lower bound for covering every
+// group (the tour bound over the full set) exceeds the current upper bound
+// cannot lie on a tree cheaper than that bound, so it never holds a useful
+// state of level >= 2.  Live vertices get their T segments initialised here
+// and a live flag; only they are ever read or written later.
 __device__ void item_live(Slot* sp, uint32_t tile, uint64_t V, WaveLDS* L) {
   const int lane = lane_id();
   const int k = sp->k, W = sp->W, ND = sp->ND;
@@ -485,12 +491,10 @@ __device__ void item_live(Slot* sp, uint32_t tile, uint64_t V, WaveLDS* L) {
     const uint64_t v = vb + lane;
     bool lv = false;
     if (v < v1) {
-      uint32_t m = 0;
-      for (int i = 0; i < k; i++) {
-        const uint32_t c = sp->C[v * k + i];
-        m = c > m ? c : m;
-      }
-      lv = (m < kInf) && (m <= bst);
+      // lower bound on any tree through v that covers every group
+      const uint32_t lbf = lower_bound(sp->C + v * k, k, (uint32_t)W - 1, sp->R);
+      lv = (lbf < kInf) && (lbf <= bst);
+      if (lv) sp->lvf[v] = 1;
     }
     const uint64_t ball = __ballot(lv);
     if (ball == 0) continue;
@@ -880,7 +884,7 @@ __device__ uint32_t plan_step(Slot* s, uint64_t V) {
   uint32_t work = 0;
   switch (s->phase) {
     case PH_FILL: {
-      const uint64_t total = V * (uint64_t)s->ND * s->k + 3 * V + ((uint64_t)V * s->k + 1) / 2;
+      const uint64_t total = V * (uint64_t)s->ND * s->k + 3 * V + ((uint64_t)V * s->k + 1) / 2 + (V + 7) / 8;
       work = ceil_div(total, kFillWords);
       break;
     }

@@ -161,7 +161,7 @@ struct QueryReq {
 };
 
 struct Layout {
-  size_t T, S1, live, mask, C, list, loff, R, edges, rq, total;
+  size_t T, S1, live, lvf, mask, C, list, loff, R, edges, rq, total;
   int edge_cap;
 };
 
@@ -176,6 +176,7 @@ Layout layout_for(const gst_graph* g, int k, int D) {
   L.T = off;    off += align256(V * ND * W * 8);
   L.S1 = off;   off += align256(V * ND * k * 8);
   L.live = off; off += align256(V * 4);
+  L.lvf = off;  off += align256(((V + 7) / 8) * 8);
   L.mask = off; off += align256(3 * V * 8);
   L.C = off;    off += align256(((V * k + 1) / 2) * 8);
   L.list = off; off += align256(3 * V * 4);
@@ -306,6 +307,7 @@ class Engine {
     d.T = (uint64_t*)(base + L.T);
     d.S1 = (uint64_t*)(base + L.S1);
     d.live = (uint32_t*)(base + L.live);
+    d.lvf = (uint8_t*)(base + L.lvf);
     d.mask = (uint64_t*)(base + L.mask);
     d.C = (uint32_t*)(base + L.C);
     d.list = (uint32_t*)(base + L.list);

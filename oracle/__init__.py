@@ -49,6 +49,8 @@ def lib():
         L.orc_sssp.restype = ctypes.c_int
         L.orc_steiner_dw.argtypes = [ctypes.c_int64, P, P, P, ctypes.c_int32, P, P]
         L.orc_steiner_dw.restype = ctypes.c_int
+        L.orc_dpbf_pruned.argtypes = [ctypes.c_int64, P, P, P, ctypes.c_int32, P, P, P, P]
+        L.orc_dpbf_pruned.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -95,6 +97,25 @@ def dpbf(rowptr, col, wt, groups: Sequence[Sequence[int]], D: int = -1,
     out = [(edges[i].u, edges[i].v, edges[i].w, edges[i].parent)
            for i in range(min(ne.value, cap))]
     return cost.value, root.value, out, stats
+
+
+def dpbf_pruned(rowptr, col, wt, groups: Sequence[Sequence[int]]):
+    """Exact optimum (unbounded) by the pruned best-first search; cost only."""
+    rowptr, col, wt = _csr(rowptr, col, wt)
+    V = rowptr.shape[0] - 1
+    k = len(groups)
+    gptr = np.zeros(k + 1, dtype=np.int64)
+    for i, g in enumerate(groups):
+        gptr[i + 1] = gptr[i] + len(g)
+    gmem = np.ascontiguousarray(np.concatenate([np.asarray(g, dtype=np.int32) for g in groups]),
+                                dtype=np.int32)
+    cost = ctypes.c_int32()
+    stats = np.zeros(2, dtype=np.int64)
+    rc = lib().orc_dpbf_pruned(V, _ptr(rowptr), _ptr(col), _ptr(wt), k, _ptr(gptr), _ptr(gmem),
+                               ctypes.byref(cost), _ptr(stats))
+    if rc < 0:
+        raise RuntimeError(f"orc_dpbf_pruned failed rc={rc}")
+    return cost.value, stats
 
 
 def sssp(rowptr, col, wt, sources) -> np.ndarray:

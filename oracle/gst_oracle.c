@@ -367,3 +367,168 @@ int orc_steiner_dw(int64_t V, const int64_t *rowptr, const int32_t *col,
   free(h.a);
   return 0;
 }
+
+/* ---------------- exact pruned best-first DPBF (CPU baseline) -------------- */
+/* Exact-mode restatement of PrunedDP++ (code/PrunedDP++/include/CPUNONHOP.h:
+ * 499-992) for the unbounded DP: k multi-source Dijkstra runs for the
+ * group->vertex distances (:21-80), inter-group distances and route tables
+ * (:175-210, :257-386), a best-first loop over (v, p) popping states by cost
+ * (:635-985) with lower-bound pruning of grow and merge candidates (:436-496)
+ * and an upper bound from tree + shortest paths (:684-726) and complement
+ * merges.  Deviations that keep it exact: no approximation-ratio exit
+ * (:730-755), no 1/2 and 2/3 decomposition rules, and a candidate is dropped
+ * only when cost + lb > best (strict), which never removes a state of an
+ * optimal witness.  Returns the optimal cost only. */
+static uint32_t orc_lb(const int32_t *c, int k, uint32_t x, const int32_t *R) {
+  int64_t lb1 = 0;
+  for (int i = 0; i < k; i++)
+    if ((x >> i) & 1u) lb1 = c[i] > lb1 ? c[i] : lb1;
+  if (lb1 >= GST_ORC_INF) return GST_ORC_INF;
+  if (__builtin_popcount(x) >= 2) {
+    int64_t tour = (int64_t)3 * GST_ORC_INF;
+    const int32_t *Rx = R + (size_t)x * k * k;
+    for (int i = 0; i < k; i++) {
+      if (!((x >> i) & 1u)) continue;
+      for (int j = 0; j < k; j++) {
+        if (j == i || !((x >> j) & 1u)) continue;
+        int64_t t = (int64_t)c[i] + Rx[i * k + j] + c[j];
+        if (t < tour) tour = t;
+      }
+    }
+    int64_t lb2 = (tour + 1) / 2;
+    if (lb2 > lb1) lb1 = lb2;
+  }
+  return lb1 >= GST_ORC_INF ? GST_ORC_INF : (uint32_t)lb1;
+}
+
+int orc_dpbf_pruned(int64_t V, const int64_t *rowptr, const int32_t *col, const int32_t *wt,
+                    int32_t k, const int64_t *gptr, const int32_t *gmem, int32_t *cost_out,
+                    int64_t *stats) {
+  if (k < 1 || k > 16 || V <= 0) return -2;
+  const int64_t W = (int64_t)1 << k;
+  const int32_t full = (int32_t)(W - 1);
+  *cost_out = GST_ORC_INF;
+  for (int i = 0; i < k; i++)
+    if (gptr[i + 1] == gptr[i]) return 0; /* empty group: infeasible */
+  if (k == 1) {
+    *cost_out = 0;
+    return 0;
+  }
+  int32_t *C = (int32_t *)malloc((size_t)(V * k) * sizeof(int32_t));
+  int32_t *dist = (int32_t *)malloc((size_t)V * sizeof(int32_t));
+  int32_t *R = (int32_t *)malloc((size_t)(W * k * k) * sizeof(int32_t));
+  int32_t *cost = (int32_t *)malloc((size_t)(V * W) * sizeof(int32_t));
+  uint8_t *done = (uint8_t *)calloc((size_t)(V * W), 1);
+  uint32_t *gmask = (uint32_t *)calloc((size_t)V, sizeof(uint32_t));
+  if (!C || !dist || !R || !cost || !done || !gmask) {
+    free(C); free(dist); free(R); free(cost); free(done); free(gmask);
+    return -1;
+  }
+  for (int i = 0; i < k; i++) {
+    orc_sssp(V, rowptr, col, wt, gptr[i + 1] - gptr[i], gmem + gptr[i], dist);
+    for (int64_t v = 0; v < V; v++) C[v * k + i] = dist[v];
+    for (int64_t j = gptr[i]; j < gptr[i + 1]; j++) gmask[gmem[j]] |= 1u << i;
+  }
+  int32_t vv[16 * 16];
+  for (int i = 0; i < k; i++)
+    for (int j = 0; j < k; j++) {
+      int32_t m = GST_ORC_INF;
+      for (int64_t t = gptr[i]; t < gptr[i + 1]; t++) {
+        int32_t c = C[(int64_t)gmem[t] * k + j];
+        if (c < m) m = c;
+      }
+      vv[i * k + j] = (i == j) ? 0 : m;
+    }
+  for (int64_t t = 0; t < W * k * k; t++) R[t] = GST_ORC_INF;
+  for (int c = 2; c <= k; c++)
+    for (int64_t x = 1; x < W; x++) {
+      if (__builtin_popcountll(x) != c) continue;
+      for (int i = 0; i < k; i++)
+        for (int j = 0; j < k; j++) {
+          if (i == j || !((x >> i) & 1) || !((x >> j) & 1)) continue;
+          int64_t best = GST_ORC_INF;
+          if (c == 2) {
+            best = vv[i * k + j];
+          } else {
+            int64_t y = x ^ (1LL << j);
+            for (int m = 0; m < k; m++) {
+              if (m == i || !((y >> m) & 1)) continue;
+              int64_t cand = (int64_t)R[y * k * k + i * k + m] + vv[m * k + j];
+              if (cand < best) best = cand;
+            }
+          }
+          R[x * k * k + i * k + j] = best >= GST_ORC_INF ? GST_ORC_INF : (int32_t)best;
+        }
+    }
+  /* star upper bound */
+  int64_t best = GST_ORC_INF;
+  for (int64_t v = 0; v < V; v++) {
+    int64_t s = 0;
+    for (int i = 0; i < k; i++) s += C[v * k + i];
+    if (s < best) best = s;
+  }
+  for (int64_t s = 0; s < V * W; s++) cost[s] = GST_ORC_INF;
+  heap_t h = {0, 0, 0};
+  int64_t pops = 0;
+  for (int64_t v = 0; v < V; v++) {
+    uint32_t m = gmask[v];
+    if (!m) continue;
+    for (uint32_t p = m; p; p = (p - 1) & m) {
+      cost[v * W + p] = 0;
+      heap_push(&h, (uint64_t)(v * W + p));
+    }
+  }
+  int32_t answer = GST_ORC_INF;
+  while (h.n) {
+    uint64_t key = heap_pop(&h);
+    int64_t s = (int64_t)(key & STATE_MASK);
+    int32_t c = (int32_t)(key >> STATE_BITS);
+    if (c != cost[s] || done[s]) continue;
+    done[s] = 1;
+    pops++;
+    int32_t p = (int32_t)(s % W);
+    int64_t v = s / W;
+    if (p == full) {
+      answer = c;
+      break;
+    }
+    const int32_t *cv = C + v * k;
+    int32_t x = full ^ p;
+    /* upper bounds: tree + shortest paths, complement merge */
+    int64_t ub = c;
+    for (int i = 0; i < k; i++)
+      if ((x >> i) & 1) ub += cv[i];
+    if (cost[v * W + x] < GST_ORC_INF && (int64_t)c + cost[v * W + x] < ub) ub = (int64_t)c + cost[v * W + x];
+    if (ub < best) best = ub;
+    /* grow */
+    for (int64_t e = rowptr[v]; e < rowptr[v + 1]; e++) {
+      int32_t u = col[e];
+      int64_t nc = (int64_t)c + wt[e];
+      int64_t t = (int64_t)u * W + p;
+      if (done[t] || nc >= cost[t]) continue;
+      if (nc + orc_lb(C + (int64_t)u * k, k, (uint32_t)x, R) > best) continue;
+      cost[t] = (int32_t)nc;
+      heap_push(&h, ((uint64_t)nc << STATE_BITS) | (uint64_t)t);
+    }
+    /* merge with settled disjoint partners */
+    for (int32_t p2 = x; p2; p2 = (p2 - 1) & x) {
+      int64_t s2 = v * W + p2;
+      if (!done[s2]) continue;
+      int32_t q = p | p2;
+      int64_t nc = (int64_t)c + cost[s2];
+      int64_t t = v * W + q;
+      if (done[t] || nc >= cost[t]) continue;
+      if (q != full && nc + orc_lb(cv, k, (uint32_t)(full ^ q), R) > best) continue;
+      cost[t] = (int32_t)nc;
+      heap_push(&h, ((uint64_t)nc << STATE_BITS) | (uint64_t)t);
+    }
+  }
+  *cost_out = answer;
+  if (stats) {
+    stats[0] = pops;
+    stats[1] = (int64_t)h.cap;
+  }
+  free(h.a);
+  free(C); free(dist); free(R); free(cost); free(done); free(gmask);
+  return 0;
+}

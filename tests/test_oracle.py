@@ -210,3 +210,19 @@ def test_planted_mistake_is_caught():
     bad = e.copy()
     bad[0, 2] += 1
     assert not validate_tree(inst.rowptr, inst.col, inst.wt, gs, c + 1, root, bad)["ok"]
+
+
+def test_pruned_cpu_solver_matches_unpruned():
+    """The pruned best-first solver (the CPU baseline) must give the same
+    optimum as the unpruned oracle."""
+    rng = random.Random(21)
+    for _ in range(60):
+        V = rng.randint(3, 80)
+        u, v = data.chung_lu_edges(V, min(rng.randint(V, 3 * V), V * (V - 1) // 2), 2.3, rng.randrange(1 << 30))
+        rp, col, wt, _ = data.weighted_csr(V, u, v, 1, rng.choice([3, 20, 100]), rng.randrange(1 << 30))
+        groups = [rng.sample(range(V), rng.randint(1, 3)) for _ in range(rng.randint(1, 6))]
+        assert O.dpbf(rp, col, wt, groups)[0] == O.dpbf_pruned(rp, col, wt, groups)[0]
+    inst = data.make_config(1)
+    for qi in range(0, 50, 5):
+        gs = inst.query_groups(qi)
+        assert O.dpbf(inst.rowptr, inst.col, inst.wt, gs)[0] == O.dpbf_pruned(inst.rowptr, inst.col, inst.wt, gs)[0]
