@@ -980,28 +980,31 @@ __global__ __launch_bounds__(kBlock) void gst_engine_kernel(GraphDev g, Slot* sl
   __shared__ WaveLDS lds[kWaves];
   __shared__ uint32_t sh_items[kMaxSlots];
   __shared__ uint32_t sh_scan[2 * kBlock];
-  __shared__ uint32_t sh_flag;
+  __shared__ uint32_t sh_ctl[4];  // last-arriver flag, leave flag, item total, active slots
   const int wid = threadIdx.x >> 6;
   const int lane = lane_id();
   WaveLDS* L = &lds[wid];
   const uint64_t V = (uint64_t)g.V;
-  uint32_t my_gen = __hip_atomic_load(&ctrl->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+  uint32_t my_gen = 0;
+  if (threadIdx.x == 0) my_gen = __hip_atomic_load(&ctrl->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
 
-  for (;;) {
+  // The arrival counter only grows during a launch (the host zeroes it before
+  // each launch): barrier b completes when it reaches (b + 1) * gridDim.x, so
+  // it never has to be reset while other blocks may be arriving.
+  for (uint32_t b = 0;; b++) {
     // ---------------- grid barrier; the last arriving block runs phase B
     __syncthreads();
     if (threadIdx.x == 0) {
       __threadfence();
       const uint32_t a = atomicAdd(&ctrl->arrive, 1u);
-      sh_flag = (a == gridDim.x - 1) ? 1u : 0u;
+      sh_ctl[0] = (a == (b + 1) * gridDim.x - 1) ? 1u : 0u;
     }
     __syncthreads();
-    if (sh_flag) {
+    if (sh_ctl[0]) {
       __threadfence();
       phase_b(slots, ctrl, V, sh_items, sh_scan);
       __syncthreads();
       if (threadIdx.x == 0) {
-        ctrl->arrive = 0;
         __threadfence();
         __hip_atomic_store(&ctrl->gen, my_gen + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
       }
@@ -1016,15 +1019,24 @@ __global__ __launch_bounds__(kBlock) void gst_engine_kernel(GraphDev g, Slot* sl
         }
       }
     }
-    my_gen++;
+    // one thread reads every block-uniform control value and broadcasts it,
+    // so all waves of the block take the same path
+    if (threadIdx.x == 0) {
+      my_gen++;
+      __threadfence();
+      const uint32_t ab = __hip_atomic_load(&ctrl->abort_flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+      const uint32_t ex = __hip_atomic_load(&ctrl->exit_flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+      sh_ctl[1] = (ab || ex) ? 1u : 0u;
+      sh_ctl[2] = __hip_atomic_load(&ctrl->item_total, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+      sh_ctl[3] = __hip_atomic_load(&ctrl->n_active, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    }
     __syncthreads();
     __threadfence();
-    if (__hip_atomic_load(&ctrl->abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
-    if (ctrl->exit_flag) break;
+    if (sh_ctl[1]) break;
 
     // ---------------- phase A: pull work items
-    const uint32_t total = ctrl->item_total;
-    const uint32_t nact = ctrl->n_active;
+    const uint32_t total = sh_ctl[2];
+    const uint32_t nact = sh_ctl[3];
     for (;;) {
       uint32_t it = 0;
       if (lane == 0) it = atomicAdd(&ctrl->item_next, 1u);

@@ -113,7 +113,7 @@ struct Options {
   int64_t chunk = 512;
   int64_t edge_cap = 65536;
   int64_t timeout_ms = 10000;
-  int64_t cooperative = 1;
+  int64_t cooperative = 0;  // hipLaunchCooperativeKernel instead of a plain launch
   int64_t graph = 0;  // replay each epoch as a captured hipGraph
 };
 
@@ -333,6 +333,8 @@ class Engine {
   gst_status run_epoch(uint32_t exit_after, uint32_t max_steps, std::vector<int>* finished) {
     finished->clear();
     if (nslots_used_ == 0) return GST_OK;
+    step_before_.resize(max_slots_);
+    for (int i = 0; i < nslots_used_; i++) step_before_[i] = h_slots_[i].step;
     const size_t slot_bytes = sizeof(gst::Slot) * nslots_used_;
     if (!g_->opt.graph) HIP_TRY(hipMemcpyAsync(d_slots_, h_slots_, slot_bytes, hipMemcpyHostToDevice, stream_));
     gst::Ctrl& c = *h_ctrl_;
@@ -367,7 +369,18 @@ class Engine {
       HIP_TRY(hipMemcpyAsync(h_slots_, d_slots_, slot_bytes, hipMemcpyDeviceToHost, stream_));
       HIP_TRY(hipMemcpyAsync(h_ctrl_, d_ctrl_, hdr, hipMemcpyDeviceToHost, stream_));
     }
-    HIP_TRY(hipStreamSynchronize(stream_));
+    {
+      // bounded wait: a launch that does not finish is reported, not waited on forever
+      const double t_end = now_ms() + (double)g_->opt.timeout_ms * 3 + 30000.0;
+      for (;;) {
+        const hipError_t q = hipStreamQuery(stream_);
+        if (q == hipSuccess) break;
+        if (q != hipErrorNotReady)
+          return fail(GST_DEVICE_ERROR, "engine epoch failed: %s", hipGetErrorString(q));
+        if (now_ms() > t_end) return fail(GST_DEVICE_ERROR, "engine epoch did not finish in time");
+        std::this_thread::sleep_for(std::chrono::microseconds(50));
+      }
+    }
     float ms = 0;
     HIP_TRY(hipEventElapsedTime(&ms, ev0_, ev1_));
     kernel_ns_ += (int64_t)((double)ms * 1e6);
@@ -376,6 +389,15 @@ class Engine {
     last_steps_ = h_ctrl_->steps;
     if (h_ctrl_->abort_flag)
       return fail(GST_DEVICE_ERROR, "engine grid barrier timed out (grid %d blocks)", grid);
+    // progress guard: every epoch runs at least one step of some admitted query
+    if (h_ctrl_->steps == 0)
+      return fail(GST_DEVICE_ERROR, "engine epoch returned without running (grid %d blocks)", grid);
+    bool any_step = false;
+    for (int i = 0; i < nslots_used_; i++)
+      if (sh_[i].used && (h_slots_[i].step != step_before_[i] || h_slots_[i].phase == gst::PH_DONE))
+        any_step = true;
+    if (!any_step)
+      return fail(GST_DEVICE_ERROR, "engine epoch made no progress on %d admitted queries", active_);
     for (int i = 0; i < nslots_used_; i++)
       if (sh_[i].used && h_slots_[i].phase == gst::PH_DONE) finished->push_back(i);
     return GST_OK;
@@ -473,6 +495,17 @@ class Engine {
     return st;
   }
 
+  // Drops every admitted query (after a device error) so the engine is reusable.
+  void reset_slots() {
+    for (int i = 0; i < max_slots_; i++) {
+      if (sh_[i].used) arena_.release(sh_[i].off, sh_[i].len);
+      sh_[i].used = false;
+      memset(&h_slots_[i], 0, sizeof(gst::Slot));
+    }
+    active_ = 0;
+    nslots_used_ = 0;
+  }
+
   uint32_t last_steps() const { return last_steps_; }
   int64_t kernel_ns() const { return kernel_ns_; }
   int64_t launches() const { return launches_; }
@@ -497,6 +530,7 @@ class Engine {
   char* arena_mem_ = nullptr;
   Arena arena_;
   std::vector<SlotHost> sh_;
+  std::vector<uint32_t> step_before_;
   int nslots_used_ = 0;
   int active_ = 0;
   uint32_t last_steps_ = 0;
@@ -547,6 +581,10 @@ gst_status engine_for(gst_graph* g, bool batch, size_t need, Engine** out) {
   return GST_OK;
 }
 
+// Global steps per launch; the grid barrier's 32-bit arrival counter stays far
+// from wrapping (steps * grid < 2^32).
+constexpr uint32_t kEpochMaxSteps = 1u << 20;
+
 // Drives an engine over a list of queries: admits while memory and slots
 // allow, runs epochs, harvests finished slots, refills.
 gst_status run_queries(gst_graph* g, Engine* eng, std::vector<QueryReq>& qs, gst_result* out) {
@@ -568,8 +606,11 @@ gst_status run_queries(gst_graph* g, Engine* eng, std::vector<QueryReq>& qs, gst
     }
     const int act = eng->active();
     const uint32_t exit_after = pending.empty() ? (uint32_t)act : std::max(1u, (uint32_t)act / 8);
-    gst_status st = eng->run_epoch(exit_after, 0xFFFFFFFFu, &fin);
-    if (st != GST_OK) return st;
+    gst_status st = eng->run_epoch(exit_after, kEpochMaxSteps, &fin);
+    if (st != GST_OK) {
+      eng->reset_slots();
+      return st;
+    }
     for (int s : fin) {
       uint64_t tag = 0;
       gst_result r;
@@ -908,6 +949,7 @@ void server_loop(gst_server* s) {
     {
       std::lock_guard<std::mutex> dk(s->g->dev_mu);
       st = s->eng->run_epoch(exit_after, s->epoch_steps, &fin);
+      if (st != GST_OK) s->eng->reset_slots();
     }
     std::lock_guard<std::mutex> lk(s->mu);
     if (st != GST_OK) {
