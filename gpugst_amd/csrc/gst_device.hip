@@ -1,9 +1,10 @@
-// gst_device.hip -- the persistent DPBF engine kernel for gfx950 (CDNA4, wave64).
+// gst_device.hip -- the DPBF engine kernels for gfx950 (CDNA4, wave64).
 //
-// One launch runs many queries ("slots") at once.  The grid loops over
-// global steps; between steps the last block to arrive at a software grid
-// barrier advances every slot's state machine and publishes the next step's
-// work items (prefix over slots); waves then pull items with one atomic each.
+// Many queries ("slots") advance together.  One engine step is one launch of
+// gst_step_kernel: waves pull the step's work items (a prefix over slots) with
+// one atomic each.  Between steps the one-block gst_plan_kernel advances every
+// slot's state machine and publishes the next step's items.  Blocks never wait
+// for each other; every hand-off between blocks happens at a kernel boundary.
 //
 // Per query the DP is evaluated level by level in |p| (Dreyfus-Wagner order):
 //   level 1  : SEED + frontier rounds  -> T(v,{i},d) = hop-bounded / plain
@@ -975,120 +976,76 @@ __device__ void phase_b(Slot* slots, Ctrl* ctrl, uint64_t V, uint32_t* sh_items,
   }
 }
 
-__global__ __launch_bounds__(kBlock) void gst_engine_kernel(GraphDev g, Slot* slots, Ctrl* ctrl,
-                                                            uint64_t timeout_ticks) {
+// One engine step: every wave pulls work items of the current plan until it
+// is exhausted.  There is no waiting between blocks; the step ends when the
+// kernel ends, and the next plan kernel sees everything it wrote.
+__global__ __launch_bounds__(kBlock) void gst_step_kernel(GraphDev g, Slot* slots, Ctrl* ctrl) {
   __shared__ WaveLDS lds[kWaves];
-  __shared__ uint32_t sh_items[kMaxSlots];
-  __shared__ uint32_t sh_scan[2 * kBlock];
-  __shared__ uint32_t sh_ctl[4];  // last-arriver flag, leave flag, item total, active slots
   const int wid = threadIdx.x >> 6;
   const int lane = lane_id();
   WaveLDS* L = &lds[wid];
   const uint64_t V = (uint64_t)g.V;
-  uint32_t my_gen = 0;
-  if (threadIdx.x == 0) my_gen = __hip_atomic_load(&ctrl->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-
-  // The arrival counter only grows during a launch (the host zeroes it before
-  // each launch): barrier b completes when it reaches (b + 1) * gridDim.x, so
-  // it never has to be reset while other blocks may be arriving.
-  for (uint32_t b = 0;; b++) {
-    // ---------------- grid barrier; the last arriving block runs phase B
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      __threadfence();
-      const uint32_t a = atomicAdd(&ctrl->arrive, 1u);
-      sh_ctl[0] = (a == (b + 1) * gridDim.x - 1) ? 1u : 0u;
+  if (ctrl->exit_flag) return;
+  const uint32_t total = ctrl->item_total;
+  const uint32_t nact = ctrl->n_active;
+  for (;;) {
+    uint32_t it = 0;
+    if (lane == 0) it = atomicAdd(&ctrl->item_next, 1u);
+    it = __shfl(it, 0, 64);
+    if (it >= total) break;
+    uint32_t lo = 0, hi = nact;  // last active slot whose prefix <= it
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (ctrl->prefix[mid] <= it) lo = mid; else hi = mid;
     }
-    __syncthreads();
-    if (sh_ctl[0]) {
-      __threadfence();
-      phase_b(slots, ctrl, V, sh_items, sh_scan);
-      __syncthreads();
-      if (threadIdx.x == 0) {
-        __threadfence();
-        __hip_atomic_store(&ctrl->gen, my_gen + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-      }
-    } else if (threadIdx.x == 0) {
-      const uint64_t t0 = wall_clock64();
-      while (__hip_atomic_load(&ctrl->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == my_gen) {
-        __builtin_amdgcn_s_sleep(1);
-        if (__hip_atomic_load(&ctrl->abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
-        if (wall_clock64() - t0 > timeout_ticks) {
-          atomicExch(&ctrl->abort_flag, 1u);
-          break;
-        }
-      }
+    Slot* sp = &slots[ctrl->active[lo]];
+    const uint32_t local = it - ctrl->prefix[lo];
+    const uint32_t nwork = sp->n_work;
+    if (local >= nwork) {
+      item_clear(sp, local - nwork, V);
+      continue;
     }
-    // one thread reads every block-uniform control value and broadcasts it,
-    // so all waves of the block take the same path
-    if (threadIdx.x == 0) {
-      my_gen++;
-      __threadfence();
-      const uint32_t ab = __hip_atomic_load(&ctrl->abort_flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-      const uint32_t ex = __hip_atomic_load(&ctrl->exit_flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-      sh_ctl[1] = (ab || ex) ? 1u : 0u;
-      sh_ctl[2] = __hip_atomic_load(&ctrl->item_total, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-      sh_ctl[3] = __hip_atomic_load(&ctrl->n_active, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    __syncthreads();
-    __threadfence();
-    if (sh_ctl[1]) break;
-
-    // ---------------- phase A: pull work items
-    const uint32_t total = sh_ctl[2];
-    const uint32_t nact = sh_ctl[3];
-    for (;;) {
-      uint32_t it = 0;
-      if (lane == 0) it = atomicAdd(&ctrl->item_next, 1u);
-      it = __shfl(it, 0, 64);
-      if (it >= total) break;
-      uint32_t lo = 0, hi = nact;  // last active slot whose prefix <= it
-      while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (ctrl->prefix[mid] <= it) lo = mid; else hi = mid;
-      }
-      Slot* sp = &slots[ctrl->active[lo]];
-      const uint32_t local = it - ctrl->prefix[lo];
-      const uint32_t nwork = sp->n_work;
-      if (local >= nwork) {
-        item_clear(sp, local - nwork, V);
-        continue;
-      }
-      switch (sp->phase) {
-        case PH_FILL: item_fill(sp, local, V); break;
-        case PH_SEED: item_seed(g, sp, local); break;
-        case PH_ROUND: item_round(g, sp, local, L); break;
-        case PH_DIST:
-          if (local + 1 == nwork) item_vvr(g, sp, L);
-          else item_dist(g, sp, local);
-          break;
-        case PH_LIVE: item_live(sp, local, V, L); break;
-        case PH_MERGE: item_merge(g, sp, local, L); break;
-        case PH_FINAL: item_final(g, sp, local, L); break;
-        case PH_RECON: item_recon(g, sp); break;
-        default: break;
-      }
+    switch (sp->phase) {
+      case PH_FILL: item_fill(sp, local, V); break;
+      case PH_SEED: item_seed(g, sp, local); break;
+      case PH_ROUND: item_round(g, sp, local, L); break;
+      case PH_DIST:
+        if (local + 1 == nwork) item_vvr(g, sp, L);
+        else item_dist(g, sp, local);
+        break;
+      case PH_LIVE: item_live(sp, local, V, L); break;
+      case PH_MERGE: item_merge(g, sp, local, L); break;
+      case PH_FINAL: item_final(g, sp, local, L); break;
+      case PH_RECON: item_recon(g, sp); break;
+      default: break;
     }
   }
+}
+
+// Between two steps, one block advances every slot's state machine and
+// publishes the next step's work items.  Once it has set exit_flag it leaves
+// everything untouched, so the host may enqueue more steps than are needed.
+__global__ __launch_bounds__(kBlock) void gst_plan_kernel(Slot* slots, Ctrl* ctrl, uint64_t V) {
+  __shared__ uint32_t sh_items[kMaxSlots];
+  __shared__ uint32_t sh_scan[2 * kBlock];
+  if (ctrl->exit_flag) return;
+  phase_b(slots, ctrl, V, sh_items, sh_scan);
 }
 
 }  // namespace gst
 
-extern "C" hipError_t gst_launch_engine(const gst::GraphDev* g, gst::Slot* slots, gst::Ctrl* ctrl,
-                                        uint64_t timeout_ticks, int grid, int cooperative,
-                                        hipStream_t stream) {
-  gst::GraphDev gd = *g;
-  if (cooperative) {
-    void* args[] = {(void*)&gd, (void*)&slots, (void*)&ctrl, (void*)&timeout_ticks};
-    return hipLaunchCooperativeKernel((const void*)gst::gst_engine_kernel, dim3(grid), dim3(gst::kBlock),
-                                      args, 0, stream);
-  }
-  hipLaunchKernelGGL(gst::gst_engine_kernel, dim3(grid), dim3(gst::kBlock), 0, stream, gd, slots, ctrl,
-                     timeout_ticks);
+extern "C" hipError_t gst_launch_step(const gst::GraphDev* g, gst::Slot* slots, gst::Ctrl* ctrl, int grid,
+                                      hipStream_t stream) {
+  hipLaunchKernelGGL(gst::gst_step_kernel, dim3(grid), dim3(gst::kBlock), 0, stream, *g, slots, ctrl);
   return hipGetLastError();
 }
 
-extern "C" hipError_t gst_engine_occupancy(int* blocks_per_cu) {
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, (const void*)gst::gst_engine_kernel,
+extern "C" hipError_t gst_launch_plan(gst::Slot* slots, gst::Ctrl* ctrl, uint64_t V, hipStream_t stream) {
+  hipLaunchKernelGGL(gst::gst_plan_kernel, dim3(1), dim3(gst::kBlock), 0, stream, slots, ctrl, V);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t gst_step_occupancy(int* blocks_per_cu) {
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, (const void*)gst::gst_step_kernel,
                                                        gst::kBlock, 0);
 }

@@ -23,10 +23,10 @@
 #include "gst.h"
 #include "gst_common.h"
 
-extern "C" hipError_t gst_launch_engine(const gst::GraphDev* g, gst::Slot* slots, gst::Ctrl* ctrl,
-                                        uint64_t timeout_ticks, int grid, int cooperative,
-                                        hipStream_t stream);
-extern "C" hipError_t gst_engine_occupancy(int* blocks_per_cu);
+extern "C" hipError_t gst_launch_step(const gst::GraphDev* g, gst::Slot* slots, gst::Ctrl* ctrl, int grid,
+                                      hipStream_t stream);
+extern "C" hipError_t gst_launch_plan(gst::Slot* slots, gst::Ctrl* ctrl, uint64_t V, hipStream_t stream);
+extern "C" hipError_t gst_step_occupancy(int* blocks_per_cu);
 
 namespace {
 
@@ -56,6 +56,11 @@ double now_ms() {
   } while (0)
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+// Engine steps per epoch before control returns to the host loop.
+constexpr uint32_t kEpochMaxSteps = 1u << 20;
+// (step, plan) pairs per captured hipGraph chunk.
+constexpr int kGraphChunk = 16;
 
 // First-fit allocator over one device region, with coalescing of free runs.
 class Arena {
@@ -112,9 +117,8 @@ struct Options {
   int64_t blocks_per_cu = 0;  // 0: occupancy maximum
   int64_t chunk = 512;
   int64_t edge_cap = 65536;
-  int64_t timeout_ms = 10000;
-  int64_t cooperative = 0;  // hipLaunchCooperativeKernel instead of a plain launch
-  int64_t graph = 0;  // replay each epoch as a captured hipGraph
+  int64_t timeout_ms = 60000;  // host-side deadline for one chunk of engine steps
+  int64_t graph = 0;  // replay chunks of engine steps as a captured hipGraph
 };
 
 }  // namespace
@@ -123,7 +127,6 @@ struct gst_graph {
   int device = 0;
   int num_cu = 0;
   int occ = 1;
-  int64_t wall_khz = 100000;
   int64_t V = 0, n_arcs = 0;
   int64_t n_groups = 0;
   std::vector<int64_t> h_gptr;
@@ -330,15 +333,22 @@ class Engine {
 
   // Runs the engine kernel until `exit_after` queries finish (or max_steps
   // global steps pass); returns the finished slot indices.
+  // Runs engine steps until `exit_after` queries finish (or max_steps steps
+  // pass, or nothing is left to do); returns the finished slot indices.
+  // Steps are enqueued in chunks of (step kernel, plan kernel) pairs; the
+  // plan kernel turns into a no-op once it has decided to stop, so a chunk may
+  // overshoot safely.  The control header is read back after each chunk.
   gst_status run_epoch(uint32_t exit_after, uint32_t max_steps, std::vector<int>* finished) {
     finished->clear();
     if (nslots_used_ == 0) return GST_OK;
     step_before_.resize(max_slots_);
     for (int i = 0; i < nslots_used_; i++) step_before_[i] = h_slots_[i].step;
-    const size_t slot_bytes = sizeof(gst::Slot) * nslots_used_;
-    if (!g_->opt.graph) HIP_TRY(hipMemcpyAsync(d_slots_, h_slots_, slot_bytes, hipMemcpyHostToDevice, stream_));
+    const bool use_graph = g_->opt.graph != 0;
+    const size_t slot_bytes = sizeof(gst::Slot) * (use_graph ? max_slots_ : nslots_used_);
+    HIP_TRY(hipMemcpyAsync(d_slots_, h_slots_, slot_bytes, hipMemcpyHostToDevice, stream_));
     gst::Ctrl& c = *h_ctrl_;
     c.arrive = 0;
+    c.gen = 0;
     c.item_next = 0;
     c.item_total = 0;
     c.n_active = 0;
@@ -348,50 +358,47 @@ class Engine {
     c.steps = 0;
     c.exit_after = std::max(1u, exit_after);
     c.max_steps = max_steps;
-    c.nslots = (uint32_t)(g_->opt.graph ? max_slots_ : nslots_used_);
-    // gen keeps the value read back after the previous epoch
+    c.nslots = (uint32_t)(use_graph ? max_slots_ : nslots_used_);
     const size_t hdr = offsetof(gst::Ctrl, active);
-    const uint64_t ticks = (uint64_t)g_->wall_khz * (uint64_t)g_->opt.timeout_ms;
+    HIP_TRY(hipMemcpyAsync(d_ctrl_, h_ctrl_, hdr, hipMemcpyHostToDevice, stream_));
     const int grid = grid_size();
-    if (g_->opt.graph) {
-      // the whole epoch (table upload, engine kernel, readback) replays as one
-      // captured hipGraph; re-captured only when the launch shape changes
-      if (gexec_ && (graph_grid_ != grid || graph_ticks_ != ticks)) drop_graph();
-      if (!gexec_) {
-        gst_status st = capture_graph(grid, ticks);
-        if (st != GST_OK) return st;
+    const gst::GraphDev gd = g_->dev();
+    HIP_TRY(hipEventRecord(ev0_, stream_));
+    hipError_t le = gst_launch_plan(d_slots_, d_ctrl_, (uint64_t)g_->V, stream_);
+    if (le != hipSuccess) return fail(GST_DEVICE_ERROR, "plan launch failed: %s", hipGetErrorString(le));
+    int chunk = use_graph ? kGraphChunk : 4;
+    for (;;) {
+      if (use_graph) {
+        if (gexec_ && graph_grid_ != grid) drop_graph();
+        if (!gexec_) {
+          gst_status st = capture_chunk(grid);
+          if (st != GST_OK) return st;
+        }
+        HIP_TRY(hipGraphLaunch(gexec_, stream_));
+      } else {
+        for (int i = 0; i < chunk; i++) {
+          le = gst_launch_step(&gd, d_slots_, d_ctrl_, grid, stream_);
+          if (le == hipSuccess) le = gst_launch_plan(d_slots_, d_ctrl_, (uint64_t)g_->V, stream_);
+          if (le != hipSuccess) return fail(GST_DEVICE_ERROR, "step launch failed: %s", hipGetErrorString(le));
+        }
       }
-      HIP_TRY(hipGraphLaunch(gexec_, stream_));
-    } else {
-      HIP_TRY(hipMemcpyAsync(d_ctrl_, h_ctrl_, hdr, hipMemcpyHostToDevice, stream_));
-      gst_status st = enqueue_kernel(grid, ticks, (int)g_->opt.cooperative);
-      if (st != GST_OK) return st;
-      HIP_TRY(hipMemcpyAsync(h_slots_, d_slots_, slot_bytes, hipMemcpyDeviceToHost, stream_));
       HIP_TRY(hipMemcpyAsync(h_ctrl_, d_ctrl_, hdr, hipMemcpyDeviceToHost, stream_));
+      gst_status st = wait_stream();
+      if (st != GST_OK) return st;
+      if (h_ctrl_->exit_flag) break;
+      if (!use_graph) chunk = std::min(chunk * 2, 64);
     }
-    {
-      // bounded wait: a launch that does not finish is reported, not waited on forever
-      const double t_end = now_ms() + (double)g_->opt.timeout_ms * 3 + 30000.0;
-      for (;;) {
-        const hipError_t q = hipStreamQuery(stream_);
-        if (q == hipSuccess) break;
-        if (q != hipErrorNotReady)
-          return fail(GST_DEVICE_ERROR, "engine epoch failed: %s", hipGetErrorString(q));
-        if (now_ms() > t_end) return fail(GST_DEVICE_ERROR, "engine epoch did not finish in time");
-        std::this_thread::sleep_for(std::chrono::microseconds(50));
-      }
-    }
+    HIP_TRY(hipEventRecord(ev1_, stream_));
+    HIP_TRY(hipMemcpyAsync(h_slots_, d_slots_, sizeof(gst::Slot) * nslots_used_, hipMemcpyDeviceToHost, stream_));
+    gst_status st = wait_stream();
+    if (st != GST_OK) return st;
     float ms = 0;
     HIP_TRY(hipEventElapsedTime(&ms, ev0_, ev1_));
     kernel_ns_ += (int64_t)((double)ms * 1e6);
     launches_++;
     steps_total_ += h_ctrl_->steps;
     last_steps_ = h_ctrl_->steps;
-    if (h_ctrl_->abort_flag)
-      return fail(GST_DEVICE_ERROR, "engine grid barrier timed out (grid %d blocks)", grid);
     // progress guard: every epoch runs at least one step of some admitted query
-    if (h_ctrl_->steps == 0)
-      return fail(GST_DEVICE_ERROR, "engine epoch returned without running (grid %d blocks)", grid);
     bool any_step = false;
     for (int i = 0; i < nslots_used_; i++)
       if (sh_[i].used && (h_slots_[i].step != step_before_[i] || h_slots_[i].phase == gst::PH_DONE))
@@ -403,32 +410,32 @@ class Engine {
     return GST_OK;
   }
 
-  gst_status enqueue_kernel(int grid, uint64_t ticks, int cooperative) {
-    const gst::GraphDev gd = g_->dev();
-    HIP_TRY(hipEventRecord(ev0_, stream_));
-    hipError_t le = gst_launch_engine(&gd, d_slots_, d_ctrl_, ticks, grid, cooperative, stream_);
-    if (le != hipSuccess && cooperative) {
-      (void)hipGetLastError();
-      g_->opt.cooperative = 0;
-      le = gst_launch_engine(&gd, d_slots_, d_ctrl_, ticks, grid, 0, stream_);
+  // Waits for the stream with a deadline, so a launch that never finishes is
+  // reported instead of blocking the caller forever.
+  gst_status wait_stream() {
+    const double t_end = now_ms() + (double)g_->opt.timeout_ms;
+    for (;;) {
+      const hipError_t q = hipStreamQuery(stream_);
+      if (q == hipSuccess) return GST_OK;
+      if (q != hipErrorNotReady) return fail(GST_DEVICE_ERROR, "engine step failed: %s", hipGetErrorString(q));
+      if (now_ms() > t_end) return fail(GST_DEVICE_ERROR, "engine step did not finish within %lld ms",
+                                        (long long)g_->opt.timeout_ms);
+      std::this_thread::sleep_for(std::chrono::microseconds(20));
     }
-    if (le != hipSuccess) return fail(GST_DEVICE_ERROR, "engine launch failed: %s", hipGetErrorString(le));
-    HIP_TRY(hipEventRecord(ev1_, stream_));
-    return GST_OK;
   }
 
-  gst_status capture_graph(int grid, uint64_t ticks) {
-    const size_t all = sizeof(gst::Slot) * max_slots_;
-    const size_t hdr = offsetof(gst::Ctrl, active);
+  // A chunk of kGraphChunk (step, plan) launches captured once and replayed.
+  gst_status capture_chunk(int grid) {
+    const gst::GraphDev gd = g_->dev();
     HIP_TRY(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-    (void)hipMemcpyAsync(d_slots_, h_slots_, all, hipMemcpyHostToDevice, stream_);
-    (void)hipMemcpyAsync(d_ctrl_, h_ctrl_, hdr, hipMemcpyHostToDevice, stream_);
-    gst_status st = enqueue_kernel(grid, ticks, 0);
-    (void)hipMemcpyAsync(h_slots_, d_slots_, all, hipMemcpyDeviceToHost, stream_);
-    (void)hipMemcpyAsync(h_ctrl_, d_ctrl_, hdr, hipMemcpyDeviceToHost, stream_);
+    hipError_t le = hipSuccess;
+    for (int i = 0; i < kGraphChunk && le == hipSuccess; i++) {
+      le = gst_launch_step(&gd, d_slots_, d_ctrl_, grid, stream_);
+      if (le == hipSuccess) le = gst_launch_plan(d_slots_, d_ctrl_, (uint64_t)g_->V, stream_);
+    }
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamEndCapture(stream_, &graph);
-    if (st != GST_OK) return st;
+    if (le != hipSuccess) return fail(GST_DEVICE_ERROR, "capture launch failed: %s", hipGetErrorString(le));
     if (e != hipSuccess) return fail(GST_DEVICE_ERROR, "hipStreamEndCapture: %s", hipGetErrorString(e));
     e = hipGraphInstantiate(&gexec_, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
@@ -437,7 +444,6 @@ class Engine {
       return fail(GST_DEVICE_ERROR, "hipGraphInstantiate: %s", hipGetErrorString(e));
     }
     graph_grid_ = grid;
-    graph_ticks_ = ticks;
     return GST_OK;
   }
 
@@ -537,7 +543,6 @@ class Engine {
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
   hipGraphExec_t gexec_ = nullptr;
   int graph_grid_ = 0;
-  uint64_t graph_ticks_ = 0;
   int64_t kernel_ns_ = 0, launches_ = 0, steps_total_ = 0;
 };
 
@@ -580,10 +585,6 @@ gst_status engine_for(gst_graph* g, bool batch, size_t need, Engine** out) {
   *out = e.get();
   return GST_OK;
 }
-
-// Global steps per launch; the grid barrier's 32-bit arrival counter stays far
-// from wrapping (steps * grid < 2^32).
-constexpr uint32_t kEpochMaxSteps = 1u << 20;
 
 // Drives an engine over a list of queries: admits while memory and slots
 // allow, runs epochs, harvests finished slots, refills.
@@ -700,11 +701,8 @@ gst_status gst_graph_create(const int64_t* beg_pos, int64_t V, const int32_t* ad
   hipDeviceProp_t prop;
   HIP_TRY(hipGetDeviceProperties(&prop, g->device));
   g->num_cu = prop.multiProcessorCount;
-  int wr = 0;
-  if (hipDeviceGetAttribute(&wr, hipDeviceAttributeWallClockRate, g->device) == hipSuccess && wr > 0)
-    g->wall_khz = wr;
   int occ = 1;
-  if (gst_engine_occupancy(&occ) == hipSuccess && occ > 0) g->occ = occ;
+  if (gst_step_occupancy(&occ) == hipSuccess && occ > 0) g->occ = occ;
   g->V = V;
   g->n_arcs = n_arcs;
   HIP_TRY(hipMalloc(&g->d_rowptr, sizeof(int64_t) * (V + 1)));
@@ -769,7 +767,6 @@ static int64_t* option_ref(gst_graph* g, const std::string& k) {
   if (k == "chunk") return &g->opt.chunk;
   if (k == "edge_cap") return &g->opt.edge_cap;
   if (k == "timeout_ms") return &g->opt.timeout_ms;
-  if (k == "cooperative") return &g->opt.cooperative;
   if (k == "graph") return &g->opt.graph;
   return nullptr;
 }
