@@ -70,7 +70,8 @@ _lib = None
 
 
 def lib_path() -> str:
-    return _build.LIBGST
+    # GST_LIB selects another build of the same ABI (the CPU build used by tests)
+    return os.environ.get("GST_LIB", _build.LIBGST)
 
 
 def load():

@@ -976,22 +976,24 @@ __device__ void phase_b(Slot* slots, Ctrl* ctrl, uint64_t V, uint32_t* sh_items,
   }
 }
 
-// One engine step: every wave pulls work items of the current plan until it
-// is exhausted.  There is no waiting between blocks; the step ends when the
-// kernel ends, and the next plan kernel sees everything it wrote.
-__global__ __launch_bounds__(kBlock) void gst_step_kernel(GraphDev g, Slot* slots, Ctrl* ctrl) {
-  __shared__ WaveLDS lds[kWaves];
-  const int wid = threadIdx.x >> 6;
+// One engine step for one wave: pull work items of the current plan until it
+// is exhausted.
+__device__ void step_body(const GraphDev& g, Slot* slots, Ctrl* ctrl, WaveLDS* L) {
   const int lane = lane_id();
-  WaveLDS* L = &lds[wid];
   const uint64_t V = (uint64_t)g.V;
   if (ctrl->exit_flag) return;
   const uint32_t total = ctrl->item_total;
   const uint32_t nact = ctrl->n_active;
   for (;;) {
-    uint32_t it = 0;
-    if (lane == 0) it = atomicAdd(&ctrl->item_next, 1u);
-    it = __shfl(it, 0, 64);
+    // Claim the next item for the whole wave.  Every lane executes the atomic
+    // (only lane 0 adds one) and the index is read from lane 0 into a scalar
+    // register, so the claim sits under no lane-dependent branch and the loop
+    // test is uniform.  With the claim under `if (lane == 0)`, the optimizer
+    // threads the lane != 0 edge of the `if (lane == 0)` blocks that end
+    // several items straight to the broadcast: the loop splits in two and
+    // lanes 1..63 re-run the same item without lane 0, never finishing.
+    const uint32_t mine = atomicAdd(&ctrl->item_next, lane == 0 ? 1u : 0u);
+    const uint32_t it = __builtin_amdgcn_readfirstlane(mine);
     if (it >= total) break;
     uint32_t lo = 0, hi = nact;  // last active slot whose prefix <= it
     while (hi - lo > 1) {
@@ -1022,6 +1024,15 @@ __global__ __launch_bounds__(kBlock) void gst_step_kernel(GraphDev g, Slot* slot
   }
 }
 
+#ifndef GST_CPU_SIM
+// One engine step: every wave pulls work items until the plan is exhausted.
+// There is no waiting between blocks; the step ends when the kernel ends, and
+// the next plan kernel sees everything it wrote.
+__global__ __launch_bounds__(kBlock) void gst_step_kernel(GraphDev g, Slot* slots, Ctrl* ctrl) {
+  __shared__ WaveLDS lds[kWaves];
+  step_body(g, slots, ctrl, &lds[threadIdx.x >> 6]);
+}
+
 // Between two steps, one block advances every slot's state machine and
 // publishes the next step's work items.  Once it has set exit_flag it leaves
 // everything untouched, so the host may enqueue more steps than are needed.
@@ -1031,9 +1042,11 @@ __global__ __launch_bounds__(kBlock) void gst_plan_kernel(Slot* slots, Ctrl* ctr
   if (ctrl->exit_flag) return;
   phase_b(slots, ctrl, V, sh_items, sh_scan);
 }
+#endif
 
 }  // namespace gst
 
+#ifndef GST_CPU_SIM
 extern "C" hipError_t gst_launch_step(const gst::GraphDev* g, gst::Slot* slots, gst::Ctrl* ctrl, int grid,
                                       hipStream_t stream) {
   hipLaunchKernelGGL(gst::gst_step_kernel, dim3(grid), dim3(gst::kBlock), 0, stream, *g, slots, ctrl);
@@ -1049,3 +1062,4 @@ extern "C" hipError_t gst_step_occupancy(int* blocks_per_cu) {
   return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, (const void*)gst::gst_step_kernel,
                                                        gst::kBlock, 0);
 }
+#endif  // GST_CPU_SIM

@@ -22,9 +22,9 @@
  * GSTnonHop.cu:435-440).  No C++ types cross this boundary.
  *
  * Threading: a graph handle is immutable after gst_groups_set and may be
- * shared by threads; device work from one process is serialised on the
- * device (one persistent engine kernel at a time), so concurrency comes from
- * gst_solve_batch and the server, which run many queries inside one kernel.
+ * shared by threads; device work from one graph handle is serialised (one
+ * engine at a time), so concurrency comes from gst_solve_batch and the
+ * server, which advance many queries in every engine step.
  */
 #ifndef GST_H_
 #define GST_H_
@@ -119,7 +119,7 @@ gst_status gst_groups_set(gst_graph *g, const int64_t *grp_ptr,
 gst_status gst_solve(gst_graph *g, const int32_t *group_ids, int32_t k,
                      int32_t D, gst_result *out);
 
-/* Many queries through one persistent engine kernel with memory-based
+/* Many queries advanced together in shared engine steps, with memory-based
  * admission (continuous batching).  out[i] answers q[i]. */
 gst_status gst_solve_batch(gst_graph *g, int64_t n, const gst_query *q,
                            gst_result *out);
@@ -127,7 +127,9 @@ gst_status gst_solve_batch(gst_graph *g, int64_t n, const gst_query *q,
 void gst_result_free(gst_result *r);
 
 /* Tunables: "max_slots", "budget_mb", "blocks_per_cu", "chunk",
- * "edge_cap", "timeout_ms".  Unknown keys return GST_INVALID_ARG. */
+ * "edge_cap", "timeout_ms", "graph" (replay engine steps as a hipGraph).
+ * Read-only: "num_cu", "occupancy", "grid", "{plain,batch}_{kernel_ns,
+ * launches,steps}".  Unknown keys return GST_INVALID_ARG. */
 gst_status gst_set_option(gst_graph *g, const char *key, int64_t value);
 gst_status gst_get_option(gst_graph *g, const char *key, int64_t *value);
 

@@ -1,0 +1,59 @@
+// Kernel launches of the CPU build of libgst: the engine's step and plan
+// kernels run as host threads (test infrastructure).
+#define GST_CPU_SIM 1
+#define GST_CPU_SIM_HOST 1
+#include "gst_device.hip"
+
+#include <thread>
+#include <vector>
+
+namespace gsim {
+thread_local int lane = 0;
+thread_local Wave* wave = nullptr;
+thread_local Block* block = nullptr;
+thread_local ThreadIdx thread_idx = {0};
+thread_local uint64_t collectives = 0;
+}  // namespace gsim
+
+extern "C" hipError_t gst_launch_plan(gst::Slot* slots, gst::Ctrl* ctrl, uint64_t V, hipStream_t) {
+  if (ctrl->exit_flag) return hipSuccess;
+  static uint32_t sh_items[gst::kMaxSlots];
+  static uint32_t sh_scan[2 * gst::kBlock];
+  gsim::Block blk;
+  std::vector<gsim::Wave> waves(gst::kWaves);
+  std::vector<std::thread> th;
+  for (int t = 0; t < gst::kBlock; t++)
+    th.emplace_back([&, t]() {
+      gsim::lane = t & 63;
+      gsim::wave = &waves[t >> 6];
+      gsim::block = &blk;
+      gsim::thread_idx.x = (unsigned)t;
+      gst::phase_b(slots, ctrl, V, sh_items, sh_scan);
+    });
+  for (auto& x : th) x.join();
+  return hipSuccess;
+}
+
+extern "C" hipError_t gst_launch_step(const gst::GraphDev* g, gst::Slot* slots, gst::Ctrl* ctrl, int grid,
+                                      hipStream_t) {
+  const int n_waves = grid * gst::kWaves;
+  std::vector<gsim::Wave> waves(n_waves);
+  std::vector<gst::WaveLDS> lds(n_waves);
+  std::vector<std::thread> th;
+  const gst::GraphDev gd = *g;
+  for (int w = 0; w < n_waves; w++)
+    for (int l = 0; l < 64; l++)
+      th.emplace_back([&, w, l]() {
+        gsim::lane = l;
+        gsim::wave = &waves[w];
+        gsim::thread_idx.x = (unsigned)((w % gst::kWaves) * 64 + l);
+        gst::step_body(gd, slots, ctrl, &lds[w]);
+      });
+  for (auto& x : th) x.join();
+  return hipSuccess;
+}
+
+extern "C" hipError_t gst_step_occupancy(int* blocks_per_cu) {
+  *blocks_per_cu = 1;
+  return hipSuccess;
+}

@@ -1,0 +1,105 @@
+"""Code-generation checks of the gfx950 engine kernels (CPU: hipcc cross-compiles).
+
+Regression test for the work-claim loop of gst_step_kernel.  Each wave claims
+items with one atomic on Ctrl::item_next (offset 8) and broadcasts the index to
+its 64 lanes.  When the claim was written as `if (lane == 0) it = atomicAdd();
+it = __shfl(it, 0)`, the optimizer threaded the lane != 0 edge of the
+`if (lane == 0)` blocks that end several items directly to the broadcast.  The
+machine code then had the claim in an outer loop and the broadcast at the head
+of an inner loop around all items; lanes 1..63 looped in the inner loop,
+broadcast without lane 0 and re-ran the same item without end (the engine
+step never finished).  The checks below pin the fixed shape: the claim and the
+test on the claimed index sit in one loop, with a scalar (uniform) exit branch,
+and no loop nested in the item loop holds all the items.
+"""
+import os
+import re
+import shutil
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CSRC = os.path.join(ROOT, "gpugst_amd", "csrc")
+HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+STEP = "_ZN3gst15gst_step_kernelENS_8GraphDevEPNS_4SlotEPNS_4CtrlE"
+
+
+@pytest.fixture(scope="module")
+def step_asm(tmp_path_factory):
+    if not os.path.exists(HIPCC):
+        pytest.skip("hipcc not available")
+    out = str(tmp_path_factory.mktemp("asm") / "dev.s")
+    cmd = [HIPCC, "-O3", "-std=c++17", "--offload-arch=gfx950", "-Xclang", "-target-feature",
+           "-Xclang", "-packed-fp32-ops", "-I", os.path.join(ROOT, "include"), "-I", CSRC,
+           "-x", "hip", "--cuda-device-only", "-S", os.path.join(CSRC, "gst_device.hip"), "-o", out]
+    subprocess.run(cmd, check=True, capture_output=True)
+    text = open(out).read()
+    start = text.index(STEP + ":")
+    end = text.index(".Lfunc_end0", start)
+    return text[start:end].splitlines()
+
+
+def _blocks(lines):
+    """(label, innermost loop header, depth) of the block each line belongs to."""
+    cur = ("entry", None, 0)
+    out = []
+    for i, ln in enumerate(lines):
+        m = re.match(r"^(\.LBB\d+_\d+|; %bb\.\d+):?\s*(;.*)?$", ln.strip())
+        if m:
+            label = m.group(1).lstrip("; %").lstrip(".")
+            note = (m.group(2) or "") + " " + (lines[i + 1] if i + 1 < len(lines) else "")
+            h = re.search(r"Loop Header: Depth=(\d+)", note)
+            inl = re.search(r"in Loop: Header=(BB\d+_\d+) Depth=(\d+)", note)
+            if h:
+                cur = (label, label.replace("LBB", "BB"), int(h.group(1)))
+            elif inl:
+                cur = (label, inl.group(1), int(inl.group(2)))
+            else:
+                cur = (label, None, 0)
+        out.append(cur)
+    return out
+
+
+def test_claim_and_broadcast_share_one_loop(step_asm):
+    lines = step_asm
+    blk = _blocks(lines)
+    claims = [i for i, ln in enumerate(lines)
+              if "global_atomic_add" in ln and "offset:8" in ln and "sc0" in ln]
+    assert len(claims) == 1, claims
+    c = claims[0]
+    # the first scalar conditional branch after the claim is the loop test;
+    # the claimed index reaches it through v_readfirstlane and a scalar compare
+    br = next(i for i in range(c, len(lines)) if "s_cbranch_scc" in lines[i])
+    seg = lines[c:br]
+    assert any("v_readfirstlane_b32" in ln for ln in seg), seg
+    assert any(re.search(r"s_cmp_(ge|lt|le|gt)_u32", ln) for ln in seg), seg
+    _, hc, dc = blk[c]
+    _, hb, db = blk[br]
+    assert (hc, dc) == (hb, db), (blk[c], blk[br])
+    assert dc == 1, blk[c]
+
+
+def test_no_inner_loop_wraps_the_items(step_asm):
+    # before the fix, one depth-2 loop enclosed every item (depth up to 7);
+    # now the items' own loops hang directly off the item loop
+    lines = step_asm
+    head = next(i for i, ln in enumerate(lines) if "This Loop Header: Depth=1" in ln)
+    kids = []
+    for ln in lines[head + 1:]:
+        m = re.match(r"^\s*;\s+Child Loop (BB\d+_\d+) Depth (\d+)", ln)
+        if not m:
+            break
+        kids.append((m.group(1), int(m.group(2))))
+    depth2 = [k for k, d in kids if d == 2]
+    assert len(depth2) >= 8, kids  # one or more per item kind
+    # no depth-2 loop holds more than a fraction of the nest
+    per = {}
+    cur = None
+    for k, d in kids:
+        if d == 2:
+            cur = k
+            per[cur] = 0
+        elif cur:
+            per[cur] += 1
+    assert max(per.values()) < len(kids) // 2, per
