@@ -1,27 +1,29 @@
 """Benchmark: optimal-GST queries/sec on one MI355X, batched vs plain.
 
-Workload (one "step" = one batch of B queries through the persistent engine,
-inputs resident in HBM): the seeded R-MAT graph of 1.1M vertices / 3.0M
-undirected edges (a=.57, b=c=.19, weights U[1,100]) with 500k geometric groups
-(mean 6), queries of k = 5 distinct group ids, no diameter bound.  The plain
-path answers the same queries one gst_solve call at a time.
+Workload: the 1.1M-vertex R-MAT graph (a=0.57, b=c=0.19, ~3.0M undirected
+edges, int32 weights uniform in [1,100]) with 500,000 groups (geometric sizes,
+mean 6), queries of k = 5 distinct group ids, no diameter bound.  The plain
+path answers queries one call at a time (gst_solve); the batched path hands
+`--batch` queries per step to gst_solve_batch, which runs them together in
+shared engine steps: one group-distance job per up to 256 labels (all the
+batch's groups in one multi-label SSSP) and the per-query DP on the live set.
 
 Prints one JSON line.  ``value`` is the batched throughput; the plain
-throughput, their ratio, latency percentiles, cost parity, the roofline of the
-engine kernel and the CPU oracle baseline ride along.
-"""
-from __future__ import annotations
+throughput, latency percentiles, parity checks (plain vs batched costs, tree
+validation, the CPU oracle on a bounded sample), the roofline of the dominant
+step kernel and the CPU baseline ride along in the same object.
 
+Roofline: one extra batch (not in the timed steps) runs with every step's
+lean and heavy kernels bracketed by HIP events on the engine stream; the
+dominant kernel's average launch time and the algorithmic bytes its items
+moved (from the device counters, model below) give `achieved`.
+"""
 import argparse
 import json
-import os
 import sys
 import time
 
 import numpy as np
-
-ROOT = os.path.dirname(os.path.abspath(__file__))
-sys.path.insert(0, ROOT)
 
 METRIC = "optimal-GST queries/sec + p50/p99 latency, batched vs plain, 1 MI355X"
 HBM_PEAK_GBS = 8000.0
@@ -35,21 +37,44 @@ def pct(x, q):
     return float(np.percentile(np.asarray(x, dtype=np.float64), q)) if len(x) else None
 
 
-def algorithmic_bytes(res_list, V, k, ND=1):
-    """Bytes the DP has to move for these queries, from the per-query device
-    counters: per frontier entry 24 B (list id, arc offset, dirty mask, row
-    pointer pair), per scanned arc 8 B (column + weight), per relaxation
-    candidate 16 B (source and target state), per improvement 8 B (the packed
-    atomic); per query 8 B per state to initialise, 8 B per state per merge
-    pass (k - 1 passes read every vertex segment) and the level-1 distance
-    table (4 B per vertex and group)."""
+def lean_bytes(g, res_list, V, k):
+    """Algorithmic bytes of the lean kernel's items for these queries (one
+    group-distance job per up to 256 labels):
+      job FILL        4 B per (vertex, label) + 24 B per vertex (3 masks)
+      job ROUND       per frontier entry 24 B (list id, arc offset, mask, row
+                      pointer) + 12 B clear; per arc 8 B (column + weight);
+                      per 4-label chunk relaxed 16 B (target chunk read); per
+                      improved chunk 16 B (write back)
+      job SCATTER     4 B per (vertex, label) read + 4 B per (vertex, label) written
+      query FILL      24 B per vertex (3 masks) + 1/8 B live flags"""
+    labels = k * len(res_list)
+    jobs = max(1, g.get_option("batch_jobs"))
+    lp = labels + 3 * jobs  # rows padded to whole 4-label chunks
+    ent = g.get_option("batch_job_entries")
+    arcs = g.get_option("batch_job_arcs")
+    relax = g.get_option("batch_job_relaxations")
+    impr = g.get_option("batch_job_improvements")
+    b = 4 * V * lp + 24 * V * jobs
+    b += 36 * ent + 8 * arcs + 16 * (relax // 4) + 16 * impr
+    b += 4 * V * lp + 4 * V * labels
+    b += (24 * V + V // 8) * len(res_list)
+    return b
+
+
+def heavy_bytes(res_list, V, k, ND=1):
+    """Algorithmic bytes of the heavy kernel's items: per query the live test
+    (C row, 4 B per vertex and group), T segments of live vertices (8 B per
+    state: initialised once, read by every merge level and FINAL), and the
+    level >= 2 grow rounds (24 B per frontier entry, 8 B per arc, 16 B per
+    relaxation candidate, 8 B per improvement) -- the counters of a query fed
+    by a job hold only these rounds."""
     W = 1 << k
     total = 0
     for r in res_list:
         c = r.counters
-        total += (24 * c["frontier_entries"] + 8 * c["arcs"] + 16 * c["relaxations"]
-                  + 8 * c["improvements"])
-        total += 8 * V * W * ND * (1 + max(k - 1, 0)) + 4 * V * k
+        nl = c.get("live_vertices", 0)
+        total += 4 * V * k + nl * W * ND * 8 * (1 + max(k - 1, 1))
+        total += 24 * c["frontier_entries"] + 8 * c["arcs"] + 16 * c["relaxations"] + 8 * c["improvements"]
     return total
 
 
@@ -65,6 +90,7 @@ def main():
     ap.add_argument("--scale", type=float, default=1.0, help="shrink the graph (tests only)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU oracle sample budget")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--max-slots", type=int, default=0)
     args = ap.parse_args()
 
@@ -79,7 +105,8 @@ def main():
     t0 = time.time()
     inst = data.make_config(3, scale=args.scale, nq=1)
     V = inst.V
-    n_batched = (args.warmup + args.steps) * args.batch
+    n_roof = 0 if args.no_roofline else args.batch
+    n_batched = (args.warmup + args.steps) * args.batch + n_roof
     nq = max(n_batched, args.plain)
     queries = data.make_queries(nq, args.k, inst.meta["eligible"], 3000 + 4 + 10 * args.k + 1)
     log(f"graph V={V} arcs={inst.col.shape[0]} groups={inst.gptr.shape[0]-1} "
@@ -92,18 +119,20 @@ def main():
     Ds = [args.D] * nq
 
     # ---------------- plain: one query per call
-    g.solve(queries[0], D=args.D)  # warm-up (allocates the single-query arena)
-    g.set_option("reset_stats", 0)
     plain_res, plain_lat = [], []
-    tp = time.perf_counter()
-    for i in range(args.plain):
-        t = time.perf_counter()
-        plain_res.append(g.solve(queries[i], D=args.D))
-        plain_lat.append((time.perf_counter() - t) * 1e3)
-    plain_s = time.perf_counter() - tp
-    plain_qps = args.plain / plain_s
-    log(f"plain: {args.plain} queries {plain_s:.3f}s -> {plain_qps:.1f} q/s "
-        f"p50 {pct(plain_lat, 50):.2f} ms p99 {pct(plain_lat, 99):.2f} ms")
+    plain_qps = None
+    if args.plain > 0:
+        g.solve(queries[0], D=args.D)  # warm-up (allocates the single-query arena)
+        g.set_option("reset_stats", 0)
+        tp = time.perf_counter()
+        for i in range(args.plain):
+            t = time.perf_counter()
+            plain_res.append(g.solve(queries[i], D=args.D))
+            plain_lat.append((time.perf_counter() - t) * 1e3)
+        plain_s = time.perf_counter() - tp
+        plain_qps = args.plain / plain_s
+        log(f"plain: {args.plain} queries {plain_s:.3f}s -> {plain_qps:.1f} q/s "
+            f"p50 {pct(plain_lat, 50):.2f} ms p99 {pct(plain_lat, 99):.2f} ms")
 
     # ---------------- batched: warmup + timed steps
     bres = []
@@ -128,12 +157,38 @@ def main():
     kern_ns = g.get_option("batch_kernel_ns")
     launches = g.get_option("batch_launches")
     log(f"batched: {len(timed)} queries {batch_s:.3f}s -> {batch_qps:.1f} q/s; "
-        f"kernel {kern_ns/1e6:.1f} ms in {launches} launches")
+        f"device {kern_ns/1e6:.1f} ms in {launches} epochs")
+
+    # ---------------- roofline pass: one more batch with per-kernel event timing
+    roof = None
+    if n_roof:
+        a = (args.warmup + args.steps) * args.batch
+        qs = queries[a:a + args.batch]
+        g.set_option("reset_stats", 0)
+        g.set_option("time_steps", 1)
+        rr = g.solve_batch(list(qs), Ds[:len(qs)])
+        g.set_option("time_steps", 0)
+        n_steps = max(1, g.get_option("batch_timed_steps"))
+        lean_ms = g.get_option("batch_lean_ns") / 1e6 / n_steps
+        heavy_ms = g.get_option("batch_heavy_ns") / 1e6 / n_steps
+        lb = lean_bytes(g, rr, V, args.k) if args.D < 0 else None
+        hb = heavy_bytes(rr, V, args.k, 1 if args.D < 0 else args.D + 1)
+        dom = "gst_lean_kernel" if lean_ms >= heavy_ms else "gst_heavy_kernel"
+        ms, byts = (lean_ms, lb) if dom == "gst_lean_kernel" else (heavy_ms, hb)
+        achieved = byts / n_steps / (ms * 1e-3) / 1e9 if (byts and ms > 0) else 0.0
+        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                "algorithmic_bytes_per_launch": int(byts / n_steps) if byts else None,
+                "avg_launch_ms": round(ms, 4), "launches": int(n_steps),
+                "other_kernel_avg_ms": round(heavy_ms if dom == "gst_lean_kernel" else lean_ms, 4),
+                "pass": f"{len(rr)} further queries of the same workload, events around every step kernel"}
+        log(f"roofline pass: {n_steps} steps, lean {lean_ms:.3f} ms/launch, heavy {heavy_ms:.3f} ms/launch, "
+            f"{dom} {achieved:.1f} GB/s algorithmic")
 
     # ---------------- parity: plain vs batched on shared queries, trees valid
     all_b = bres + timed
     mism = 0
-    for i in range(min(args.plain, len(all_b))):
+    for i in range(min(len(plain_res), len(all_b))):
         if plain_res[i].cost != all_b[i].cost:
             mism += 1
     bad_trees = 0
@@ -170,14 +225,6 @@ def main():
                          f"single thread, {cs:.1f} s"}
         log(f"cpu oracle: {n} queries in {cs:.1f}s, mismatches {oracle_mism}")
 
-    # ---------------- roofline of the engine kernel over the timed steps
-    alg = algorithmic_bytes(timed, V, args.k, 1 if args.D < 0 else args.D + 1)
-    achieved = alg / (kern_ns * 1e-9) / 1e9 if kern_ns else 0.0
-    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "gst_engine_kernel", "algorithmic_bytes_per_launch": int(alg / max(launches, 1)),
-            "avg_launch_ms": round(kern_ns / 1e6 / max(launches, 1), 3)}
-
     lat = [r.latency_ms for r in timed]
     dev = [r.device_ms for r in timed]
     out = {
@@ -197,7 +244,7 @@ def main():
                                f"batched, {args.batch} queries/step",
                    "V": V, "arcs": int(inst.col.shape[0]), "groups": int(inst.gptr.shape[0] - 1),
                    "k": args.k, "D": args.D, "batch": args.batch, "scale": args.scale},
-        "plain_qps": round(plain_qps, 2),
+        "plain_qps": round(plain_qps, 2) if plain_qps else None,
         "batched_vs_plain": round(batch_qps / plain_qps, 2) if plain_qps else None,
         "latency_ms": {"plain_p50": pct(plain_lat, 50), "plain_p99": pct(plain_lat, 99),
                        "batched_device_p50": pct(dev, 50), "batched_device_p99": pct(dev, 99),

@@ -27,6 +27,9 @@ constexpr int kMaxSlots = 1024;         // concurrently admitted queries per eng
 constexpr uint32_t kPredSeed = 0u;
 constexpr uint32_t kPredGrow = 1u << 30;
 constexpr uint32_t kPredMerge = 2u << 30;
+// singleton state read from the distance table C (no stored predecessor):
+// reconstruction finds the next edge of a shortest path by scanning the row
+constexpr uint32_t kPredPath = 3u << 30;
 constexpr uint32_t kArcMask = (1u << 30) - 1;
 
 // block shape and work-item granularities of the engine kernel
@@ -38,6 +41,20 @@ constexpr uint32_t kDistTV = 256;       // vertices per DIST item
 constexpr uint32_t kClearPer = 1024;    // mask clears per CLEAR item
 constexpr int kStageWords = 1152;       // per-wave LDS staging (u64): 64*ND rows or the W*k*k route table
 
+// Group-distance jobs: one multi-label SSSP per batch of unbounded queries.
+// Labels are (query, group) pairs; the distance row of a vertex holds all of
+// them (u32, padded to whole 4-label chunks), so an arc relaxation reads and
+// writes coalesced 16-byte chunks shared by every query of the job.
+constexpr int kJobLabels = 256;         // labels per job: 64 chunks, one dirty bit each
+constexpr uint32_t kJobIter = 64;       // wave iterations per job ROUND item
+constexpr uint32_t kScatterTV = 64;     // vertices per SCATTER item
+
+// work classes: each has its own kernel, so the arc-parallel items do not run
+// at the register budget (and occupancy) of the merge items
+constexpr int kClsLean = 0;   // FILL, SEED, job ROUND, SCATTER, CLEAR
+constexpr int kClsHeavy = 1;  // query ROUND, DIST, LIVE, MERGE, FINAL, RECON
+constexpr int kNumCls = 2;
+
 enum Phase : int32_t {
   PH_FREE = 0,
   PH_FILL = 1,   // initialise T, masks, distance table
@@ -48,7 +65,14 @@ enum Phase : int32_t {
   PH_FINAL = 6,  // full-set merge + argmin
   PH_RECON = 7,  // predecessor chase -> tree edges
   PH_DONE = 8,
-  PH_LIVE = 9    // live-vertex list + initialisation of their T segments
+  PH_LIVE = 9,   // live-vertex list + initialisation of their T segments
+  PH_WAITJ = 10, // query waiting for its group-distance job (no work items)
+  PH_SCATTER = 11  // job: per-query distance tables C and star bounds
+};
+
+enum SlotKind : int32_t {
+  KIND_QUERY = 0,
+  KIND_JOB = 1
 };
 
 struct GraphDev {
@@ -67,14 +91,26 @@ struct EdgeDev {
 };
 
 struct alignas(64) Slot {
-  // ---- query (set at admission) ----
+  // ---- query or job (set at admission) ----
+  int32_t kind;         // KIND_QUERY or KIND_JOB
+  int32_t job;          // query fed by a job: its slot index, else -1
+  int32_t lab;          // query fed by a job: its first label in the job
+  uint32_t jdone;       // set by the job when this query's C is complete
   int32_t k, W, ND, D;  // D < 0: unbounded (ND == 1)
   int32_t qg[kMaxK];
   int32_t edge_cap;
   int32_t chunk;        // arcs per ROUND item
+  uint32_t istride;     // entries per buffer of istart
+  int32_t L, Lp, G;     // job: labels, padded row length, lanes per arc (power of two)
+  int32_t nmem;         // job: member queries
+  int32_t pad0;
   uint64_t nT;          // V * ND * W
   uint64_t* T;          // states with |p| >= 2; only live vertices' segments are touched
-  uint64_t* S1;         // V * ND * k singleton states T(v, {i}, d) = hop-bounded distances
+  uint64_t* S1;         // V * ND * k singleton states T(v, {i}, d); null when C comes from a job
+  uint32_t* istart;     // 3 * istride: frontier index holding the first arc of each ROUND item
+  uint32_t* dist;       // job: V * Lp distances
+  int32_t* lgrp;        // job: group id of each label
+  uint32_t* jmem;       // job: member slot indices
   uint32_t* live;       // V: vertices that can lie on a tree cheaper than the bound
   uint8_t* lvf;         // V: live flag per vertex
   uint64_t* mask;       // 3 * V (triple-buffered dirty sub-masks)
@@ -87,7 +123,8 @@ struct alignas(64) Slot {
   // ---- engine state ----
   int32_t phase, level;
   uint32_t step, in_step;
-  uint32_t n_work, n_items;
+  uint32_t n_work, n_clr;   // this step's phase items and CLEAR items
+  int32_t cls, pad2;        // work class of the phase items
   uint64_t app[3];      // (count << 32) | arcs of the three frontier buffers
   uint32_t best;        // upper bound on the optimum (cost of a real walk-tree)
   uint32_t status;
@@ -100,20 +137,17 @@ struct alignas(64) Slot {
 };
 
 struct alignas(64) Ctrl {
-  uint32_t arrive;
-  uint32_t gen;
-  uint32_t item_next;
-  uint32_t item_total;
-  uint32_t n_active;
+  uint32_t item_next[kNumCls];
+  uint32_t item_total[kNumCls];
+  uint32_t n_active[kNumCls];
   uint32_t exit_flag;
-  uint32_t abort_flag;
   uint32_t finished;
   uint32_t steps;
   uint32_t exit_after;
   uint32_t max_steps;
   uint32_t nslots;
-  uint32_t active[kMaxSlots];
-  uint32_t prefix[kMaxSlots + 1];
+  uint32_t active[kNumCls][kMaxSlots];
+  uint32_t prefix[kNumCls][kMaxSlots + 1];
 };
 
 }  // namespace gst

@@ -1,15 +1,21 @@
 // gst_device.hip -- the DPBF engine kernels for gfx950 (CDNA4, wave64).
 //
 // Many queries ("slots") advance together.  One engine step is one launch of
-// gst_step_kernel: waves pull the step's work items (a prefix over slots) with
-// one atomic each.  Between steps the one-block gst_plan_kernel advances every
-// slot's state machine and publishes the next step's items.  Blocks never wait
-// for each other; every hand-off between blocks happens at a kernel boundary.
+// each work-class kernel (gst_lean_kernel: fill, seed and the group-distance
+// jobs' items; gst_heavy_kernel: the per-query DP items): waves pull the
+// step's items (a prefix over slots)
+// with one atomic each.  Between steps the one-block gst_plan_kernel advances
+// every slot's state machine and publishes the next step's items.  Blocks never
+// wait for each other; every hand-off between blocks happens at a kernel
+// boundary.
 //
 // Per query the DP is evaluated level by level in |p| (Dreyfus-Wagner order):
-//   level 1  : SEED + frontier rounds  -> T(v,{i},d) = hop-bounded / plain
-//              group->vertex distances (the reference's dis0_init_1 SSSP,
-//              code/GPUGST/src/DPQ.cu:241-271, and the singleton states)
+//   level 1  : group->vertex distances, either as SEED + frontier rounds of the
+//              query itself (hop-bounded DP: T(v,{i},d) per hop depth d; the
+//              reference's dis0_init_1 SSSP, code/GPUGST/src/DPQ.cu:241-271)
+//              or, for unbounded queries, from a group-distance job: one
+//              multi-label SSSP shared by a batch of queries (SCATTER hands each
+//              query its table C)
 //   DIST     : compact distance table C[v][i], inter-group distances vv,
 //              route table R (host loops of code/GPUGST/src/DPQ.cu:409-483
 //              moved on the device)
@@ -36,12 +42,18 @@
 
 namespace gst {
 
-struct WaveLDS {
+// per-wave LDS of the arc-parallel items: 64 staged frontier entries
+struct FrontLDS {
   uint32_t wv[64];
   uint32_t woff[64];
   uint32_t wrs[64];
   uint32_t pad[64];
   uint64_t wm[64];
+};
+
+// per-wave LDS of the vertex-segment items
+struct WaveLDS {
+  FrontLDS f;
   uint64_t stage[kStageWords];
 };
 
@@ -90,15 +102,26 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t x) {
 
 __device__ __forceinline__ uint32_t sat(uint32_t a) { return a < kInf ? a : kInf; }
 
-// Address of state (v, p, d): singletons in the compact S1 table, every other
-// subset in the vertex-major T segments.
+// State (v, p, d): singletons in the compact S1 table or, when the slot's
+// distances come from a group-distance job, synthesised from C with the
+// kPredPath predecessor; every other subset in the vertex-major T segments.
 struct States {
   uint64_t* T;
   uint64_t* S1;
+  const uint32_t* C;
   int k, ND, W;
+  // address of a state that is written (never a singleton of a C-fed slot)
   __device__ __forceinline__ uint64_t* at(uint32_t v, uint32_t p, uint32_t d) const {
     if ((p & (p - 1u)) == 0u) return S1 + ((uint64_t)v * ND + d) * (uint64_t)k + (uint32_t)(__ffs(p) - 1);
     return T + ((uint64_t)v * ND + d) * (uint64_t)W + p;
+  }
+  __device__ __forceinline__ uint64_t get(uint32_t v, uint32_t p, uint32_t d) const {
+    if ((p & (p - 1u)) == 0u) {
+      const uint32_t i = (uint32_t)(__ffs(p) - 1);
+      if (S1) return S1[((uint64_t)v * ND + d) * (uint64_t)k + i];
+      return ((uint64_t)C[(uint64_t)v * k + i] << 32) | kPredPath;
+    }
+    return T[((uint64_t)v * ND + d) * (uint64_t)W + p];
   }
 };
 
@@ -106,6 +129,7 @@ __device__ __forceinline__ States states_of(const Slot* sp) {
   States st;
   st.T = sp->T;
   st.S1 = sp->S1;
+  st.C = sp->C;
   st.k = sp->k;
   st.ND = sp->ND;
   st.W = sp->W;
@@ -148,8 +172,10 @@ __device__ __forceinline__ uint32_t lower_bound(const uint32_t* __restrict__ Cro
 
 // Wave-aggregated append of vertex u (when flag) to frontier buffer `buf`:
 // one 64-bit atomicAdd of (count << 32 | degree sum) per wave gives every
-// appended vertex both its list position and its arc offset, so the next
-// round can split its arcs into equal chunks without a separate scan.
+// appended vertex both its list position and its arc offset.  Each ROUND item
+// of the next round covers `chunk` arcs of that arc space; the vertex that
+// holds an item's first arc records its list position in istart, so the item
+// starts without a search.
 __device__ __forceinline__ void wave_append(bool flag, uint32_t u, Slot* sp, int buf,
                                             const GraphDev& g, uint64_t V) {
   const uint64_t ball = __ballot(flag);
@@ -169,40 +195,64 @@ __device__ __forceinline__ void wave_append(bool flag, uint32_t u, Slot* sp, int
   if (flag) {
     const uint32_t rank = __popcll(ball & __lanemask_lt());
     const uint32_t pos = (uint32_t)(old >> 32) + rank;
+    const uint32_t off = (uint32_t)old + excl;
     sp->list[(uint64_t)buf * V + pos] = u;
-    sp->loff[(uint64_t)buf * V + pos] = (uint32_t)old + excl;
+    sp->loff[(uint64_t)buf * V + pos] = off;
+    if (deg) {
+      const uint32_t ch = (uint32_t)sp->chunk;
+      uint32_t* ist = sp->istart + (uint64_t)buf * sp->istride;
+      for (uint32_t c = (off + ch - 1) / ch; c * ch < off + deg; c++) ist[c] = pos;
+    }
   }
 }
 
 // ---------------------------------------------------------------- items ----
 
+// u64 words that FILL initialises: job [distances][masks]; query [S1][masks]
+// [C][live flags] (S1 and C only when the query computes its own level 1)
+__device__ __forceinline__ void fill_regions(const Slot* sp, uint64_t V, uint64_t* nA, uint64_t* nM,
+                                             uint64_t* nC, uint64_t* nF) {
+  if (sp->kind == KIND_JOB) {
+    *nA = V * (uint64_t)sp->Lp / 2;
+    *nM = 3 * V;
+    *nC = 0;
+    *nF = 0;
+    return;
+  }
+  const bool own = sp->S1 != nullptr;
+  *nA = own ? V * (uint64_t)sp->ND * (uint64_t)sp->k : 0;
+  *nM = 3 * V;
+  *nC = own ? ((uint64_t)V * sp->k + 1) / 2 : 0;
+  *nF = (V + 7) / 8;
+}
+
 __device__ void item_fill(Slot* sp, uint32_t idx, uint64_t V) {
-  const uint64_t nT = V * (uint64_t)sp->ND * (uint64_t)sp->k;  // S1 words
-  const uint64_t nM = 3 * V;
-  const uint64_t nC = ((uint64_t)V * sp->k + 1) / 2;
-  const uint64_t nF = (V + 7) / 8;                              // live flags
-  const uint64_t total = nT + nM + nC + nF;
+  uint64_t nA, nM, nC, nF;
+  fill_regions(sp, V, &nA, &nM, &nC, &nF);
+  const uint64_t total = nA + nM + nC + nF;
   const uint64_t base = (uint64_t)idx * kFillWords;
   const uint64_t end = base + kFillWords < total ? base + kFillWords : total;
-  uint64_t* T = sp->S1;
+  const bool job = sp->kind == KIND_JOB;
+  uint64_t* A = job ? (uint64_t*)sp->dist : sp->S1;
   uint64_t* M = sp->mask;
   uint64_t* C64 = (uint64_t*)sp->C;
   uint64_t* F64 = (uint64_t*)sp->lvf;
   const uint64_t cinf = ((uint64_t)kInf << 32) | kInf;
+  const uint64_t aval = job ? cinf : kInfState;
   for (uint64_t t = base + lane_id(); t < end; t += 64) {
-    if (t < nT)
-      T[t] = kInfState;
-    else if (t < nT + nM)
-      M[t - nT] = 0;
-    else if (t < nT + nM + nC)
-      C64[t - nT - nM] = cinf;
+    if (t < nA)
+      A[t] = aval;
+    else if (t < nA + nM)
+      M[t - nA] = 0;
+    else if (t < nA + nM + nC)
+      C64[t - nA - nM] = cinf;
     else
-      F64[t - nT - nM - nC] = 0;
+      F64[t - nA - nM - nC] = 0;
   }
 }
 
 __device__ void item_seed(const GraphDev& g, Slot* sp, uint32_t i) {
-  const int W = sp->W, ND = sp->ND;
+  const int ND = sp->ND;
   const int nxt = (sp->step + 1) % 3;
   const uint64_t V = (uint64_t)g.V;
   const int32_t grp = sp->qg[i];
@@ -228,7 +278,6 @@ __device__ void item_seed(const GraphDev& g, Slot* sp, uint32_t i) {
     vmin = wave_min_u32(vmin);
     if (lane_id() == 0 && vmin != 0xFFFFFFFFu) atomicMin((unsigned long long*)&sp->ans, (unsigned long long)vmin);
   }
-  (void)W;
 }
 
 __device__ void item_clear(Slot* sp, uint32_t idx, uint64_t V) {
@@ -241,9 +290,33 @@ __device__ void item_clear(Slot* sp, uint32_t idx, uint64_t V) {
   for (uint32_t t = base + lane_id(); t < end; t += 64) m[lst[t]] = 0;
 }
 
+// Stages the 64 frontier entries from list index i on: vertex, arc offset,
+// row start and dirty mask; entries past the end get offset arcs_total.
+__device__ __forceinline__ void stage_front(const GraphDev& g, FrontLDS* L, uint32_t i, uint32_t n,
+                                            uint32_t arcs_total, const uint32_t* lst, const uint32_t* lof,
+                                            const uint64_t* mcur) {
+  const int lane = lane_id();
+  const uint32_t idx = i + lane;
+  const bool valid = idx < n;
+  const uint32_t v = valid ? lst[idx] : 0u;
+  L->wv[lane] = v;
+  L->woff[lane] = valid ? lof[idx] : arcs_total;
+  L->wrs[lane] = valid ? (uint32_t)g.rowptr[v] : 0u;
+  L->wm[lane] = valid ? mcur[v] : 0ull;
+}
+
+// staged entry whose arc range holds arc position pos
+__device__ __forceinline__ int find_entry(const FrontLDS* L, uint32_t pos) {
+  int o = 0;
+#pragma unroll
+  for (int s = 32; s > 0; s >>= 1)
+    if (L->woff[o + s] <= pos) o += s;
+  return o;
+}
+
 // One chunk of arcs of the current frontier: grow T(v,p,d) -> T(u,p,d') for
 // every arc v->u in the chunk and every dirty subset p of v.
-__device__ void item_round(const GraphDev& g, Slot* sp, uint32_t chunk, WaveLDS* L) {
+__device__ void item_round(const GraphDev& g, Slot* sp, uint32_t chunk, FrontLDS* L) {
   const int lane = lane_id();
   const uint64_t V = (uint64_t)g.V;
   const int k = sp->k, W = sp->W, D = sp->D;
@@ -267,22 +340,10 @@ __device__ void item_round(const GraphDev& g, Slot* sp, uint32_t chunk, WaveLDS*
   const uint32_t* C = sp->C;
   const uint32_t* R = sp->R;
 
-  // last frontier entry whose arc range starts at or before a0
-  uint32_t lo = 0, hi = n;
-  while (hi - lo > 1) {
-    uint32_t mid = (lo + hi) >> 1;
-    if (lof[mid] <= a0) lo = mid; else hi = mid;
-  }
   uint64_t n_relax = 0, n_impr = 0;
   uint32_t a = a0;
-  for (uint32_t i = lo; a < a1; i += 64) {
-    const uint32_t idx = i + lane;
-    const bool valid = idx < n;
-    const uint32_t v = valid ? lst[idx] : 0u;
-    L->wv[lane] = v;
-    L->woff[lane] = valid ? lof[idx] : arcs_total;
-    L->wrs[lane] = valid ? (uint32_t)g.rowptr[v] : 0u;
-    L->wm[lane] = valid ? mcur[v] : 0ull;
+  for (uint32_t i = sp->istart[(uint64_t)cur * sp->istride + chunk]; a < a1; i += 64) {
+    stage_front(g, L, i, n, arcs_total, lst, lof, mcur);
     const uint32_t wend = (i + 64 < n) ? lof[i + 64] : arcs_total;
     wave_sync();
     const uint32_t hia = a1 < wend ? a1 : wend;
@@ -291,10 +352,7 @@ __device__ void item_round(const GraphDev& g, Slot* sp, uint32_t chunk, WaveLDS*
       bool app = false;
       uint32_t u = 0;
       if (pos < hia) {
-        int o = 0;
-#pragma unroll
-        for (int s = 32; s > 0; s >>= 1)
-          if (L->woff[o + s] <= pos) o += s;
+        const int o = find_entry(L, pos);
         const uint32_t vs = L->wv[o];
         const uint64_t m = L->wm[o];
         const uint32_t e = L->wrs[o] + (pos - L->woff[o]);
@@ -310,7 +368,7 @@ __device__ void item_round(const GraphDev& g, Slot* sp, uint32_t chunk, WaveLDS*
           const uint32_t x = full ^ p;
           uint32_t lbx = 0xFFFFFFFFu;
           for (int d = 0; d < dmax; d++) {
-            const uint32_t sc = (uint32_t)(*S.at(vs, p, d) >> 32);
+            const uint32_t sc = (uint32_t)(S.get(vs, p, d) >> 32);
             if (sc >= kInf) continue;
             const uint32_t cand = sc + w;
             if (cand >= kInf) continue;
@@ -337,7 +395,7 @@ __device__ void item_round(const GraphDev& g, Slot* sp, uint32_t chunk, WaveLDS*
                 uint32_t cbest = kInf;
                 const int d2max = hop ? (D - td) : 0;
                 for (int d2 = 0; d2 <= d2max; d2++) {
-                  const uint32_t c2 = (uint32_t)(*S.at(u, x, d2) >> 32);
+                  const uint32_t c2 = (uint32_t)(S.get(u, x, d2) >> 32);
                   cbest = c2 < cbest ? c2 : cbest;
                 }
                 if (cbest < kInf) {
@@ -367,6 +425,148 @@ __device__ void item_round(const GraphDev& g, Slot* sp, uint32_t chunk, WaveLDS*
   }
 }
 
+// ------------------------------------------------- group-distance jobs ----
+
+// Job SEED: distance 0 at every member of label l's group.
+__device__ void item_jseed(const GraphDev& g, Slot* sp, uint32_t l) {
+  const int nxt = (sp->step + 1) % 3;
+  const uint64_t V = (uint64_t)g.V;
+  const int32_t grp = sp->lgrp[l];
+  const int64_t beg = g.gptr[grp], end = g.gptr[grp + 1];
+  uint64_t* mn = sp->mask + (uint64_t)nxt * V;
+  const uint64_t Lp = (uint64_t)sp->Lp;
+  for (int64_t base = beg; base < end; base += 64) {
+    const int64_t t = base + lane_id();
+    bool app = false;
+    uint32_t v = 0;
+    if (t < end) {
+      v = (uint32_t)g.gmem[t];
+      sp->dist[(uint64_t)v * Lp + l] = 0u;
+      const unsigned long long old = atomicOr((unsigned long long*)&mn[v], 1ull << (l >> 2));
+      app = (old == 0);
+    }
+    wave_append(app, v, sp, nxt, g, V);
+  }
+}
+
+// one label of a job relaxation: target word tw holds t, candidate s + w
+__device__ __forceinline__ bool jrelax(uint32_t s, uint32_t t, uint32_t w, uint32_t* tw) {
+  if (s >= kInf) return false;
+  const uint32_t cand = s + w;
+  if (cand >= t || cand >= kInf) return false;
+  return atomicMin(tw, cand) > cand;
+}
+
+// Job ROUND: one chunk of arcs of the job's frontier.  G lanes share an arc;
+// lane r of the group relaxes the dirty 4-label chunks r, r + G, ... of the
+// source row into the target row (16-byte loads, coalesced over the group).
+__device__ void item_jround(const GraphDev& g, Slot* sp, uint32_t item, FrontLDS* L) {
+  const int lane = lane_id();
+  const uint64_t V = (uint64_t)g.V;
+  const uint32_t r = sp->step;
+  const int cur = r % 3, nxt = (r + 1) % 3;
+  const uint64_t appc = sp->app[cur];
+  const uint32_t n = (uint32_t)(appc >> 32);
+  const uint32_t arcs_total = (uint32_t)appc;
+  const uint32_t CH = (uint32_t)sp->chunk;
+  const uint32_t a0 = item * CH;
+  const uint32_t a1 = a0 + CH < arcs_total ? a0 + CH : arcs_total;
+  const uint32_t* lst = sp->list + (uint64_t)cur * V;
+  const uint32_t* lof = sp->loff + (uint64_t)cur * V;
+  const uint64_t* mcur = sp->mask + (uint64_t)cur * V;
+  uint64_t* mnxt = sp->mask + (uint64_t)nxt * V;
+  uint32_t* dist = sp->dist;
+  const uint64_t Lp = (uint64_t)sp->Lp;
+  const int nch = sp->Lp >> 2;
+  const int G = sp->G;
+  const int APW = 64 / G;
+  const int sub = lane / G, rr = lane - sub * G;
+  uint64_t n_relax = 0, n_impr = 0;
+  uint32_t a = a0;
+  for (uint32_t i = sp->istart[(uint64_t)cur * sp->istride + item]; a < a1; i += 64) {
+    stage_front(g, L, i, n, arcs_total, lst, lof, mcur);
+    const uint32_t wend = (i + 64 < n) ? lof[i + 64] : arcs_total;
+    wave_sync();
+    const uint32_t hia = a1 < wend ? a1 : wend;
+    for (uint32_t base = a; base < hia; base += (uint32_t)APW) {
+      const uint32_t pos = base + (uint32_t)sub;
+      uint64_t imp = 0;
+      uint32_t u = 0;
+      if (pos < hia) {
+        const int o = find_entry(L, pos);
+        const uint32_t vs = L->wv[o];
+        const uint64_t m = L->wm[o];
+        const uint32_t e = L->wrs[o] + (pos - L->woff[o]);
+        u = (uint32_t)g.col[e];
+        const uint32_t w = (uint32_t)g.wt[e];
+        const uint4* srow = (const uint4*)(dist + (uint64_t)vs * Lp);
+        const uint4* trow = (const uint4*)(dist + (uint64_t)u * Lp);
+        uint32_t* tw = dist + (uint64_t)u * Lp;
+        for (int c = rr; c < nch; c += G) {
+          if (!((m >> c) & 1ull)) continue;
+          const uint4 sv = srow[c];
+          const uint4 tv = trow[c];
+          bool im = jrelax(sv.x, tv.x, w, tw + 4 * c);
+          im |= jrelax(sv.y, tv.y, w, tw + 4 * c + 1);
+          im |= jrelax(sv.z, tv.z, w, tw + 4 * c + 2);
+          im |= jrelax(sv.w, tv.w, w, tw + 4 * c + 3);
+          n_relax += 4;
+          if (im) {
+            imp |= 1ull << c;
+            n_impr++;
+          }
+        }
+      }
+      // the improved chunks of one arc, gathered on the group's first lane
+      for (int o2 = 1; o2 < G; o2 <<= 1) imp |= (uint64_t)__shfl_xor((unsigned long long)imp, o2, 64);
+      bool app = false;
+      if (pos < hia && rr == 0 && imp) {
+        const unsigned long long om = atomicOr((unsigned long long*)&mnxt[u], (unsigned long long)imp);
+        app = (om == 0);
+      }
+      wave_append(app, u, sp, nxt, g, V);
+    }
+    a = hia;
+    wave_sync();
+  }
+  n_relax = wave_sum_u64(n_relax);
+  n_impr = wave_sum_u64(n_impr);
+  if (lane == 0) {
+    atomicAdd(&sp->ctr[3], (unsigned long long)n_relax);
+    atomicAdd(&sp->ctr[4], (unsigned long long)n_impr);
+  }
+}
+
+// Job SCATTER: each member query's distance table C (its k labels of every
+// row) and its star bound min_v sum_i c_i(v).  One lane per member.
+__device__ void item_jscatter(Slot* slots, Slot* sp, uint32_t tile, uint64_t V) {
+  const uint64_t v0 = (uint64_t)tile * kScatterTV;
+  const uint64_t v1 = v0 + kScatterTV < V ? v0 + kScatterTV : V;
+  const uint64_t Lp = (uint64_t)sp->Lp;
+  for (int mi = lane_id(); mi < sp->nmem; mi += 64) {
+    Slot* q = &slots[sp->jmem[mi]];
+    const int k = q->k;
+    const uint32_t* src = sp->dist + q->lab;
+    uint32_t* C = q->C;
+    uint32_t star = kInf;
+    for (uint64_t v = v0; v < v1; v++) {
+      const uint32_t* row = src + v * Lp;
+      uint32_t s = 0;
+      for (int i = 0; i < k; i++) {
+        const uint32_t c = row[i];
+        C[v * k + i] = c;
+        s = sat(s + c);
+      }
+      star = s < star ? s : star;
+    }
+    // star: shortest paths from one vertex to every group form a connected
+    // cover, so their sum bounds the optimum from above
+    if (star < kInf) atomicMin(&q->best, star);
+  }
+}
+
+// ------------------------------------------------------ per-query DIST ----
+
 // Distance table, inter-group distances and route table after level 1.
 __device__ void item_dist(const GraphDev& g, Slot* sp, uint32_t tile) {
   const uint64_t V = (uint64_t)g.V;
@@ -389,8 +589,6 @@ __device__ void item_dist(const GraphDev& g, Slot* sp, uint32_t tile) {
     star = s < star ? s : star;
   }
   if (!hop) {
-    // star: shortest paths from one vertex to every group form a connected
-    // cover, so their sum bounds the optimum from above
     star = wave_min_u32(star);
     if (lane_id() == 0 && star < kInf) atomicMin(&sp->best, star);
   }
@@ -399,7 +597,7 @@ __device__ void item_dist(const GraphDev& g, Slot* sp, uint32_t tile) {
 __device__ void item_vvr(const GraphDev& g, Slot* sp, WaveLDS* L) {
   const int lane = lane_id();
   const int k = sp->k, W = sp->W, ND = sp->ND;
-  uint32_t* vvl = (uint32_t*)L->wv;  // k*k <= 36 words
+  uint32_t* vvl = (uint32_t*)L->f.wv;  // k*k <= 36 words
   for (int i = 0; i < k; i++) {
     const int32_t grp = sp->qg[i];
     const int64_t beg = g.gptr[grp], end = g.gptr[grp + 1];
@@ -411,10 +609,17 @@ __device__ void item_vvr(const GraphDev& g, Slot* sp, WaveLDS* L) {
 #pragma unroll
       for (int j = 0; j < kMaxK; j++) {
         if (j >= k) continue;
-        for (int d = 0; d < ND; d++) {
-          const uint32_t c = (uint32_t)(sp->S1[((uint64_t)v * ND + d) * k + j] >> 32);
-          loc[j] = c < loc[j] ? c : loc[j];
+        uint32_t c = kInf;
+        if (sp->S1) {
+          // C is being written by this step's DIST tiles: read S1
+          for (int d = 0; d < ND; d++) {
+            const uint32_t t2 = (uint32_t)(sp->S1[((uint64_t)v * ND + d) * k + j] >> 32);
+            c = t2 < c ? t2 : c;
+          }
+        } else {
+          c = sp->C[(uint64_t)v * k + j];
         }
+        loc[j] = c < loc[j] ? c : loc[j];
       }
     }
 #pragma unroll
@@ -457,22 +662,16 @@ __device__ void item_vvr(const GraphDev& g, Slot* sp, WaveLDS* L) {
 }
 
 // Stages the W * ND segments of nv live vertices into LDS (row-major like T);
-// singleton entries are taken from S1, the unused p = 0 column is INF.
+// singleton entries are taken from S1 or C, the unused p = 0 column is INF.
 __device__ __forceinline__ void stage_rows(const Slot* sp, const uint32_t* ids, int nv, int rowlen,
-                                           int W, int ND, int k, uint64_t* stage) {
+                                           int W, uint64_t* stage) {
+  const States S = states_of(sp);
   for (int t = lane_id(); t < nv * rowlen; t += 64) {
     const int jj = t / rowlen, w = t - jj * rowlen;
     const uint32_t v = ids[jj];
     const int d = w / W;
     const uint32_t p = (uint32_t)(w - d * W);
-    uint64_t val;
-    if (p == 0)
-      val = kInfState;
-    else if ((p & (p - 1u)) == 0u)
-      val = sp->S1[((uint64_t)v * ND + d) * (uint64_t)k + (uint32_t)(__ffs(p) - 1)];
-    else
-      val = sp->T[(uint64_t)v * rowlen + w];
-    stage[t] = val;
+    stage[t] = (p == 0) ? kInfState : S.get(v, p, (uint32_t)d);
   }
 }
 
@@ -507,12 +706,12 @@ __device__ void item_live(Slot* sp, uint32_t tile, uint64_t V, WaveLDS* L) {
     base = __shfl(base, leader, 64);
     if (lv) {
       sp->live[base + rank] = (uint32_t)v;
-      L->wv[rank] = (uint32_t)v;
+      L->f.wv[rank] = (uint32_t)v;
     }
     wave_sync();
     for (int t = lane; t < n * rowlen; t += 64) {
       const int jj = t / rowlen;
-      sp->T[(uint64_t)L->wv[jj] * rowlen + (t - jj * rowlen)] = kInfState;
+      sp->T[(uint64_t)L->f.wv[jj] * rowlen + (t - jj * rowlen)] = kInfState;
     }
     wave_sync();
   }
@@ -543,7 +742,7 @@ __device__ void item_merge(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* 
   uint64_t n_eval = 0;
   for (uint64_t vb = v0; vb < v1; vb += VPW) {
     const int nv = (int)((v1 - vb) < (uint64_t)VPW ? (v1 - vb) : (uint64_t)VPW);
-    stage_rows(sp, sp->live + vb, nv, rowlen, W, ND, k, L->stage);
+    stage_rows(sp, sp->live + vb, nv, rowlen, W, L->stage);
     wave_sync();
     const uint32_t v = (j < nv) ? sp->live[vb + j] : 0u;
     const bool act = (j < nv) && p != 0 && __popc(p) == level;
@@ -629,7 +828,7 @@ __device__ void item_merge(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* 
 // Full-set merge: W/2 lanes per vertex, one lane per split (sub contains bit 0).
 __device__ void item_final(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* L) {
   const int lane = lane_id();
-  const int k = sp->k, W = sp->W, ND = sp->ND, D = sp->D;
+  const int W = sp->W, ND = sp->ND, D = sp->D;
   const bool hop = D >= 0;
   const int Dlim = hop ? D : 0;
   const uint32_t full = (uint32_t)W - 1;
@@ -649,7 +848,7 @@ __device__ void item_final(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* 
   uint64_t n_eval = 0;
   for (uint64_t vb = v0; vb < v1; vb += VPF) {
     const int nv = (int)((v1 - vb) < (uint64_t)VPF ? (v1 - vb) : (uint64_t)VPF);
-    stage_rows(sp, sp->live + vb, nv, rowlen, W, ND, k, L->stage);
+    stage_rows(sp, sp->live + vb, nv, rowlen, W, L->stage);
     wave_sync();
     const bool vact = j < nv;
     const uint32_t v = vact ? sp->live[vb + j] : 0u;
@@ -703,10 +902,12 @@ __device__ void item_final(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* 
 
 // Predecessor chase from the optimal root: breadth-first over the witness,
 // one lane per state; edges get their parent edge index so that a tree-like
-// walk (hop DP) keeps its shape.
+// walk (hop DP) keeps its shape.  A singleton read from C (kPredPath) has no
+// stored predecessor: its next edge is an arc v->u with C[u][i] + w == C[v][i],
+// found by a wave-wide scan of v's row (it exists because C is exact).
 __device__ void item_recon(const GraphDev& g, Slot* sp) {
   const int lane = lane_id();
-  const int W = sp->W, ND = sp->ND, D = sp->D;
+  const int W = sp->W, ND = sp->ND, D = sp->D, k = sp->k;
   const States S = states_of(sp);
   const bool hop = D >= 0;
   const uint32_t full = (uint32_t)W - 1;
@@ -724,7 +925,7 @@ __device__ void item_recon(const GraphDev& g, Slot* sp) {
   const uint32_t root = (uint32_t)ans;
   // smallest depth holding the optimum at the root
   uint32_t cd = kInf;
-  if (lane < ND) cd = (uint32_t)(*S.at(root, full, lane) >> 32);
+  if (lane < ND) cd = (uint32_t)(S.get(root, full, (uint32_t)lane) >> 32);
   const uint64_t hit = __ballot(lane < ND && cd == cost);
   const int dstar = hit ? (__ffsll((long long)hit) - 1) : 0;
   const uint32_t qcap = (uint32_t)sp->edge_cap + 2 * kMaxK;
@@ -739,12 +940,12 @@ __device__ void item_recon(const GraphDev& g, Slot* sp) {
   __threadfence();
   wave_sync();
   uint32_t head = 0, tail = 1, ne = 0;
-  bool overflow = false;
+  bool overflow = false, broken = false;
   while (head < tail) {
     const uint32_t cnt = (tail - head) < 64u ? (tail - head) : 64u;
     const uint32_t t = head + lane;
     const bool act = (uint32_t)lane < cnt;
-    uint32_t v = 0, p = 0, d = 0, pred = 0;
+    uint32_t v = 0, p = 0, d = 0, pred = 0, sc = 0;
     int32_t att = -1;
     if (act) {
       v = qv[t];
@@ -752,9 +953,48 @@ __device__ void item_recon(const GraphDev& g, Slot* sp) {
       p = pd & 0xFFFFu;
       d = pd >> 16;
       att = qat[t];
-      pred = (uint32_t)*S.at(v, p, d);
+      const uint64_t st = S.get(v, p, d);
+      pred = (uint32_t)st;
+      sc = (uint32_t)(st >> 32);
     }
-    const uint32_t type = pred >> 30;
+    uint32_t type = pred >> 30;
+    // shortest-path singletons: resolve the next arc with the whole wave
+    uint32_t parc = 0xFFFFFFFFu;
+    uint64_t pb = __ballot(act && type == 3u && sc > 0u);
+    while (pb) {
+      const int l = __ffsll((long long)pb) - 1;
+      pb &= pb - 1;
+      const uint32_t lv = __shfl(v, l, 64);
+      const uint32_t li = (uint32_t)(__ffs(__shfl(p, l, 64)) - 1);
+      const uint32_t lc = __shfl(sc, l, 64);
+      const uint32_t e0 = (uint32_t)g.rowptr[lv], e1 = (uint32_t)g.rowptr[lv + 1];
+      uint32_t found = 0xFFFFFFFFu;
+      for (uint32_t b = e0; b < e1; b += 64) {
+        const uint32_t e = b + lane;
+        bool ok = false;
+        if (e < e1) {
+          const uint32_t u = (uint32_t)g.col[e];
+          ok = sp->C[(uint64_t)u * k + li] + (uint32_t)g.wt[e] == lc;
+        }
+        const uint64_t hb = __ballot(ok);
+        if (hb) {
+          found = b + (uint32_t)(__ffsll((long long)hb) - 1);
+          break;
+        }
+      }
+      if (lane == l) parc = found;
+    }
+    if (act && type == 3u) {
+      if (sc == 0u) {
+        type = 0u;  // a member of the group: leaf
+      } else if (parc == 0xFFFFFFFFu) {
+        type = 0u;
+        broken = true;
+      } else {
+        type = 1u;
+        pred = kPredGrow | parc;
+      }
+    }
     const bool isg = act && type == 1u;
     const bool ism = act && type == 2u;
     const uint64_t gb = __ballot(isg);
@@ -807,12 +1047,14 @@ __device__ void item_recon(const GraphDev& g, Slot* sp) {
     __threadfence();
     wave_sync();
   }
+  const bool bad = __ballot(broken) != 0;
   if (lane == 0) {
     sp->out_cost = (int32_t)cost;
     sp->out_root = (int32_t)root;
     sp->out_nedges = (int32_t)ne;
     sp->out_d = dstar;
     if (overflow || ne > (uint32_t)sp->edge_cap) sp->status = 1;  // edge capacity exceeded
+    if (bad) sp->status = 2;                                       // no shortest-path arc found
   }
 }
 
@@ -831,89 +1073,121 @@ __device__ void advance_level(Slot* s) {
   }
 }
 
-__device__ void complete_step(Slot* s, Ctrl* ctrl) {
+__device__ void complete_step(Slot* slots, Slot* s, Ctrl* ctrl) {
   const uint32_t r = s->step;
-  switch (s->phase) {
-    case PH_FILL:
-      s->phase = PH_SEED;
-      break;
-    case PH_SEED:
-      if (s->k == 1) {
-        s->phase = PH_RECON;  // the answer was set while seeding
-      } else {
+  if (s->kind == KIND_JOB) {
+    switch (s->phase) {
+      case PH_FILL:
+        s->phase = PH_SEED;
+        break;
+      case PH_SEED:
         s->phase = PH_ROUND;
-        s->level = 1;
-      }
-      break;
-    case PH_ROUND:
-      s->ctr[1] += (unsigned long long)(s->app[r % 3] >> 32);
-      s->ctr[2] += (unsigned long long)(uint32_t)s->app[r % 3];
-      if ((s->app[(r + 1) % 3] >> 32) == 0) advance_level(s);
-      break;
-    case PH_DIST:
-      s->phase = PH_LIVE;
-      break;
-    case PH_LIVE:
-      if (s->k == 2) {
-        s->phase = PH_FINAL;
-      } else {
-        s->phase = PH_MERGE;
-        s->level = 2;
-      }
-      break;
-    case PH_MERGE:
-      if ((s->app[(r + 1) % 3] >> 32) == 0) advance_level(s);
-      else s->phase = PH_ROUND;
-      break;
-    case PH_FINAL:
-      s->phase = PH_RECON;
-      break;
-    case PH_RECON:
-      s->phase = PH_DONE;
-      atomicAdd(&ctrl->finished, 1u);
-      break;
-    default:
-      break;
+        break;
+      case PH_ROUND:
+        s->ctr[1] += (unsigned long long)(s->app[r % 3] >> 32);
+        s->ctr[2] += (unsigned long long)(uint32_t)s->app[r % 3];
+        if ((s->app[(r + 1) % 3] >> 32) == 0) s->phase = PH_SCATTER;
+        break;
+      case PH_SCATTER:
+        for (int m = 0; m < s->nmem; m++) slots[s->jmem[m]].jdone = 1u;
+        s->phase = PH_DONE;
+        break;
+      default:
+        break;
+    }
+  } else {
+    switch (s->phase) {
+      case PH_FILL:
+        s->phase = s->S1 ? PH_SEED : PH_WAITJ;
+        break;
+      case PH_SEED:
+        if (s->k == 1) {
+          s->phase = PH_RECON;  // the answer was set while seeding
+        } else {
+          s->phase = PH_ROUND;
+          s->level = 1;
+        }
+        break;
+      case PH_ROUND:
+        s->ctr[1] += (unsigned long long)(s->app[r % 3] >> 32);
+        s->ctr[2] += (unsigned long long)(uint32_t)s->app[r % 3];
+        if ((s->app[(r + 1) % 3] >> 32) == 0) advance_level(s);
+        break;
+      case PH_DIST:
+        s->phase = PH_LIVE;
+        break;
+      case PH_LIVE:
+        if (s->k == 2) {
+          s->phase = PH_FINAL;
+        } else {
+          s->phase = PH_MERGE;
+          s->level = 2;
+        }
+        break;
+      case PH_MERGE:
+        if ((s->app[(r + 1) % 3] >> 32) == 0) advance_level(s);
+        else s->phase = PH_ROUND;
+        break;
+      case PH_FINAL:
+        s->phase = PH_RECON;
+        break;
+      case PH_RECON:
+        s->phase = PH_DONE;
+        atomicAdd(&ctrl->finished, 1u);
+        break;
+      default:
+        break;
+    }
   }
   s->app[(r + 2) % 3] = 0;
   s->step = r + 1;
   s->ctr[0]++;
 }
 
+// Sets the slot's work for the next step: n_work items of class cls for its
+// phase and n_clr CLEAR items (lean); returns their sum.
 __device__ uint32_t plan_step(Slot* s, uint64_t V) {
   const uint32_t r = s->step;
   uint32_t work = 0;
+  int cls = kClsLean;
+  const bool job = s->kind == KIND_JOB;
   switch (s->phase) {
     case PH_FILL: {
-      const uint64_t total = V * (uint64_t)s->ND * s->k + 3 * V + ((uint64_t)V * s->k + 1) / 2 + (V + 7) / 8;
-      work = ceil_div(total, kFillWords);
+      uint64_t nA, nM, nC, nF;
+      fill_regions(s, V, &nA, &nM, &nC, &nF);
+      work = ceil_div(nA + nM + nC + nF, kFillWords);
       break;
     }
-    case PH_SEED: work = (uint32_t)s->k; break;
-    case PH_ROUND: work = ceil_div((uint32_t)s->app[r % 3], (uint32_t)s->chunk); break;
-    case PH_DIST: work = ceil_div(V, kDistTV) + 1; break;
-    case PH_LIVE: work = ceil_div(V, kDistTV); break;
+    case PH_SEED: work = job ? (uint32_t)s->L : (uint32_t)s->k; break;
+    case PH_ROUND:
+      work = ceil_div((uint32_t)s->app[r % 3], (uint32_t)s->chunk);
+      if (!job) cls = kClsHeavy;  // the per-query grow carries the lower-bound code
+      break;
+    case PH_SCATTER: work = ceil_div(V, kScatterTV); break;
+    case PH_DIST: work = (s->S1 ? ceil_div(V, kDistTV) : 0u) + 1u; cls = kClsHeavy; break;
+    case PH_LIVE: work = ceil_div(V, kDistTV); cls = kClsHeavy; break;
     case PH_MERGE:
-    case PH_FINAL: work = ceil_div(s->nlive, kMergeTV); break;
-    case PH_RECON: work = 1; break;
-    default: return (s->n_items = s->n_work = 0);
+    case PH_FINAL: work = ceil_div(s->nlive, kMergeTV); cls = kClsHeavy; break;
+    case PH_RECON: work = 1; cls = kClsHeavy; break;
+    default: return (s->n_work = s->n_clr = 0);
   }
   // every active step runs at least one (possibly empty) item so that it completes
   if (work == 0) work = 1;
-  const uint32_t nclr = ceil_div((uint32_t)(s->app[(r + 2) % 3] >> 32), kClearPer);
   s->n_work = work;
-  s->n_items = work + nclr;
-  return s->n_items;
+  s->cls = cls;
+  s->n_clr = ceil_div((uint32_t)(s->app[(r + 2) % 3] >> 32), kClearPer);
+  return work + s->n_clr;
 }
 
-// Run by the last block to reach the grid barrier.
-__device__ void phase_b(Slot* slots, Ctrl* ctrl, uint64_t V, uint32_t* sh_items, uint32_t* sh_scan) {
+// Run by the one-block plan kernel between steps.
+__device__ void phase_b(Slot* slots, Ctrl* ctrl, uint64_t V, uint32_t (*sh_items)[kMaxSlots],
+                        uint32_t (*sh_scan)[2][kBlock]) {
   const int tid = threadIdx.x;
   const int nslots = (int)ctrl->nslots;
   for (int i = tid; i < nslots; i += kBlock) {
     Slot* s = &slots[i];
     if (s->in_step) {
-      complete_step(s, ctrl);
+      complete_step(slots, s, ctrl);
       s->in_step = 0;
     }
   }
@@ -922,68 +1196,88 @@ __device__ void phase_b(Slot* slots, Ctrl* ctrl, uint64_t V, uint32_t* sh_items,
   if (tid == 0) {
     const uint32_t steps = ++ctrl->steps;  // only this block touches it now
     const uint32_t fin = __hip_atomic_load(&ctrl->finished, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    sh_scan[0] = (fin >= ctrl->exit_after || steps >= ctrl->max_steps) ? 1u : 0u;
+    sh_scan[0][0][0] = (fin >= ctrl->exit_after || steps >= ctrl->max_steps) ? 1u : 0u;
   }
   __syncthreads();
-  const bool leave = sh_scan[0] != 0;
+  const bool leave = sh_scan[0][0][0] != 0;
   __syncthreads();
   for (int i = tid; i < nslots; i += kBlock) {
     Slot* s = &slots[i];
-    uint32_t items = 0;
+    // a query whose job has finished leaves the wait (also on the last step
+    // of an epoch, so that the job can be released by the host)
+    if (s->phase == PH_WAITJ && s->jdone) s->phase = PH_DIST;
+    uint32_t lean = 0, heavy = 0;
     if (!leave && s->phase != PH_FREE && s->phase != PH_DONE) {
-      items = plan_step(s, V);
+      const uint32_t items = plan_step(s, V);
       s->in_step = items ? 1u : 0u;
+      if (items) {
+        lean = s->n_clr + (s->cls == kClsLean ? s->n_work : 0u);
+        heavy = s->cls == kClsHeavy ? s->n_work : 0u;
+      }
     }
-    sh_items[i] = items;
+    sh_items[kClsLean][i] = lean;
+    sh_items[kClsHeavy][i] = heavy;
   }
   __syncthreads();
-  // exclusive scan over slots with work, compacting them into the active list
+  // per class: exclusive scan over slots with work, compacting them into the
+  // active list
   const int per = (nslots + kBlock - 1) / kBlock;
   const int b0 = tid * per;
   const int b1 = (b0 + per) < nslots ? (b0 + per) : nslots;
-  uint32_t cnt = 0, act = 0;
-  for (int i = b0; i < b1; i++) {
-    cnt += sh_items[i];
-    act += sh_items[i] ? 1u : 0u;
+  for (int c = 0; c < kNumCls; c++) {
+    uint32_t cnt = 0, act = 0;
+    for (int i = b0; i < b1; i++) {
+      cnt += sh_items[c][i];
+      act += sh_items[c][i] ? 1u : 0u;
+    }
+    sh_scan[c][0][tid] = cnt;
+    sh_scan[c][1][tid] = act;
   }
-  sh_scan[tid] = cnt;
-  sh_scan[kBlock + tid] = act;
   __syncthreads();
   if (tid == 0) {
-    uint32_t c = 0, a = 0;
-    for (int t = 0; t < kBlock; t++) {
-      const uint32_t x = sh_scan[t], y = sh_scan[kBlock + t];
-      sh_scan[t] = c;
-      sh_scan[kBlock + t] = a;
-      c += x;
-      a += y;
+    uint32_t any = 0;
+    for (int c = 0; c < kNumCls; c++) {
+      uint32_t cs = 0, as = 0;
+      for (int t = 0; t < kBlock; t++) {
+        const uint32_t x = sh_scan[c][0][t], y = sh_scan[c][1][t];
+        sh_scan[c][0][t] = cs;
+        sh_scan[c][1][t] = as;
+        cs += x;
+        as += y;
+      }
+      ctrl->item_total[c] = cs;
+      ctrl->n_active[c] = as;
+      ctrl->prefix[c][as] = cs;
+      ctrl->item_next[c] = 0;
+      any += as;
     }
-    ctrl->item_total = c;
-    ctrl->n_active = a;
-    ctrl->prefix[a] = c;
-    ctrl->item_next = 0;
-    ctrl->exit_flag = (leave || a == 0) ? 1u : 0u;
+    ctrl->exit_flag = (leave || any == 0) ? 1u : 0u;
   }
   __syncthreads();
-  uint32_t c = sh_scan[tid], a = sh_scan[kBlock + tid];
-  for (int i = b0; i < b1; i++) {
-    if (sh_items[i]) {
-      ctrl->active[a] = (uint32_t)i;
-      ctrl->prefix[a] = c;
-      a++;
-      c += sh_items[i];
+  for (int c = 0; c < kNumCls; c++) {
+    uint32_t cs = sh_scan[c][0][tid], as = sh_scan[c][1][tid];
+    for (int i = b0; i < b1; i++) {
+      if (sh_items[c][i]) {
+        ctrl->active[c][as] = (uint32_t)i;
+        ctrl->prefix[c][as] = cs;
+        as++;
+        cs += sh_items[c][i];
+      }
     }
   }
 }
 
-// One engine step for one wave: pull work items of the current plan until it
-// is exhausted.
-__device__ void step_body(const GraphDev& g, Slot* slots, Ctrl* ctrl, WaveLDS* L) {
+// One engine step of one work class for one wave: pull items of the current
+// plan until they are exhausted.
+template <int CLS, class LDS>
+__device__ void step_body(const GraphDev& g, Slot* slots, Ctrl* ctrl, LDS* L) {
   const int lane = lane_id();
   const uint64_t V = (uint64_t)g.V;
   if (ctrl->exit_flag) return;
-  const uint32_t total = ctrl->item_total;
-  const uint32_t nact = ctrl->n_active;
+  const uint32_t total = ctrl->item_total[CLS];
+  const uint32_t nact = ctrl->n_active[CLS];
+  const uint32_t* prefix = ctrl->prefix[CLS];
+  const uint32_t* active = ctrl->active[CLS];
   for (;;) {
     // Claim the next item for the whole wave.  Every lane executes the atomic
     // (only lane 0 adds one) and the index is read from lane 0 into a scalar
@@ -992,53 +1286,75 @@ __device__ void step_body(const GraphDev& g, Slot* slots, Ctrl* ctrl, WaveLDS* L
     // threads the lane != 0 edge of the `if (lane == 0)` blocks that end
     // several items straight to the broadcast: the loop splits in two and
     // lanes 1..63 re-run the same item without lane 0, never finishing.
-    const uint32_t mine = atomicAdd(&ctrl->item_next, lane == 0 ? 1u : 0u);
+    const uint32_t mine = atomicAdd(&ctrl->item_next[CLS], lane == 0 ? 1u : 0u);
     const uint32_t it = __builtin_amdgcn_readfirstlane(mine);
     if (it >= total) break;
     uint32_t lo = 0, hi = nact;  // last active slot whose prefix <= it
     while (hi - lo > 1) {
       const uint32_t mid = (lo + hi) >> 1;
-      if (ctrl->prefix[mid] <= it) lo = mid; else hi = mid;
+      if (prefix[mid] <= it) lo = mid; else hi = mid;
     }
-    Slot* sp = &slots[ctrl->active[lo]];
-    const uint32_t local = it - ctrl->prefix[lo];
-    const uint32_t nwork = sp->n_work;
-    if (local >= nwork) {
-      item_clear(sp, local - nwork, V);
-      continue;
-    }
-    switch (sp->phase) {
-      case PH_FILL: item_fill(sp, local, V); break;
-      case PH_SEED: item_seed(g, sp, local); break;
-      case PH_ROUND: item_round(g, sp, local, L); break;
-      case PH_DIST:
-        if (local + 1 == nwork) item_vvr(g, sp, L);
-        else item_dist(g, sp, local);
-        break;
-      case PH_LIVE: item_live(sp, local, V, L); break;
-      case PH_MERGE: item_merge(g, sp, local, L); break;
-      case PH_FINAL: item_final(g, sp, local, L); break;
-      case PH_RECON: item_recon(g, sp); break;
-      default: break;
+    Slot* sp = &slots[active[lo]];
+    const uint32_t local = it - prefix[lo];
+    if constexpr (CLS == kClsLean) {
+      const uint32_t nwork = sp->cls == kClsLean ? sp->n_work : 0u;
+      if (local >= nwork) {
+        item_clear(sp, local - nwork, V);
+        continue;
+      }
+      if (sp->kind == KIND_JOB) {
+        switch (sp->phase) {
+          case PH_FILL: item_fill(sp, local, V); break;
+          case PH_SEED: item_jseed(g, sp, local); break;
+          case PH_ROUND: item_jround(g, sp, local, (FrontLDS*)L); break;
+          case PH_SCATTER: item_jscatter(slots, sp, local, V); break;
+          default: break;
+        }
+      } else {
+        switch (sp->phase) {
+          case PH_FILL: item_fill(sp, local, V); break;
+          case PH_SEED: item_seed(g, sp, local); break;
+          default: break;
+        }
+      }
+    } else {
+      const uint32_t nwork = sp->n_work;
+      switch (sp->phase) {
+        case PH_DIST:
+          if (local + 1 == nwork) item_vvr(g, sp, (WaveLDS*)L);
+          else item_dist(g, sp, local);
+          break;
+        case PH_ROUND: item_round(g, sp, local, &((WaveLDS*)L)->f); break;
+        case PH_LIVE: item_live(sp, local, V, (WaveLDS*)L); break;
+        case PH_MERGE: item_merge(g, sp, local, (WaveLDS*)L); break;
+        case PH_FINAL: item_final(g, sp, local, (WaveLDS*)L); break;
+        case PH_RECON: item_recon(g, sp); break;
+        default: break;
+      }
     }
   }
 }
 
 #ifndef GST_CPU_SIM
-// One engine step: every wave pulls work items until the plan is exhausted.
-// There is no waiting between blocks; the step ends when the kernel ends, and
-// the next plan kernel sees everything it wrote.
-__global__ __launch_bounds__(kBlock) void gst_step_kernel(GraphDev g, Slot* slots, Ctrl* ctrl) {
+// One engine step of each class: every wave pulls items until the class's
+// plan is exhausted.  There is no waiting between blocks; the step ends when
+// the kernels end, and the next plan kernel sees everything they wrote.
+__global__ __launch_bounds__(kBlock) void gst_lean_kernel(GraphDev g, Slot* slots, Ctrl* ctrl) {
+  __shared__ FrontLDS lds[kWaves];
+  step_body<kClsLean>(g, slots, ctrl, &lds[threadIdx.x >> 6]);
+}
+
+__global__ __launch_bounds__(kBlock) void gst_heavy_kernel(GraphDev g, Slot* slots, Ctrl* ctrl) {
   __shared__ WaveLDS lds[kWaves];
-  step_body(g, slots, ctrl, &lds[threadIdx.x >> 6]);
+  step_body<kClsHeavy>(g, slots, ctrl, &lds[threadIdx.x >> 6]);
 }
 
 // Between two steps, one block advances every slot's state machine and
 // publishes the next step's work items.  Once it has set exit_flag it leaves
 // everything untouched, so the host may enqueue more steps than are needed.
 __global__ __launch_bounds__(kBlock) void gst_plan_kernel(Slot* slots, Ctrl* ctrl, uint64_t V) {
-  __shared__ uint32_t sh_items[kMaxSlots];
-  __shared__ uint32_t sh_scan[2 * kBlock];
+  __shared__ uint32_t sh_items[kNumCls][kMaxSlots];
+  __shared__ uint32_t sh_scan[kNumCls][2][kBlock];
   if (ctrl->exit_flag) return;
   phase_b(slots, ctrl, V, sh_items, sh_scan);
 }
@@ -1047,9 +1363,12 @@ __global__ __launch_bounds__(kBlock) void gst_plan_kernel(Slot* slots, Ctrl* ctr
 }  // namespace gst
 
 #ifndef GST_CPU_SIM
-extern "C" hipError_t gst_launch_step(const gst::GraphDev* g, gst::Slot* slots, gst::Ctrl* ctrl, int grid,
-                                      hipStream_t stream) {
-  hipLaunchKernelGGL(gst::gst_step_kernel, dim3(grid), dim3(gst::kBlock), 0, stream, *g, slots, ctrl);
+extern "C" hipError_t gst_launch_step(int cls, const gst::GraphDev* g, gst::Slot* slots, gst::Ctrl* ctrl,
+                                      int grid, hipStream_t stream) {
+  if (cls == gst::kClsLean)
+    hipLaunchKernelGGL(gst::gst_lean_kernel, dim3(grid), dim3(gst::kBlock), 0, stream, *g, slots, ctrl);
+  else
+    hipLaunchKernelGGL(gst::gst_heavy_kernel, dim3(grid), dim3(gst::kBlock), 0, stream, *g, slots, ctrl);
   return hipGetLastError();
 }
 
@@ -1058,8 +1377,8 @@ extern "C" hipError_t gst_launch_plan(gst::Slot* slots, gst::Ctrl* ctrl, uint64_
   return hipGetLastError();
 }
 
-extern "C" hipError_t gst_step_occupancy(int* blocks_per_cu) {
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, (const void*)gst::gst_step_kernel,
-                                                       gst::kBlock, 0);
+extern "C" hipError_t gst_step_occupancy(int cls, int* blocks_per_cu) {
+  const void* f = cls == gst::kClsLean ? (const void*)gst::gst_lean_kernel : (const void*)gst::gst_heavy_kernel;
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, f, gst::kBlock, 0);
 }
 #endif  // GST_CPU_SIM

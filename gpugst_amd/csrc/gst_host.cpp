@@ -1,6 +1,6 @@
 // gst_host.cpp -- host side of libgst: graph residency, the slot engine that
-// drives the persistent kernel, memory admission, batch and server paths, and
-// the C ABI declared in include/gst.h.
+// drives the step and plan kernels, group-distance jobs, memory admission,
+// batch and server paths, and the C ABI declared in include/gst.h.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -23,10 +23,10 @@
 #include "gst.h"
 #include "gst_common.h"
 
-extern "C" hipError_t gst_launch_step(const gst::GraphDev* g, gst::Slot* slots, gst::Ctrl* ctrl, int grid,
-                                      hipStream_t stream);
+extern "C" hipError_t gst_launch_step(int cls, const gst::GraphDev* g, gst::Slot* slots, gst::Ctrl* ctrl,
+                                      int grid, hipStream_t stream);
 extern "C" hipError_t gst_launch_plan(gst::Slot* slots, gst::Ctrl* ctrl, uint64_t V, hipStream_t stream);
-extern "C" hipError_t gst_step_occupancy(int* blocks_per_cu);
+extern "C" hipError_t gst_step_occupancy(int cls, int* blocks_per_cu);
 
 namespace {
 
@@ -119,6 +119,8 @@ struct Options {
   int64_t edge_cap = 65536;
   int64_t timeout_ms = 60000;  // host-side deadline for one chunk of engine steps
   int64_t graph = 0;  // replay chunks of engine steps as a captured hipGraph
+  int64_t group_pass = 1;  // unbounded queries take level 1 from shared group-distance jobs
+  int64_t time_steps = 0;  // time each step's lean and heavy kernels with events
 };
 
 }  // namespace
@@ -126,7 +128,7 @@ struct Options {
 struct gst_graph {
   int device = 0;
   int num_cu = 0;
-  int occ = 1;
+  int occ[gst::kNumCls] = {1, 1};  // blocks per CU of the lean and heavy step kernels
   int64_t V = 0, n_arcs = 0;
   int64_t n_groups = 0;
   std::vector<int64_t> h_gptr;
@@ -137,7 +139,7 @@ struct gst_graph {
   int64_t* d_gptr = nullptr;
   int32_t* d_gmem = nullptr;
   Options opt;
-  std::mutex dev_mu;  // one persistent engine kernel at a time
+  std::mutex dev_mu;  // one engine on the device at a time
   std::unique_ptr<class Engine> plain;
   std::unique_ptr<class Engine> batch;
   gst::GraphDev dev() const {
@@ -164,31 +166,74 @@ struct QueryReq {
 };
 
 struct Layout {
-  size_t T, S1, live, lvf, mask, C, list, loff, R, edges, rq, total;
+  size_t T, S1, live, lvf, mask, C, list, loff, istart, R, edges, rq, total;
   int edge_cap;
+  uint32_t istride;
 };
 
-Layout layout_for(const gst_graph* g, int k, int D) {
+// entries per buffer of istart for ROUND items of `chunk` arcs
+uint32_t istride_for(const gst_graph* g, int64_t chunk) {
+  return (uint32_t)(g->n_arcs / std::max<int64_t>(chunk, 1) + 2);
+}
+
+// True when the query takes its level 1 from a group-distance job: unbounded
+// DP with k >= 2 (k = 1 is answered at SEED).
+bool uses_job(const gst_graph* g, int k, int D) { return g->opt.group_pass != 0 && D < 0 && k >= 2; }
+
+// Device memory of one query; `own` = it computes its own level 1 (S1 table).
+Layout layout_for(const gst_graph* g, int k, int D, bool own) {
   const size_t V = (size_t)g->V;
   const size_t W = size_t(1) << k;
   const size_t ND = D < 0 ? 1 : (size_t)D + 1;
   Layout L;
   int64_t cap = std::min<int64_t>((int64_t)(V * ND), g->opt.edge_cap);
   L.edge_cap = (int)std::max<int64_t>(cap, 64);
+  L.istride = istride_for(g, g->opt.chunk);
   size_t off = 0;
   L.T = off;    off += align256(V * ND * W * 8);
-  L.S1 = off;   off += align256(V * ND * k * 8);
+  L.S1 = off;   off += own ? align256(V * ND * k * 8) : 0;
   L.live = off; off += align256(V * 4);
   L.lvf = off;  off += align256(((V + 7) / 8) * 8);
   L.mask = off; off += align256(3 * V * 8);
   L.C = off;    off += align256(((V * k + 1) / 2) * 8);
   L.list = off; off += align256(3 * V * 4);
   L.loff = off; off += align256(3 * V * 4);
+  L.istart = off; off += align256(3 * (size_t)L.istride * 4);
   L.R = off;    off += align256(W * k * k * 4);
   L.edges = off; off += align256((size_t)L.edge_cap * sizeof(gst::EdgeDev));
   L.rq = off;   off += align256(3 * ((size_t)L.edge_cap + 2 * gst::kMaxK) * 4);
   L.total = off;
   return L;
+}
+
+Layout layout_for(const gst_graph* g, int k, int D) { return layout_for(g, k, D, !uses_job(g, k, D)); }
+
+// Group-distance job over L labels.
+struct JobLayout {
+  size_t dist, mask, list, loff, istart, lgrp, jmem, total;
+  int Lp, G, chunk;
+  uint32_t istride;
+};
+
+JobLayout job_layout(const gst_graph* g, int L, int nmem) {
+  const size_t V = (size_t)g->V;
+  JobLayout J;
+  J.Lp = (L + 3) & ~3;
+  const int nch = J.Lp / 4;
+  J.G = 1;
+  while (J.G < nch) J.G <<= 1;
+  J.chunk = (int)gst::kJobIter * (64 / J.G);
+  J.istride = istride_for(g, J.chunk);
+  size_t off = 0;
+  J.dist = off;   off += align256(V * (size_t)J.Lp * 4);
+  J.mask = off;   off += align256(3 * V * 8);
+  J.list = off;   off += align256(3 * V * 4);
+  J.loff = off;   off += align256(3 * V * 4);
+  J.istart = off; off += align256(3 * (size_t)J.istride * 4);
+  J.lgrp = off;   off += align256((size_t)L * 4);
+  J.jmem = off;   off += align256((size_t)nmem * 4);
+  J.total = off;
+  return J;
 }
 
 gst_status validate_query(const gst_graph* g, const int32_t* groups, int32_t k, int32_t D) {
@@ -213,9 +258,11 @@ void fill_infeasible(gst_result* r) {
 
 struct SlotHost {
   bool used = false;
+  bool job = false;  // a group-distance job (no result of its own)
   size_t off = 0, len = 0;
   uint64_t tag = 0;
   double t_submit = 0, t_admit = 0;
+  int64_t extra[5] = {0, 0, 0, 0, 0};  // level-1 work charged from the query's job
 };
 
 }  // namespace
@@ -233,6 +280,7 @@ class Engine {
     drop_graph();
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);
+    for (auto e : tev_) (void)hipEventDestroy(e);
     if (stream_) (void)hipStreamDestroy(stream_);
   }
 
@@ -273,24 +321,67 @@ class Engine {
   }
 
   size_t arena_size() const { return arena_.size(); }
+  // admitted queries (group-distance jobs are not counted)
   int active() const { return active_; }
-  bool has_free_slot() const { return active_ < max_slots_; }
+  bool has_free_slot() const { return used_slots_ < max_slots_; }
 
-  // Returns the slot index, -1 when memory or slots are exhausted now.
-  int admit(const QueryReq& q) {
-    if (active_ >= max_slots_) return -1;
+  // Admits queries from the front of `pend` while slots and memory allow and
+  // returns how many were admitted.  Consecutive unbounded queries (k >= 2)
+  // share a group-distance job of up to kJobLabels labels; a job is sized for
+  // the members that fit together with it.
+  int admit_front(std::deque<QueryReq*>& pend) {
+    int n = 0;
+    while (!pend.empty()) {
+      QueryReq* q0 = pend.front();
+      const int k0 = (int)q0->groups.size();
+      if (!uses_job(g_, k0, q0->D)) {
+        if (used_slots_ >= max_slots_ || admit_query(*q0, -1, 0) < 0) break;
+        pend.pop_front();
+        n++;
+        continue;
+      }
+      std::vector<QueryReq*> grp;
+      int L = 0;
+      for (auto it = pend.begin(); it != pend.end(); ++it) {
+        const int k = (int)(*it)->groups.size();
+        if (!uses_job(g_, k, (*it)->D) || L + k > gst::kJobLabels) break;
+        if (used_slots_ + (int)grp.size() + 2 > max_slots_) break;
+        grp.push_back(*it);
+        L += k;
+      }
+      const size_t want = grp.size();
+      while (!grp.empty() && !admit_job_group(grp, L)) {
+        L -= (int)grp.back()->groups.size();
+        grp.pop_back();
+      }
+      for (size_t i = 0; i < grp.size(); i++) pend.pop_front();
+      n += (int)grp.size();
+      if (grp.size() < want || grp.empty()) break;
+    }
+    return n;
+  }
+
+ private:
+  int free_slot() const {
+    for (int i = 0; i < max_slots_; i++)
+      if (!sh_[i].used) return i;
+    return -1;
+  }
+
+  // A query slot; job >= 0 names the slot of its group-distance job and lab
+  // its first label there.  Returns the slot index, -1 when memory or slots
+  // are exhausted.
+  int admit_query(const QueryReq& q, int job, int lab) {
     const int k = (int)q.groups.size();
-    const Layout L = layout_for(g_, k, q.D);
+    const bool own = job < 0;
+    const Layout L = layout_for(g_, k, q.D, own);
+    const int s = free_slot();
+    if (s < 0) return -1;
     size_t off = 0;
     if (!arena_.alloc(L.total, &off)) return -1;
-    int s = -1;
-    for (int i = 0; i < max_slots_; i++)
-      if (!sh_[i].used) {
-        s = i;
-        break;
-      }
     SlotHost& h = sh_[s];
     h.used = true;
+    h.job = false;
     h.off = off;
     h.len = L.total;
     h.tag = q.tag;
@@ -298,6 +389,9 @@ class Engine {
     h.t_admit = now_ms();
     gst::Slot& d = h_slots_[s];
     memset(&d, 0, sizeof(d));
+    d.kind = gst::KIND_QUERY;
+    d.job = job;
+    d.lab = lab;
     d.k = k;
     d.W = 1 << k;
     d.ND = q.D < 0 ? 1 : q.D + 1;
@@ -305,79 +399,148 @@ class Engine {
     for (int i = 0; i < k; i++) d.qg[i] = q.groups[i];
     d.edge_cap = L.edge_cap;
     d.chunk = (int32_t)g_->opt.chunk;
+    d.istride = L.istride;
     d.nT = (uint64_t)g_->V * d.ND * d.W;
     char* base = arena_mem_ + off;
     d.T = (uint64_t*)(base + L.T);
-    d.S1 = (uint64_t*)(base + L.S1);
+    d.S1 = own ? (uint64_t*)(base + L.S1) : nullptr;
     d.live = (uint32_t*)(base + L.live);
     d.lvf = (uint8_t*)(base + L.lvf);
     d.mask = (uint64_t*)(base + L.mask);
     d.C = (uint32_t*)(base + L.C);
     d.list = (uint32_t*)(base + L.list);
     d.loff = (uint32_t*)(base + L.loff);
+    d.istart = (uint32_t*)(base + L.istart);
     d.R = (uint32_t*)(base + L.R);
     d.edges = (gst::EdgeDev*)(base + L.edges);
     d.rq = (uint32_t*)(base + L.rq);
     d.phase = gst::PH_FILL;
-    d.level = 0;
-    d.step = 0;
-    d.in_step = 0;
     d.best = gst::kInf;
     d.ans = ~0ull;
     d.out_cost = (int32_t)gst::kInf;
     d.out_root = -1;
     nslots_used_ = std::max(nslots_used_, s + 1);
+    used_slots_++;
     active_++;
     return s;
   }
 
-  // Runs the engine kernel until `exit_after` queries finish (or max_steps
-  // global steps pass); returns the finished slot indices.
+  void release_slot(int s) {
+    SlotHost& h = sh_[s];
+    arena_.release(h.off, h.len);
+    if (!h.job) active_--;
+    h.used = false;
+    h.job = false;
+    used_slots_--;
+    memset(&h_slots_[s], 0, sizeof(gst::Slot));  // PH_FREE
+    while (nslots_used_ > 0 && !sh_[nslots_used_ - 1].used) nslots_used_--;
+  }
+
+  // One job over the labels of `grp` plus its member queries; all or nothing.
+  bool admit_job_group(const std::vector<QueryReq*>& grp, int L) {
+    const JobLayout J = job_layout(g_, L, (int)grp.size());
+    const int js = free_slot();
+    if (js < 0) return false;
+    size_t off = 0;
+    if (!arena_.alloc(J.total, &off)) return false;
+    SlotHost& h = sh_[js];
+    h.used = true;
+    h.job = true;
+    h.off = off;
+    h.len = J.total;
+    used_slots_++;
+    nslots_used_ = std::max(nslots_used_, js + 1);
+    std::vector<int> mem;
+    int lab = 0;
+    for (QueryReq* q : grp) {
+      const int s = admit_query(*q, js, lab);
+      if (s < 0) {
+        for (int m : mem) release_slot(m);
+        release_slot(js);
+        return false;
+      }
+      mem.push_back(s);
+      lab += (int)q->groups.size();
+    }
+    job_members_[js] = mem;
+    gst::Slot& d = h_slots_[js];
+    memset(&d, 0, sizeof(d));
+    d.kind = gst::KIND_JOB;
+    d.job = -1;
+    d.L = L;
+    d.Lp = J.Lp;
+    d.G = J.G;
+    d.chunk = J.chunk;
+    d.istride = J.istride;
+    d.nmem = (int32_t)mem.size();
+    char* base = arena_mem_ + off;
+    d.dist = (uint32_t*)(base + J.dist);
+    d.mask = (uint64_t*)(base + J.mask);
+    d.list = (uint32_t*)(base + J.list);
+    d.loff = (uint32_t*)(base + J.loff);
+    d.istart = (uint32_t*)(base + J.istart);
+    d.lgrp = (int32_t*)(base + J.lgrp);
+    d.jmem = (uint32_t*)(base + J.jmem);
+    d.phase = gst::PH_FILL;
+    // label -> group and member lists go up with the next epoch's uploads
+    std::vector<int32_t> lg;
+    for (QueryReq* q : grp) lg.insert(lg.end(), q->groups.begin(), q->groups.end());
+    std::vector<uint32_t> jm(mem.begin(), mem.end());
+    pending_uploads_.push_back({d.lgrp, std::vector<char>((char*)lg.data(), (char*)(lg.data() + lg.size()))});
+    pending_uploads_.push_back({d.jmem, std::vector<char>((char*)jm.data(), (char*)(jm.data() + jm.size()))});
+    return true;
+  }
+
+ public:
   // Runs engine steps until `exit_after` queries finish (or max_steps steps
-  // pass, or nothing is left to do); returns the finished slot indices.
-  // Steps are enqueued in chunks of (step kernel, plan kernel) pairs; the
-  // plan kernel turns into a no-op once it has decided to stop, so a chunk may
+  // pass, or nothing is left to do); returns the finished query slots.
+  // Steps are enqueued in chunks of (lean, heavy, plan) launches; the plan
+  // kernel turns into a no-op once it has decided to stop, so a chunk may
   // overshoot safely.  The control header is read back after each chunk.
+  // Finished group-distance jobs are released here.
   gst_status run_epoch(uint32_t exit_after, uint32_t max_steps, std::vector<int>* finished) {
     finished->clear();
     if (nslots_used_ == 0) return GST_OK;
     step_before_.resize(max_slots_);
     for (int i = 0; i < nslots_used_; i++) step_before_[i] = h_slots_[i].step;
     const bool use_graph = g_->opt.graph != 0;
+    const bool timed = g_->opt.time_steps != 0 && !use_graph;
+    for (auto& u : pending_uploads_)
+      HIP_TRY(hipMemcpyAsync(u.first, u.second.data(), u.second.size(), hipMemcpyHostToDevice, stream_));
     const size_t slot_bytes = sizeof(gst::Slot) * (use_graph ? max_slots_ : nslots_used_);
     HIP_TRY(hipMemcpyAsync(d_slots_, h_slots_, slot_bytes, hipMemcpyHostToDevice, stream_));
     gst::Ctrl& c = *h_ctrl_;
-    c.arrive = 0;
-    c.gen = 0;
-    c.item_next = 0;
-    c.item_total = 0;
-    c.n_active = 0;
-    c.exit_flag = 0;
-    c.abort_flag = 0;
-    c.finished = 0;
-    c.steps = 0;
+    memset(&c, 0, offsetof(gst::Ctrl, active));
     c.exit_after = std::max(1u, exit_after);
     c.max_steps = max_steps;
     c.nslots = (uint32_t)(use_graph ? max_slots_ : nslots_used_);
     const size_t hdr = offsetof(gst::Ctrl, active);
     HIP_TRY(hipMemcpyAsync(d_ctrl_, h_ctrl_, hdr, hipMemcpyHostToDevice, stream_));
-    const int grid = grid_size();
+    const int grid_l = grid_size(gst::kClsLean), grid_h = grid_size(gst::kClsHeavy);
     const gst::GraphDev gd = g_->dev();
+    if (timed && tev_.empty()) {
+      tev_.resize(4 * 64);
+      for (auto& e : tev_) HIP_TRY(hipEventCreate(&e));
+    }
     HIP_TRY(hipEventRecord(ev0_, stream_));
     hipError_t le = gst_launch_plan(d_slots_, d_ctrl_, (uint64_t)g_->V, stream_);
     if (le != hipSuccess) return fail(GST_DEVICE_ERROR, "plan launch failed: %s", hipGetErrorString(le));
     int chunk = use_graph ? kGraphChunk : 4;
     for (;;) {
       if (use_graph) {
-        if (gexec_ && graph_grid_ != grid) drop_graph();
+        if (gexec_ && (graph_grid_[0] != grid_l || graph_grid_[1] != grid_h)) drop_graph();
         if (!gexec_) {
-          gst_status st = capture_chunk(grid);
+          gst_status st = capture_chunk(grid_l, grid_h);
           if (st != GST_OK) return st;
         }
         HIP_TRY(hipGraphLaunch(gexec_, stream_));
       } else {
         for (int i = 0; i < chunk; i++) {
-          le = gst_launch_step(&gd, d_slots_, d_ctrl_, grid, stream_);
+          if (timed) HIP_TRY(hipEventRecord(tev_[4 * i], stream_));
+          le = gst_launch_step(gst::kClsLean, &gd, d_slots_, d_ctrl_, grid_l, stream_);
+          if (timed && le == hipSuccess) le = hipEventRecord(tev_[4 * i + 1], stream_);
+          if (le == hipSuccess) le = gst_launch_step(gst::kClsHeavy, &gd, d_slots_, d_ctrl_, grid_h, stream_);
+          if (timed && le == hipSuccess) le = hipEventRecord(tev_[4 * i + 2], stream_);
           if (le == hipSuccess) le = gst_launch_plan(d_slots_, d_ctrl_, (uint64_t)g_->V, stream_);
           if (le != hipSuccess) return fail(GST_DEVICE_ERROR, "step launch failed: %s", hipGetErrorString(le));
         }
@@ -385,9 +548,20 @@ class Engine {
       HIP_TRY(hipMemcpyAsync(h_ctrl_, d_ctrl_, hdr, hipMemcpyDeviceToHost, stream_));
       gst_status st = wait_stream();
       if (st != GST_OK) return st;
+      if (timed) {
+        for (int i = 0; i < chunk; i++) {
+          float a = 0, b = 0;
+          HIP_TRY(hipEventElapsedTime(&a, tev_[4 * i], tev_[4 * i + 1]));
+          HIP_TRY(hipEventElapsedTime(&b, tev_[4 * i + 1], tev_[4 * i + 2]));
+          cls_ns_[0] += (int64_t)((double)a * 1e6);
+          cls_ns_[1] += (int64_t)((double)b * 1e6);
+          cls_launches_ += 1;
+        }
+      }
       if (h_ctrl_->exit_flag) break;
       if (!use_graph) chunk = std::min(chunk * 2, 64);
     }
+    pending_uploads_.clear();
     HIP_TRY(hipEventRecord(ev1_, stream_));
     HIP_TRY(hipMemcpyAsync(h_slots_, d_slots_, sizeof(gst::Slot) * nslots_used_, hipMemcpyDeviceToHost, stream_));
     gst_status st = wait_stream();
@@ -398,17 +572,37 @@ class Engine {
     launches_++;
     steps_total_ += h_ctrl_->steps;
     last_steps_ = h_ctrl_->steps;
-    // progress guard: every epoch runs at least one step of some admitted query
+    // progress guard: every epoch runs at least one step of some admitted slot
     bool any_step = false;
     for (int i = 0; i < nslots_used_; i++)
       if (sh_[i].used && (h_slots_[i].step != step_before_[i] || h_slots_[i].phase == gst::PH_DONE))
         any_step = true;
     if (!any_step)
       return fail(GST_DEVICE_ERROR, "engine epoch made no progress on %d admitted queries", active_);
-    for (int i = 0; i < nslots_used_; i++)
-      if (sh_[i].used && h_slots_[i].phase == gst::PH_DONE) finished->push_back(i);
+    for (int i = 0; i < nslots_used_; i++) {
+      if (!sh_[i].used || h_slots_[i].phase != gst::PH_DONE) continue;
+      if (sh_[i].job) {
+        // the job's level-1 work is charged to its members, per label
+        const gst::Slot& j = h_slots_[i];
+        std::vector<uint32_t> jm(j.nmem);
+        for (int m = 0; m < j.nmem; m++) {
+          const gst::Slot& q = h_slots_[job_members_at(i, m)];
+          SlotHost& qh = sh_[job_members_at(i, m)];
+          for (int t = 1; t <= 4; t++) qh.extra[t] += (int64_t)(j.ctr[t] * (uint64_t)q.k / (uint64_t)std::max(1, j.L));
+          qh.extra[0] += (int64_t)j.ctr[0];
+        }
+        jobs_done_++;
+        for (int t = 0; t < 6; t++) job_ctr_[t] += (int64_t)j.ctr[t];
+        job_members_.erase(i);
+        release_slot(i);
+      } else {
+        finished->push_back(i);
+      }
+    }
     return GST_OK;
   }
+
+  int job_members_at(int js, int m) const { return job_members_.at(js)[m]; }
 
   // Waits for the stream with a deadline, so a launch that never finishes is
   // reported instead of blocking the caller forever.
@@ -424,13 +618,15 @@ class Engine {
     }
   }
 
-  // A chunk of kGraphChunk (step, plan) launches captured once and replayed.
-  gst_status capture_chunk(int grid) {
+  // A chunk of kGraphChunk (lean, heavy, plan) launches captured once and
+  // replayed.
+  gst_status capture_chunk(int grid_l, int grid_h) {
     const gst::GraphDev gd = g_->dev();
     HIP_TRY(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
     hipError_t le = hipSuccess;
     for (int i = 0; i < kGraphChunk && le == hipSuccess; i++) {
-      le = gst_launch_step(&gd, d_slots_, d_ctrl_, grid, stream_);
+      le = gst_launch_step(gst::kClsLean, &gd, d_slots_, d_ctrl_, grid_l, stream_);
+      if (le == hipSuccess) le = gst_launch_step(gst::kClsHeavy, &gd, d_slots_, d_ctrl_, grid_h, stream_);
       if (le == hipSuccess) le = gst_launch_plan(d_slots_, d_ctrl_, (uint64_t)g_->V, stream_);
     }
     hipGraph_t graph = nullptr;
@@ -443,7 +639,8 @@ class Engine {
       gexec_ = nullptr;
       return fail(GST_DEVICE_ERROR, "hipGraphInstantiate: %s", hipGetErrorString(e));
     }
-    graph_grid_ = grid;
+    graph_grid_[0] = grid_l;
+    graph_grid_[1] = grid_h;
     return GST_OK;
   }
 
@@ -452,7 +649,7 @@ class Engine {
     gexec_ = nullptr;
   }
 
-  // Copies the result of a finished slot out and releases its memory.
+  // Copies the result of a finished query out and releases its memory.
   gst_status harvest(int s, gst_result* out, uint64_t* tag) {
     const gst::Slot& d = h_slots_[s];
     SlotHost& h = sh_[s];
@@ -463,16 +660,21 @@ class Engine {
     out->cost = d.out_cost;
     out->root = d.out_root;
     out->status = GST_OK;
-    out->counters.steps = (int64_t)d.ctr[0];
-    out->counters.frontier_entries = (int64_t)d.ctr[1];
-    out->counters.arcs = (int64_t)d.ctr[2];
-    out->counters.relaxations = (int64_t)d.ctr[3];
-    out->counters.improvements = (int64_t)d.ctr[4];
+    out->counters.steps = (int64_t)d.ctr[0] + h.extra[0];
+    out->counters.frontier_entries = (int64_t)d.ctr[1] + h.extra[1];
+    out->counters.arcs = (int64_t)d.ctr[2] + h.extra[2];
+    out->counters.relaxations = (int64_t)d.ctr[3] + h.extra[3];
+    out->counters.improvements = (int64_t)d.ctr[4] + h.extra[4];
     out->counters.merge_evals = (int64_t)d.ctr[5];
+    out->counters.live_vertices = (int64_t)d.nlive;
     gst_status st = GST_OK;
-    if (d.status != 0) {
+    if (d.status == 1) {
       out->status = GST_DEVICE_ERROR;
       st = fail(GST_DEVICE_ERROR, "tree of the query exceeds the edge capacity %d", d.edge_cap);
+      out->n_edges = 0;
+    } else if (d.status != 0) {
+      out->status = GST_DEVICE_ERROR;
+      st = fail(GST_DEVICE_ERROR, "tree reconstruction found no shortest-path arc (status %u)", d.status);
       out->n_edges = 0;
     } else if (d.out_cost >= (int32_t)gst::kInf) {
       out->cost = GST_INFEASIBLE;
@@ -493,35 +695,44 @@ class Engine {
       }
     }
     *tag = h.tag;
-    arena_.release(h.off, h.len);
-    h.used = false;
-    memset(&h_slots_[s], 0, sizeof(gst::Slot));  // PH_FREE
-    active_--;
-    while (nslots_used_ > 0 && !sh_[nslots_used_ - 1].used) nslots_used_--;
+    for (auto& x : h.extra) x = 0;
+    release_slot(s);
     return st;
   }
 
-  // Drops every admitted query (after a device error) so the engine is reusable.
+  // Drops every admitted query and job (after a device error) so the engine
+  // is reusable.
   void reset_slots() {
     for (int i = 0; i < max_slots_; i++) {
       if (sh_[i].used) arena_.release(sh_[i].off, sh_[i].len);
-      sh_[i].used = false;
+      sh_[i] = SlotHost();
       memset(&h_slots_[i], 0, sizeof(gst::Slot));
     }
     active_ = 0;
+    used_slots_ = 0;
     nslots_used_ = 0;
+    pending_uploads_.clear();
+    job_members_.clear();
   }
 
   uint32_t last_steps() const { return last_steps_; }
   int64_t kernel_ns() const { return kernel_ns_; }
   int64_t launches() const { return launches_; }
   int64_t steps_total() const { return steps_total_; }
-  void reset_stats() { kernel_ns_ = launches_ = steps_total_ = 0; }
+  int64_t cls_ns(int c) const { return cls_ns_[c]; }
+  int64_t cls_launches() const { return cls_launches_; }
+  int64_t jobs_done() const { return jobs_done_; }
+  int64_t job_ctr(int t) const { return job_ctr_[t]; }
+  void reset_stats() {
+    kernel_ns_ = launches_ = steps_total_ = 0;
+    cls_ns_[0] = cls_ns_[1] = cls_launches_ = jobs_done_ = 0;
+    for (auto& x : job_ctr_) x = 0;
+  }
   size_t largest_free() const { return arena_.largest(); }
 
-  int grid_size() const {
-    int bpc = g_->occ;
-    if (g_->opt.blocks_per_cu > 0) bpc = std::min<int>((int)g_->opt.blocks_per_cu, g_->occ);
+  int grid_size(int cls) const {
+    int bpc = g_->occ[cls];
+    if (g_->opt.blocks_per_cu > 0) bpc = std::min<int>((int)g_->opt.blocks_per_cu, g_->occ[cls]);
     return std::max(1, bpc) * g_->num_cu;
   }
 
@@ -538,12 +749,19 @@ class Engine {
   std::vector<SlotHost> sh_;
   std::vector<uint32_t> step_before_;
   int nslots_used_ = 0;
-  int active_ = 0;
+  int used_slots_ = 0;  // queries and jobs
+  int active_ = 0;      // queries
   uint32_t last_steps_ = 0;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+  std::vector<hipEvent_t> tev_;  // per-step class timing (option time_steps)
   hipGraphExec_t gexec_ = nullptr;
-  int graph_grid_ = 0;
+  int graph_grid_[2] = {0, 0};
   int64_t kernel_ns_ = 0, launches_ = 0, steps_total_ = 0;
+  int64_t cls_ns_[2] = {0, 0}, cls_launches_ = 0, jobs_done_ = 0;
+  int64_t job_ctr_[6] = {0, 0, 0, 0, 0, 0};
+  // host copies of job tables, uploaded at the start of the next epoch
+  std::vector<std::pair<void*, std::vector<char>>> pending_uploads_;
+  std::map<int, std::vector<int>> job_members_;
 };
 
 namespace {
@@ -566,7 +784,7 @@ void parallel_for(int64_t n, F f) {
 gst_status engine_for(gst_graph* g, bool batch, size_t need, Engine** out) {
   std::unique_ptr<Engine>& e = batch ? g->batch : g->plain;
   if (!e) {
-    int slots = batch ? (int)g->opt.max_slots : 1;
+    int slots = batch ? (int)g->opt.max_slots : 2;  // plain: the query and its job
     std::unique_ptr<Engine> ne(new Engine(g, slots));
     size_t bytes = need;
     if (batch) {
@@ -586,27 +804,32 @@ gst_status engine_for(gst_graph* g, bool batch, size_t need, Engine** out) {
   return GST_OK;
 }
 
+// Device bytes one query needs while admitted, with the share of its
+// group-distance job when it has one (a job of its own when alone).
+size_t single_query_bytes(const gst_graph* g, int k, int D) {
+  if (!uses_job(g, k, D)) return layout_for(g, k, D, true).total;
+  return layout_for(g, k, D, false).total + job_layout(g, k, 1).total;
+}
+
 // Drives an engine over a list of queries: admits while memory and slots
-// allow, runs epochs, harvests finished slots, refills.
+// allow, runs epochs, harvests finished queries, refills.
 gst_status run_queries(gst_graph* g, Engine* eng, std::vector<QueryReq>& qs, gst_result* out) {
-  std::deque<size_t> pending;
-  for (size_t i = 0; i < qs.size(); i++) pending.push_back(i);
+  std::deque<QueryReq*> pend;
+  for (size_t i = 0; i < qs.size(); i++) {
+    qs[i].tag = i;
+    pend.push_back(&qs[i]);
+  }
   std::vector<int> fin;
   gst_status first_err = GST_OK;
-  while (!pending.empty() || eng->active() > 0) {
-    while (!pending.empty()) {
-      QueryReq& q = qs[pending.front()];
-      q.tag = pending.front();
-      if (eng->admit(q) < 0) {
-        if (eng->active() == 0)
-          return fail(GST_OUT_OF_MEMORY, "query of k=%zu D=%d does not fit the state arena",
-                      q.groups.size(), q.D);
-        break;
-      }
-      pending.pop_front();
+  while (!pend.empty() || eng->active() > 0) {
+    if (!pend.empty()) {
+      const int n = eng->admit_front(pend);
+      if (n == 0 && eng->active() == 0)
+        return fail(GST_OUT_OF_MEMORY, "query of k=%zu D=%d does not fit the state arena",
+                    pend.front()->groups.size(), pend.front()->D);
     }
     const int act = eng->active();
-    const uint32_t exit_after = pending.empty() ? (uint32_t)act : std::max(1u, (uint32_t)act / 8);
+    const uint32_t exit_after = pend.empty() ? (uint32_t)act : std::max(1u, (uint32_t)act / 8);
     gst_status st = eng->run_epoch(exit_after, kEpochMaxSteps, &fin);
     if (st != GST_OK) {
       eng->reset_slots();
@@ -701,8 +924,10 @@ gst_status gst_graph_create(const int64_t* beg_pos, int64_t V, const int32_t* ad
   hipDeviceProp_t prop;
   HIP_TRY(hipGetDeviceProperties(&prop, g->device));
   g->num_cu = prop.multiProcessorCount;
-  int occ = 1;
-  if (gst_step_occupancy(&occ) == hipSuccess && occ > 0) g->occ = occ;
+  for (int c = 0; c < gst::kNumCls; c++) {
+    int occ = 1;
+    if (gst_step_occupancy(c, &occ) == hipSuccess && occ > 0) g->occ[c] = occ;
+  }
   g->V = V;
   g->n_arcs = n_arcs;
   HIP_TRY(hipMalloc(&g->d_rowptr, sizeof(int64_t) * (V + 1)));
@@ -768,6 +993,8 @@ static int64_t* option_ref(gst_graph* g, const std::string& k) {
   if (k == "edge_cap") return &g->opt.edge_cap;
   if (k == "timeout_ms") return &g->opt.timeout_ms;
   if (k == "graph") return &g->opt.graph;
+  if (k == "group_pass") return &g->opt.group_pass;
+  if (k == "time_steps") return &g->opt.time_steps;
   return nullptr;
 }
 
@@ -804,9 +1031,23 @@ gst_status gst_get_option(gst_graph* g, const char* key, int64_t* value) {
     if (f == "kernel_ns") { *value = e ? e->kernel_ns() : 0; return GST_OK; }
     if (f == "launches") { *value = e ? e->launches() : 0; return GST_OK; }
     if (f == "steps") { *value = e ? e->steps_total() : 0; return GST_OK; }
+    if (f == "lean_ns") { *value = e ? e->cls_ns(gst::kClsLean) : 0; return GST_OK; }
+    if (f == "heavy_ns") { *value = e ? e->cls_ns(gst::kClsHeavy) : 0; return GST_OK; }
+    if (f == "timed_steps") { *value = e ? e->cls_launches() : 0; return GST_OK; }
+    if (f == "jobs") { *value = e ? e->jobs_done() : 0; return GST_OK; }
+    if (f == "job_entries") { *value = e ? e->job_ctr(1) : 0; return GST_OK; }
+    if (f == "job_arcs") { *value = e ? e->job_ctr(2) : 0; return GST_OK; }
+    if (f == "job_relaxations") { *value = e ? e->job_ctr(3) : 0; return GST_OK; }
+    if (f == "job_improvements") { *value = e ? e->job_ctr(4) : 0; return GST_OK; }
   }
-  if (k == "occupancy") { *value = g->occ; return GST_OK; }
-  if (k == "grid") { *value = (int64_t)std::max<int64_t>(1, g->opt.blocks_per_cu > 0 ? std::min<int64_t>(g->opt.blocks_per_cu, g->occ) : g->occ) * g->num_cu; return GST_OK; }
+  auto grid_of = [&](int c) {
+    const int64_t b = g->opt.blocks_per_cu > 0 ? std::min<int64_t>(g->opt.blocks_per_cu, g->occ[c]) : g->occ[c];
+    return std::max<int64_t>(1, b) * g->num_cu;
+  };
+  if (k == "occupancy") { *value = g->occ[gst::kClsLean]; return GST_OK; }
+  if (k == "occupancy_heavy") { *value = g->occ[gst::kClsHeavy]; return GST_OK; }
+  if (k == "grid") { *value = grid_of(gst::kClsLean); return GST_OK; }
+  if (k == "grid_heavy") { *value = grid_of(gst::kClsHeavy); return GST_OK; }
   int64_t* r = option_ref(g, key);
   if (!r) return fail(GST_INVALID_ARG, "unknown option '%s'", key);
   *value = *r;
@@ -815,7 +1056,7 @@ gst_status gst_get_option(gst_graph* g, const char* key, int64_t* value) {
 
 gst_status gst_query_bytes(gst_graph* g, int32_t k, int32_t D, int64_t* bytes) {
   if (!g || !bytes || k < 1 || k > GST_MAX_K || D < -1 || D > GST_MAX_D) return fail(GST_INVALID_ARG, "bad arguments");
-  *bytes = (int64_t)layout_for(g, k, D).total;
+  *bytes = (int64_t)single_query_bytes(g, k, D);
   return GST_OK;
 }
 
@@ -836,7 +1077,7 @@ gst_status gst_solve(gst_graph* g, const int32_t* group_ids, int32_t k, int32_t 
   }
   std::lock_guard<std::mutex> lk(g->dev_mu);
   Engine* eng = nullptr;
-  st = engine_for(g, false, layout_for(g, k, D).total, &eng);
+  st = engine_for(g, false, single_query_bytes(g, k, D), &eng);
   if (st != GST_OK) return st;
   std::vector<QueryReq> qs(1);
   qs[0].groups.assign(group_ids, group_ids + k);
@@ -914,26 +1155,28 @@ void server_loop(gst_server* s) {
       s->cv_new.wait(lk, [&] { return s->stop || !s->pending.empty() || s->eng->active() > 0; });
       if (s->stop && s->pending.empty() && s->eng->active() == 0) break;
       while (!s->pending.empty()) {
-        auto& pr = s->pending.front();
-        pr.second.tag = pr.first;
-        if (s->eng->admit(pr.second) < 0) {
-          if (s->eng->active() == 0) {
-            // can never fit: answer with an error result
-            gst_result r;
-            memset(&r, 0, sizeof(r));
-            r.status = GST_OUT_OF_MEMORY;
-            r.cost = GST_INFEASIBLE;
-            r.root = -1;
-            s->done[pr.first] = r;
-            s->known[pr.first] = 2;
-            s->pending.pop_front();
-            s->cv_done.notify_all();
-            continue;
-          }
-          break;
+        std::deque<QueryReq*> pend;
+        for (auto& pr : s->pending) {
+          if (pend.size() >= (size_t)gst::kMaxSlots) break;
+          pr.second.tag = pr.first;
+          pend.push_back(&pr.second);
         }
-        s->known[pr.first] = 1;
+        const int n = s->eng->admit_front(pend);
+        for (int i = 0; i < n; i++) {
+          s->known[s->pending.front().first] = 1;
+          s->pending.pop_front();
+        }
+        if (n > 0 || s->eng->active() > 0) break;
+        // the first pending query can never fit: answer it with an error result
+        gst_result r;
+        memset(&r, 0, sizeof(r));
+        r.status = GST_OUT_OF_MEMORY;
+        r.cost = GST_INFEASIBLE;
+        r.root = -1;
+        s->done[s->pending.front().first] = r;
+        s->known[s->pending.front().first] = 2;
         s->pending.pop_front();
+        s->cv_done.notify_all();
       }
     }
     if (s->eng->active() == 0) continue;

@@ -78,6 +78,7 @@ typedef struct {
   int64_t relaxations;      /* (arc, subset, hop) candidates evaluated */
   int64_t improvements;     /* successful packed atomicMin updates */
   int64_t merge_evals;      /* (split, hop pair) merge candidates evaluated */
+  int64_t live_vertices;    /* vertices whose full-set lower bound admits a tree <= the bound */
 } gst_counters;
 
 typedef struct {
@@ -127,9 +128,14 @@ gst_status gst_solve_batch(gst_graph *g, int64_t n, const gst_query *q,
 void gst_result_free(gst_result *r);
 
 /* Tunables: "max_slots", "budget_mb", "blocks_per_cu", "chunk",
- * "edge_cap", "timeout_ms", "graph" (replay engine steps as a hipGraph).
- * Read-only: "num_cu", "occupancy", "grid", "{plain,batch}_{kernel_ns,
- * launches,steps}".  Unknown keys return GST_INVALID_ARG. */
+ * "edge_cap", "timeout_ms", "graph" (replay engine steps as a hipGraph),
+ * "group_pass" (1: unbounded queries share group-distance jobs; 0: each
+ * query runs its own level 1), "time_steps" (time every step's kernels).
+ * Read-only: "num_cu", "occupancy", "occupancy_heavy", "grid",
+ * "grid_heavy", "{plain,batch}_{kernel_ns,launches,steps,lean_ns,heavy_ns,
+ * timed_steps,jobs,job_entries,job_arcs,job_relaxations,job_improvements}".
+ * "reset_stats" (set) clears the statistics.  Unknown keys return
+ * GST_INVALID_ARG. */
 gst_status gst_set_option(gst_graph *g, const char *key, int64_t value);
 gst_status gst_get_option(gst_graph *g, const char *key, int64_t *value);
 

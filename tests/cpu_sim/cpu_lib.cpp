@@ -1,5 +1,8 @@
 // Kernel launches of the CPU build of libgst: the engine's step and plan
-// kernels run as host threads (test infrastructure).
+// kernels run as host threads, one per lane (test infrastructure).  A step
+// kernel's waves only claim items, so the launcher starts no more waves than
+// the class has items (none when it has none): the surplus waves of a GPU
+// launch would claim nothing and exit.
 #define GST_CPU_SIM 1
 #define GST_CPU_SIM_HOST 1
 #include "gst_device.hip"
@@ -17,8 +20,8 @@ thread_local uint64_t collectives = 0;
 
 extern "C" hipError_t gst_launch_plan(gst::Slot* slots, gst::Ctrl* ctrl, uint64_t V, hipStream_t) {
   if (ctrl->exit_flag) return hipSuccess;
-  static uint32_t sh_items[gst::kMaxSlots];
-  static uint32_t sh_scan[2 * gst::kBlock];
+  static uint32_t sh_items[gst::kNumCls][gst::kMaxSlots];
+  static uint32_t sh_scan[gst::kNumCls][2][gst::kBlock];
   gsim::Block blk;
   std::vector<gsim::Wave> waves(gst::kWaves);
   std::vector<std::thread> th;
@@ -34,11 +37,13 @@ extern "C" hipError_t gst_launch_plan(gst::Slot* slots, gst::Ctrl* ctrl, uint64_
   return hipSuccess;
 }
 
-extern "C" hipError_t gst_launch_step(const gst::GraphDev* g, gst::Slot* slots, gst::Ctrl* ctrl, int grid,
-                                      hipStream_t) {
-  const int n_waves = grid * gst::kWaves;
+template <int CLS, class LDS>
+static void run_class(const gst::GraphDev* g, gst::Slot* slots, gst::Ctrl* ctrl, int grid) {
+  if (ctrl->exit_flag || ctrl->item_total[CLS] == 0) return;
+  int n_waves = grid * gst::kWaves;
+  if ((uint32_t)n_waves > ctrl->item_total[CLS]) n_waves = (int)ctrl->item_total[CLS];
   std::vector<gsim::Wave> waves(n_waves);
-  std::vector<gst::WaveLDS> lds(n_waves);
+  std::vector<LDS> lds(n_waves);
   std::vector<std::thread> th;
   const gst::GraphDev gd = *g;
   for (int w = 0; w < n_waves; w++)
@@ -47,13 +52,21 @@ extern "C" hipError_t gst_launch_step(const gst::GraphDev* g, gst::Slot* slots, 
         gsim::lane = l;
         gsim::wave = &waves[w];
         gsim::thread_idx.x = (unsigned)((w % gst::kWaves) * 64 + l);
-        gst::step_body(gd, slots, ctrl, &lds[w]);
+        gst::step_body<CLS>(gd, slots, ctrl, &lds[w]);
       });
   for (auto& x : th) x.join();
+}
+
+extern "C" hipError_t gst_launch_step(int cls, const gst::GraphDev* g, gst::Slot* slots, gst::Ctrl* ctrl,
+                                      int grid, hipStream_t) {
+  if (cls == gst::kClsLean)
+    run_class<gst::kClsLean, gst::FrontLDS>(g, slots, ctrl, grid);
+  else
+    run_class<gst::kClsHeavy, gst::WaveLDS>(g, slots, ctrl, grid);
   return hipSuccess;
 }
 
-extern "C" hipError_t gst_step_occupancy(int* blocks_per_cu) {
+extern "C" hipError_t gst_step_occupancy(int, int* blocks_per_cu) {
   *blocks_per_cu = 1;
   return hipSuccess;
 }

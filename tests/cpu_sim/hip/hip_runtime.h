@@ -43,6 +43,10 @@ extern thread_local uint64_t collectives;  // wave collectives executed by this 
 
 inline int __lane_id() { return gsim::lane; }
 
+struct alignas(16) uint4 {
+  uint32_t x, y, z, w;
+};
+
 inline uint64_t __gsim_exchange(uint64_t v, int src) {
   gsim::Wave* w = gsim::wave;
   gsim::collectives++;

@@ -1,0 +1,100 @@
+"""Engine checks run against the CPU build of libgst (test infrastructure).
+
+Started by tests/test_engine_cpu.py in its own process with GST_LIB pointing
+at tests/cpu_sim/_build/libgst_cpu.so.  Prints one JSON line of results.
+  tiny   : golden tiny cases (every `stride`-th), every D, one call each and
+           all D variants in one batch; costs against the fixture, trees
+           validated
+  jobs   : one batch of random k=5 queries (duplicates allowed) over several
+           group-distance jobs, against the oracle and against group_pass=0
+"""
+import json
+import os
+import random
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+import oracle as O  # noqa: E402
+from gpugst_amd import Graph, data  # noqa: E402
+from gpugst_amd.validate import validate_tree  # noqa: E402
+
+
+def groups_csr(groups):
+    gptr = np.zeros(len(groups) + 1, np.int64)
+    for i, m in enumerate(groups):
+        gptr[i + 1] = gptr[i] + len(m)
+    gmem = np.concatenate([np.asarray(m, np.int32) for m in groups])
+    return gptr, gmem
+
+
+def check_tiny(stride):
+    cases = json.load(open(os.path.join(ROOT, "tests", "golden", "tiny.json")))
+    bad, n = [], 0
+    for ci in range(0, len(cases), stride):
+        c = cases[ci]
+        rp, col, wt = data.csr_from_edges(c["V"], c["edges"])
+        g = Graph(rp, col, wt)
+        g.set_groups(*groups_csr(c["groups"]))
+        ids = list(range(len(c["groups"])))
+        todo = [(-1, c["cost"])] + [(int(D), hc) for D, hc in c["hop"].items()]
+        for D, exp in todo:
+            r = g.solve(ids, D=D)
+            n += 1
+            ok = r.cost == exp
+            if ok and r.feasible:
+                ok = validate_tree(rp, col, wt, c["groups"], r.cost, r.root, r.edges, D=D)["ok"]
+            if not ok:
+                bad.append([ci, D, r.cost, exp])
+        rs = g.solve_batch([ids] * len(todo), [d for d, _ in todo])
+        for (D, exp), r in zip(todo, rs):
+            n += 1
+            if r.cost != exp:
+                bad.append([ci, D, r.cost, exp, "batch"])
+        g.close()
+    return {"checked": n, "bad": bad}
+
+
+def check_jobs(V, nq, seed):
+    rng = random.Random(seed)
+    edges = set()
+    for v in range(1, V):
+        edges.add((rng.randrange(v), v))
+    while len(edges) < 3 * V:
+        a, b = rng.randrange(V), rng.randrange(V)
+        if a != b:
+            edges.add((min(a, b), max(a, b)))
+    E = [(a, b, rng.randint(1, 20)) for a, b in sorted(edges)]
+    rp, col, wt = data.csr_from_edges(V, E)
+    G = 40
+    groups = [sorted(rng.sample(range(V), rng.randint(1, 4))) for _ in range(G)]
+    qs = [[rng.randrange(G) for _ in range(5)] for _ in range(nq)]
+    g = Graph(rp, col, wt)
+    g.set_groups(*groups_csr(groups))
+    rs = g.solve_batch(qs)
+    g.set_option("group_pass", 0)
+    r0 = g.solve_batch(qs)
+    bad = []
+    for i, q in enumerate(qs):
+        gl = [groups[x] for x in q]
+        exp = O.dpbf(rp, col, wt, gl)[0]
+        ok = rs[i].cost == exp and r0[i].cost == exp
+        if ok and rs[i].feasible:
+            ok = validate_tree(rp, col, wt, gl, rs[i].cost, rs[i].root, rs[i].edges)["ok"]
+        if not ok:
+            bad.append([i, rs[i].cost, r0[i].cost, exp])
+    return {"checked": nq, "bad": bad, "labels": 5 * nq}
+
+
+def main():
+    stride = int(sys.argv[1]) if len(sys.argv) > 1 else 3
+    nq = int(sys.argv[2]) if len(sys.argv) > 2 else 60
+    out = {"tiny": check_tiny(stride), "jobs": check_jobs(150, nq, 7)}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

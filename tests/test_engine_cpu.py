@@ -1,0 +1,45 @@
+"""The whole engine (device logic + host engine) on the CPU build of libgst.
+
+No GPU is needed: tests/cpu_sim compiles the HIP device code and the host
+engine for the host (wave lanes as threads, host memory as device memory).
+This pins the engine's logic -- group-distance jobs (multi-label level 1,
+SCATTER, shortest-path reconstruction from C), the hop-bounded per-query level
+1, batching and admission over several jobs -- against the golden fixtures and
+the oracle.  The GPU parity tests check the same properties on the MI355X.
+"""
+import json
+import os
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+
+@pytest.fixture(scope="module")
+def cpu_results():
+    import cpu_sim
+    try:
+        lib = cpu_sim.lib_path()
+    except (OSError, subprocess.CalledProcessError) as e:  # no g++
+        pytest.skip(f"CPU build unavailable: {e}")
+    env = dict(os.environ, GST_LIB=lib)
+    # every 6th tiny case (all D), and 54 k=5 queries = 270 labels = two jobs
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "tests", "cpu_sim", "run_checks.py"), "6", "54"],
+                       env=env, capture_output=True, text=True, timeout=900)
+    assert r.returncode == 0, r.stderr[-2000:]
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def test_tiny_cases_on_cpu_build(cpu_results):
+    t = cpu_results["tiny"]
+    assert t["checked"] > 100
+    assert t["bad"] == [], t["bad"]
+
+
+def test_group_distance_jobs_on_cpu_build(cpu_results):
+    j = cpu_results["jobs"]
+    assert j["labels"] > 256  # more than one job
+    assert j["bad"] == [], j["bad"]
