@@ -184,6 +184,8 @@ def main():
                 "pass": f"{len(rr)} further queries of the same workload, events around every step kernel"}
         log(f"roofline pass: {n_steps} steps, lean {lean_ms:.3f} ms/launch, heavy {heavy_ms:.3f} ms/launch, "
             f"{dom} {achieved:.1f} GB/s algorithmic")
+        log("job counters: " + " ".join(f"{f}={g.get_option('batch_job_' + f)}" for f in
+                                        ("entries", "arcs", "relaxations", "improvements", "atomics")))
 
     # ---------------- parity: plain vs batched on shared queries, trees valid
     all_b = bres + timed

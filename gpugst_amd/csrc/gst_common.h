@@ -46,7 +46,8 @@ constexpr int kStageWords = 1152;       // per-wave LDS staging (u64): 64*ND row
 // them (u32, padded to whole 4-label chunks), so an arc relaxation reads and
 // writes coalesced 16-byte chunks shared by every query of the job.
 constexpr int kJobLabels = 256;         // labels per job: 64 chunks, one dirty bit each
-constexpr uint32_t kJobIter = 64;       // wave iterations per job ROUND item
+constexpr uint32_t kJobArcs = 256;      // arcs per job ROUND item
+constexpr int kJobChunksPerLane = 4;    // 4-label chunks one lane relaxes per arc
 constexpr uint32_t kScatterTV = 64;     // vertices per SCATTER item
 
 // work classes: each has its own kernel, so the arc-parallel items do not run

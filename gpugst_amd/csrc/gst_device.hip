@@ -450,16 +450,18 @@ __device__ void item_jseed(const GraphDev& g, Slot* sp, uint32_t l) {
 }
 
 // one label of a job relaxation: target word tw holds t, candidate s + w
-__device__ __forceinline__ bool jrelax(uint32_t s, uint32_t t, uint32_t w, uint32_t* tw) {
+__device__ __forceinline__ bool jrelax(uint32_t s, uint32_t t, uint32_t w, uint32_t* tw, uint32_t& n_at) {
   if (s >= kInf) return false;
   const uint32_t cand = s + w;
   if (cand >= t || cand >= kInf) return false;
+  n_at++;
   return atomicMin(tw, cand) > cand;
 }
 
-// Job ROUND: one chunk of arcs of the job's frontier.  G lanes share an arc;
-// lane r of the group relaxes the dirty 4-label chunks r, r + G, ... of the
-// source row into the target row (16-byte loads, coalesced over the group).
+// Job ROUND: one chunk of arcs of the job's frontier.  G lanes share an arc
+// (64 / G arcs per wave iteration); lane r of the group relaxes the dirty
+// 4-label chunks r, r + G, ... of the source row into the target row (16-byte
+// loads, coalesced over the group).
 __device__ void item_jround(const GraphDev& g, Slot* sp, uint32_t item, FrontLDS* L) {
   const int lane = lane_id();
   const uint64_t V = (uint64_t)g.V;
@@ -482,6 +484,7 @@ __device__ void item_jround(const GraphDev& g, Slot* sp, uint32_t item, FrontLDS
   const int APW = 64 / G;
   const int sub = lane / G, rr = lane - sub * G;
   uint64_t n_relax = 0, n_impr = 0;
+  uint32_t n_at = 0;
   uint32_t a = a0;
   for (uint32_t i = sp->istart[(uint64_t)cur * sp->istride + item]; a < a1; i += 64) {
     stage_front(g, L, i, n, arcs_total, lst, lof, mcur);
@@ -506,10 +509,10 @@ __device__ void item_jround(const GraphDev& g, Slot* sp, uint32_t item, FrontLDS
           if (!((m >> c) & 1ull)) continue;
           const uint4 sv = srow[c];
           const uint4 tv = trow[c];
-          bool im = jrelax(sv.x, tv.x, w, tw + 4 * c);
-          im |= jrelax(sv.y, tv.y, w, tw + 4 * c + 1);
-          im |= jrelax(sv.z, tv.z, w, tw + 4 * c + 2);
-          im |= jrelax(sv.w, tv.w, w, tw + 4 * c + 3);
+          bool im = jrelax(sv.x, tv.x, w, tw + 4 * c, n_at);
+          im |= jrelax(sv.y, tv.y, w, tw + 4 * c + 1, n_at);
+          im |= jrelax(sv.z, tv.z, w, tw + 4 * c + 2, n_at);
+          im |= jrelax(sv.w, tv.w, w, tw + 4 * c + 3, n_at);
           n_relax += 4;
           if (im) {
             imp |= 1ull << c;
@@ -531,9 +534,11 @@ __device__ void item_jround(const GraphDev& g, Slot* sp, uint32_t item, FrontLDS
   }
   n_relax = wave_sum_u64(n_relax);
   n_impr = wave_sum_u64(n_impr);
+  const uint64_t n_atom = wave_sum_u64(n_at);
   if (lane == 0) {
     atomicAdd(&sp->ctr[3], (unsigned long long)n_relax);
     atomicAdd(&sp->ctr[4], (unsigned long long)n_impr);
+    atomicAdd(&sp->ctr[5], (unsigned long long)n_atom);  // job: atomicMin issued
   }
 }
 
@@ -1275,6 +1280,10 @@ __device__ void step_body(const GraphDev& g, Slot* slots, Ctrl* ctrl, LDS* L) {
   const uint64_t V = (uint64_t)g.V;
   if (ctrl->exit_flag) return;
   const uint32_t total = ctrl->item_total[CLS];
+  // waves beyond the item count have nothing to claim: leave before touching
+  // the shared counter (a small step then costs one atomic per item, not one
+  // per wave of the grid)
+  if (blockIdx.x * (uint32_t)kWaves + (threadIdx.x >> 6) >= total) return;
   const uint32_t nact = ctrl->n_active[CLS];
   const uint32_t* prefix = ctrl->prefix[CLS];
   const uint32_t* active = ctrl->active[CLS];

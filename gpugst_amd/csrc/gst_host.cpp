@@ -219,10 +219,13 @@ JobLayout job_layout(const gst_graph* g, int L, int nmem) {
   const size_t V = (size_t)g->V;
   JobLayout J;
   J.Lp = (L + 3) & ~3;
+  // lanes per arc: the smallest power of two that gives each lane at most
+  // kJobChunksPerLane 4-label chunks (256 labels: 16 lanes, 4 arcs per wave
+  // iteration; 5 labels: 1 lane, 64 arcs)
   const int nch = J.Lp / 4;
   J.G = 1;
-  while (J.G < nch) J.G <<= 1;
-  J.chunk = (int)gst::kJobIter * (64 / J.G);
+  while (J.G * gst::kJobChunksPerLane < nch) J.G <<= 1;
+  J.chunk = (int)gst::kJobArcs;
   J.istride = istride_for(g, J.chunk);
   size_t off = 0;
   J.dist = off;   off += align256(V * (size_t)J.Lp * 4);
@@ -1039,6 +1042,7 @@ gst_status gst_get_option(gst_graph* g, const char* key, int64_t* value) {
     if (f == "job_arcs") { *value = e ? e->job_ctr(2) : 0; return GST_OK; }
     if (f == "job_relaxations") { *value = e ? e->job_ctr(3) : 0; return GST_OK; }
     if (f == "job_improvements") { *value = e ? e->job_ctr(4) : 0; return GST_OK; }
+    if (f == "job_atomics") { *value = e ? e->job_ctr(5) : 0; return GST_OK; }
   }
   auto grid_of = [&](int c) {
     const int64_t b = g->opt.blocks_per_cu > 0 ? std::min<int64_t>(g->opt.blocks_per_cu, g->occ[c]) : g->occ[c];

@@ -133,7 +133,8 @@ void gst_result_free(gst_result *r);
  * query runs its own level 1), "time_steps" (time every step's kernels).
  * Read-only: "num_cu", "occupancy", "occupancy_heavy", "grid",
  * "grid_heavy", "{plain,batch}_{kernel_ns,launches,steps,lean_ns,heavy_ns,
- * timed_steps,jobs,job_entries,job_arcs,job_relaxations,job_improvements}".
+ * timed_steps,jobs,job_entries,job_arcs,job_relaxations,job_improvements,
+ * job_atomics}".
  * "reset_stats" (set) clears the statistics.  Unknown keys return
  * GST_INVALID_ARG. */
 gst_status gst_set_option(gst_graph *g, const char *key, int64_t value);
