@@ -49,6 +49,7 @@ struct FrontLDS {
   uint32_t wrs[64];
   uint32_t pad[64];
   uint64_t wm[64];
+  uint32_t av[128];  // vertices appended by this wave, not yet published
 };
 
 // per-wave LDS of the vertex-segment items
@@ -206,6 +207,71 @@ __device__ __forceinline__ void wave_append(bool flag, uint32_t u, Slot* sp, int
   }
 }
 
+// Staged append for the ROUND items: appended vertices collect in the wave's
+// LDS and are published in batches of 64 or more with one atomicAdd, so the
+// waves of a dense round do not queue on the frontier counter once per
+// iteration.  flush_append must run before the item ends.
+__device__ void flush_append(FrontLDS* L, uint32_t& apn, Slot* sp, int buf, const GraphDev& g, uint64_t V) {
+  if (apn == 0) return;
+  wave_sync();
+  const int lane = lane_id();
+  uint32_t u0 = 0, u1 = 0, d0 = 0, d1 = 0;
+  if ((uint32_t)lane < apn) {
+    u0 = L->av[lane];
+    d0 = (uint32_t)(g.rowptr[u0 + 1] - g.rowptr[u0]);
+  }
+  if ((uint32_t)lane + 64 < apn) {
+    u1 = L->av[lane + 64];
+    d1 = (uint32_t)(g.rowptr[u1 + 1] - g.rowptr[u1]);
+  }
+  const uint32_t e0 = wave_excl_scan(d0);
+  const uint32_t t0 = __shfl(e0 + d0, 63, 64);
+  const uint32_t e1 = wave_excl_scan(d1);
+  const uint32_t t1 = __shfl(e1 + d1, 63, 64);
+  // the publishing lane comes from a ballot, so no other lane-0 test of the
+  // caller can be merged with this one
+  const int leader = __ffsll((long long)__ballot(1)) - 1;
+  unsigned long long old = 0;
+  if (lane == leader)
+    old = atomicAdd((unsigned long long*)&sp->app[buf], ((unsigned long long)apn << 32) | (t0 + t1));
+  old = __shfl(old, leader, 64);
+  const uint32_t pos0 = (uint32_t)(old >> 32), off0 = (uint32_t)old;
+  const uint32_t ch = (uint32_t)sp->chunk;
+  uint32_t* ist = sp->istart + (uint64_t)buf * sp->istride;
+  for (int h = 0; h < 2; h++) {
+    const uint32_t idx = (uint32_t)lane + 64u * (uint32_t)h;
+    if (idx >= apn) continue;
+    const uint32_t u = h ? u1 : u0, deg = h ? d1 : d0;
+    const uint32_t pos = pos0 + idx;
+    const uint32_t off = off0 + (h ? t0 + e1 : e0);
+    sp->list[(uint64_t)buf * V + pos] = u;
+    sp->loff[(uint64_t)buf * V + pos] = off;
+    for (uint32_t c = (off + ch - 1) / ch; c * ch < off + deg; c++) ist[c] = pos;
+  }
+  apn = 0;
+  wave_sync();
+}
+
+__device__ __forceinline__ void stage_append(bool flag, uint32_t u, FrontLDS* L, uint32_t& apn, Slot* sp,
+                                             int buf, const GraphDev& g, uint64_t V) {
+  const uint64_t ball = __ballot(flag);
+  if (ball == 0) return;
+  if (flag) L->av[apn + __popcll(ball & __lanemask_lt())] = u;
+  apn += (uint32_t)__popcll(ball);
+  if (apn >= 64) flush_append(L, apn, sp, buf, g, V);
+}
+
+// Sets the bits `imp` of the next round's dirty mask of u; true when u had
+// none (u then joins the next frontier).  A plain read first skips the atomic
+// when the bits are already set, as they soon are on a hub that many arcs
+// improve in one round.
+__device__ __forceinline__ bool mark_dirty(uint64_t* mnxt, uint32_t u, uint64_t imp) {
+  const uint64_t seen = __hip_atomic_load(&mnxt[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if ((seen & imp) == imp) return false;
+  const unsigned long long om = atomicOr((unsigned long long*)&mnxt[u], (unsigned long long)imp);
+  return om == 0;
+}
+
 // ---------------------------------------------------------------- items ----
 
 // u64 words that FILL initialises: job [distances][masks]; query [S1][masks]
@@ -341,6 +407,7 @@ __device__ void item_round(const GraphDev& g, Slot* sp, uint32_t chunk, FrontLDS
   const uint32_t* R = sp->R;
 
   uint64_t n_relax = 0, n_impr = 0;
+  uint32_t apn = 0;
   uint32_t a = a0;
   for (uint32_t i = sp->istart[(uint64_t)cur * sp->istride + chunk]; a < a1; i += 64) {
     stage_front(g, L, i, n, arcs_total, lst, lof, mcur);
@@ -407,16 +474,14 @@ __device__ void item_round(const GraphDev& g, Slot* sp, uint32_t chunk, FrontLDS
             }
           }
         }
-        if (improved) {
-          const unsigned long long om = atomicOr((unsigned long long*)&mnxt[u], improved);
-          app = (om == 0);
-        }
+        if (improved) app = mark_dirty(mnxt, u, improved);
       }
-      wave_append(app, u, sp, nxt, g, V);
+      stage_append(app, u, L, apn, sp, nxt, g, V);
     }
     a = hia;
     wave_sync();
   }
+  flush_append(L, apn, sp, nxt, g, V);
   n_relax = wave_sum_u64(n_relax);
   n_impr = wave_sum_u64(n_impr);
   if (lane == 0) {
@@ -449,13 +514,20 @@ __device__ void item_jseed(const GraphDev& g, Slot* sp, uint32_t l) {
   }
 }
 
-// one label of a job relaxation: target word tw holds t, candidate s + w
+// One label of a job relaxation: target word tw held t when read, the
+// candidate is s + w.  The atomicMin returns nothing, so the wave never waits
+// on it (arcs into a hub all hit the same row); the chunk counts as improved
+// when the candidate beat the value read.  That may mark a chunk whose value
+// another arc lowered further in the meantime: it is relaxed once more in the
+// next round, with its current value.  Every mark follows a strict decrease
+// of an integer, so the rounds still end.
 __device__ __forceinline__ bool jrelax(uint32_t s, uint32_t t, uint32_t w, uint32_t* tw, uint32_t& n_at) {
   if (s >= kInf) return false;
   const uint32_t cand = s + w;
   if (cand >= t || cand >= kInf) return false;
   n_at++;
-  return atomicMin(tw, cand) > cand;
+  (void)__hip_atomic_fetch_min(tw, cand, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return true;
 }
 
 // Job ROUND: one chunk of arcs of the job's frontier.  G lanes share an arc
@@ -484,7 +556,7 @@ __device__ void item_jround(const GraphDev& g, Slot* sp, uint32_t item, FrontLDS
   const int APW = 64 / G;
   const int sub = lane / G, rr = lane - sub * G;
   uint64_t n_relax = 0, n_impr = 0;
-  uint32_t n_at = 0;
+  uint32_t n_at = 0, apn = 0;
   uint32_t a = a0;
   for (uint32_t i = sp->istart[(uint64_t)cur * sp->istride + item]; a < a1; i += 64) {
     stage_front(g, L, i, n, arcs_total, lst, lof, mcur);
@@ -522,16 +594,13 @@ __device__ void item_jround(const GraphDev& g, Slot* sp, uint32_t item, FrontLDS
       }
       // the improved chunks of one arc, gathered on the group's first lane
       for (int o2 = 1; o2 < G; o2 <<= 1) imp |= (uint64_t)__shfl_xor((unsigned long long)imp, o2, 64);
-      bool app = false;
-      if (pos < hia && rr == 0 && imp) {
-        const unsigned long long om = atomicOr((unsigned long long*)&mnxt[u], (unsigned long long)imp);
-        app = (om == 0);
-      }
-      wave_append(app, u, sp, nxt, g, V);
+      const bool app = (pos < hia && rr == 0 && imp) ? mark_dirty(mnxt, u, imp) : false;
+      stage_append(app, u, L, apn, sp, nxt, g, V);
     }
     a = hia;
     wave_sync();
   }
+  flush_append(L, apn, sp, nxt, g, V);
   n_relax = wave_sum_u64(n_relax);
   n_impr = wave_sum_u64(n_impr);
   const uint64_t n_atom = wave_sum_u64(n_at);

@@ -115,6 +115,8 @@ template <class T>
 inline T __hip_atomic_load(const T* p, int, int) { return __atomic_load_n(p, __ATOMIC_SEQ_CST); }
 template <class T, class U>
 inline void __hip_atomic_store(T* p, U v, int, int) { __atomic_store_n(p, (T)v, __ATOMIC_SEQ_CST); }
+template <class T, class U>
+inline T __hip_atomic_fetch_min(T* p, U v, int, int) { return atomicMin(p, v); }
 #endif
 
 inline void __threadfence() { __atomic_thread_fence(__ATOMIC_SEQ_CST); }
