@@ -52,7 +52,8 @@ class c_counters(ctypes.Structure):
     _fields_ = [("steps", ctypes.c_int64), ("frontier_entries", ctypes.c_int64),
                 ("arcs", ctypes.c_int64), ("relaxations", ctypes.c_int64),
                 ("improvements", ctypes.c_int64), ("merge_evals", ctypes.c_int64),
-                ("live_vertices", ctypes.c_int64)]
+                ("live_vertices", ctypes.c_int64), ("shared_entries", ctypes.c_int64),
+                ("shared_arcs", ctypes.c_int64), ("shared_relaxations", ctypes.c_int64)]
 
 
 class c_result(ctypes.Structure):

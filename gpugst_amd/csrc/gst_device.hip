@@ -1356,58 +1356,68 @@ __device__ void step_body(const GraphDev& g, Slot* slots, Ctrl* ctrl, LDS* L) {
   const uint32_t nact = ctrl->n_active[CLS];
   const uint32_t* prefix = ctrl->prefix[CLS];
   const uint32_t* active = ctrl->active[CLS];
+  // A wave claims a run of `run` consecutive items per atomic: all waves
+  // claim from one counter, whose atomics serialise at its L2 bank, so a step
+  // with many small items (LIVE: one per 256 vertices of every query) would
+  // otherwise queue on it.  The run leaves about 8 claims per wave.
+  const uint32_t nwaves = gridDim.x * (uint32_t)kWaves;
+  uint32_t run = total / (8u * nwaves);
+  run = run < 1u ? 1u : (run > 16u ? 16u : run);
   for (;;) {
-    // Claim the next item for the whole wave.  Every lane executes the atomic
-    // (only lane 0 adds one) and the index is read from lane 0 into a scalar
-    // register, so the claim sits under no lane-dependent branch and the loop
-    // test is uniform.  With the claim under `if (lane == 0)`, the optimizer
-    // threads the lane != 0 edge of the `if (lane == 0)` blocks that end
-    // several items straight to the broadcast: the loop splits in two and
-    // lanes 1..63 re-run the same item without lane 0, never finishing.
-    const uint32_t mine = atomicAdd(&ctrl->item_next[CLS], lane == 0 ? 1u : 0u);
-    const uint32_t it = __builtin_amdgcn_readfirstlane(mine);
-    if (it >= total) break;
-    uint32_t lo = 0, hi = nact;  // last active slot whose prefix <= it
-    while (hi - lo > 1) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (prefix[mid] <= it) lo = mid; else hi = mid;
-    }
-    Slot* sp = &slots[active[lo]];
-    const uint32_t local = it - prefix[lo];
-    if constexpr (CLS == kClsLean) {
-      const uint32_t nwork = sp->cls == kClsLean ? sp->n_work : 0u;
-      if (local >= nwork) {
-        item_clear(sp, local - nwork, V);
-        continue;
+    // Claim items for the whole wave.  Every lane executes the atomic (only
+    // lane 0 adds) and the index is read from lane 0 into a scalar register,
+    // so the claim sits under no lane-dependent branch and the loop test is
+    // uniform.  With the claim under `if (lane == 0)`, the optimizer threads
+    // the lane != 0 edge of the `if (lane == 0)` blocks that end several items
+    // straight to the broadcast: the loop splits in two and lanes 1..63 re-run
+    // the same item without lane 0, never finishing.
+    const uint32_t mine = atomicAdd(&ctrl->item_next[CLS], lane == 0 ? run : 0u);
+    const uint32_t it0 = __builtin_amdgcn_readfirstlane(mine);
+    if (it0 >= total) break;
+    const uint32_t it1 = it0 + run < total ? it0 + run : total;
+    for (uint32_t it = it0; it < it1; it++) {
+      uint32_t lo = 0, hi = nact;  // last active slot whose prefix <= it
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (prefix[mid] <= it) lo = mid; else hi = mid;
       }
-      if (sp->kind == KIND_JOB) {
-        switch (sp->phase) {
-          case PH_FILL: item_fill(sp, local, V); break;
-          case PH_SEED: item_jseed(g, sp, local); break;
-          case PH_ROUND: item_jround(g, sp, local, (FrontLDS*)L); break;
-          case PH_SCATTER: item_jscatter(slots, sp, local, V); break;
-          default: break;
+      Slot* sp = &slots[active[lo]];
+      const uint32_t local = it - prefix[lo];
+      if constexpr (CLS == kClsLean) {
+        const uint32_t nwork = sp->cls == kClsLean ? sp->n_work : 0u;
+        if (local >= nwork) {
+          item_clear(sp, local - nwork, V);
+          continue;
+        }
+        if (sp->kind == KIND_JOB) {
+          switch (sp->phase) {
+            case PH_FILL: item_fill(sp, local, V); break;
+            case PH_SEED: item_jseed(g, sp, local); break;
+            case PH_ROUND: item_jround(g, sp, local, (FrontLDS*)L); break;
+            case PH_SCATTER: item_jscatter(slots, sp, local, V); break;
+            default: break;
+          }
+        } else {
+          switch (sp->phase) {
+            case PH_FILL: item_fill(sp, local, V); break;
+            case PH_SEED: item_seed(g, sp, local); break;
+            default: break;
+          }
         }
       } else {
+        const uint32_t nwork = sp->n_work;
         switch (sp->phase) {
-          case PH_FILL: item_fill(sp, local, V); break;
-          case PH_SEED: item_seed(g, sp, local); break;
+          case PH_DIST:
+            if (local + 1 == nwork) item_vvr(g, sp, (WaveLDS*)L);
+            else item_dist(g, sp, local);
+            break;
+          case PH_ROUND: item_round(g, sp, local, &((WaveLDS*)L)->f); break;
+          case PH_LIVE: item_live(sp, local, V, (WaveLDS*)L); break;
+          case PH_MERGE: item_merge(g, sp, local, (WaveLDS*)L); break;
+          case PH_FINAL: item_final(g, sp, local, (WaveLDS*)L); break;
+          case PH_RECON: item_recon(g, sp); break;
           default: break;
         }
-      }
-    } else {
-      const uint32_t nwork = sp->n_work;
-      switch (sp->phase) {
-        case PH_DIST:
-          if (local + 1 == nwork) item_vvr(g, sp, (WaveLDS*)L);
-          else item_dist(g, sp, local);
-          break;
-        case PH_ROUND: item_round(g, sp, local, &((WaveLDS*)L)->f); break;
-        case PH_LIVE: item_live(sp, local, V, (WaveLDS*)L); break;
-        case PH_MERGE: item_merge(g, sp, local, (WaveLDS*)L); break;
-        case PH_FINAL: item_final(g, sp, local, (WaveLDS*)L); break;
-        case PH_RECON: item_recon(g, sp); break;
-        default: break;
       }
     }
   }

@@ -112,7 +112,7 @@ class Arena {
 };
 
 struct Options {
-  int64_t max_slots = 256;
+  int64_t max_slots = 512;  // queries and group-distance jobs admitted at once
   int64_t budget_mb = 0;  // 0: most of the free device memory
   int64_t blocks_per_cu = 0;  // 0: occupancy maximum
   int64_t chunk = 512;
@@ -664,12 +664,15 @@ class Engine {
     out->root = d.out_root;
     out->status = GST_OK;
     out->counters.steps = (int64_t)d.ctr[0] + h.extra[0];
-    out->counters.frontier_entries = (int64_t)d.ctr[1] + h.extra[1];
-    out->counters.arcs = (int64_t)d.ctr[2] + h.extra[2];
-    out->counters.relaxations = (int64_t)d.ctr[3] + h.extra[3];
-    out->counters.improvements = (int64_t)d.ctr[4] + h.extra[4];
+    out->counters.frontier_entries = (int64_t)d.ctr[1];
+    out->counters.arcs = (int64_t)d.ctr[2];
+    out->counters.relaxations = (int64_t)d.ctr[3];
+    out->counters.improvements = (int64_t)d.ctr[4];
     out->counters.merge_evals = (int64_t)d.ctr[5];
     out->counters.live_vertices = (int64_t)d.nlive;
+    out->counters.shared_entries = h.extra[1];
+    out->counters.shared_arcs = h.extra[2];
+    out->counters.shared_relaxations = h.extra[3];
     gst_status st = GST_OK;
     if (d.status == 1) {
       out->status = GST_DEVICE_ERROR;

@@ -70,15 +70,20 @@ typedef struct {
 
 /* Per-query device counters.  The reference's `records{process_queue_num,
  * counts}` (code/GPUGST/include/DPQ.cuh:59-62) map to frontier_entries and
- * states_written. */
+ * improvements.  The first five count the query's own work; a query whose
+ * level 1 came from a shared group-distance job (unbounded, k >= 2) also gets
+ * its share of that job (per label) in the shared_ fields. */
 typedef struct {
-  int64_t steps;            /* engine steps (rounds + merges + phases) */
+  int64_t steps;            /* engine steps (rounds + merges + phases), incl. its job's */
   int64_t frontier_entries; /* frontier vertices processed */
   int64_t arcs;             /* CSR arcs scanned */
   int64_t relaxations;      /* (arc, subset, hop) candidates evaluated */
   int64_t improvements;     /* successful packed atomicMin updates */
   int64_t merge_evals;      /* (split, hop pair) merge candidates evaluated */
   int64_t live_vertices;    /* vertices whose full-set lower bound admits a tree <= the bound */
+  int64_t shared_entries;   /* share of its job's frontier entries */
+  int64_t shared_arcs;      /* share of its job's arcs */
+  int64_t shared_relaxations; /* share of its job's label relaxations */
 } gst_counters;
 
 typedef struct {

@@ -16,6 +16,7 @@ thread_local Wave* wave = nullptr;
 thread_local Block* block = nullptr;
 thread_local ThreadIdx thread_idx = {0};
 thread_local ThreadIdx block_idx = {0};
+thread_local ThreadIdx grid_dim = {1};
 thread_local uint64_t collectives = 0;
 }  // namespace gsim
 
@@ -54,6 +55,7 @@ static void run_class(const gst::GraphDev* g, gst::Slot* slots, gst::Ctrl* ctrl,
         gsim::wave = &waves[w];
         gsim::thread_idx.x = (unsigned)((w % gst::kWaves) * 64 + l);
         gsim::block_idx.x = (unsigned)(w / gst::kWaves);
+        gsim::grid_dim.x = (unsigned)((n_waves + gst::kWaves - 1) / gst::kWaves);
         gst::step_body<CLS>(gd, slots, ctrl, &lds[w]);
       });
   for (auto& x : th) x.join();

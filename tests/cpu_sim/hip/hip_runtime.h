@@ -37,11 +37,13 @@ extern thread_local Wave* wave;
 extern thread_local Block* block;
 extern thread_local ThreadIdx thread_idx;
 extern thread_local ThreadIdx block_idx;
+extern thread_local ThreadIdx grid_dim;
 extern thread_local uint64_t collectives;  // wave collectives executed by this lane
 }  // namespace gsim
 
 #define threadIdx gsim::thread_idx
 #define blockIdx gsim::block_idx
+#define gridDim gsim::grid_dim
 
 inline int __lane_id() { return gsim::lane; }
 
