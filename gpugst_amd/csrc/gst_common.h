@@ -48,11 +48,11 @@ constexpr int kStageWords = 1152;       // per-wave LDS staging (u64): 64*ND row
 constexpr int kJobLabels = 256;         // labels per job: 64 chunks, one dirty bit each
 constexpr uint32_t kJobArcs = 256;      // arcs per job ROUND item
 constexpr int kJobChunksPerLane = 4;    // 4-label chunks one lane relaxes per arc
-constexpr uint32_t kScatterTV = 64;     // vertices per SCATTER item
+constexpr uint32_t kStarTV = 64;        // vertices per job STAR item
 
 // work classes: each has its own kernel, so the arc-parallel items do not run
 // at the register budget (and occupancy) of the merge items
-constexpr int kClsLean = 0;   // FILL, SEED, job ROUND, SCATTER, CLEAR
+constexpr int kClsLean = 0;   // FILL, SEED, job ROUND, STAR, CLEAR
 constexpr int kClsHeavy = 1;  // query ROUND, DIST, LIVE, MERGE, FINAL, RECON
 constexpr int kNumCls = 2;
 
@@ -68,7 +68,7 @@ enum Phase : int32_t {
   PH_DONE = 8,
   PH_LIVE = 9,   // live-vertex list + initialisation of their T segments
   PH_WAITJ = 10, // query waiting for its group-distance job (no work items)
-  PH_SCATTER = 11  // job: per-query distance tables C and star bounds
+  PH_STAR = 11     // job: star bound of every member query
 };
 
 enum SlotKind : int32_t {
@@ -104,7 +104,7 @@ struct alignas(64) Slot {
   uint32_t istride;     // entries per buffer of istart
   int32_t L, Lp, G;     // job: labels, padded row length, lanes per arc (power of two)
   int32_t nmem;         // job: member queries
-  int32_t pad0;
+  uint32_t cstride;     // words per vertex row of C (k, or the job's Lp)
   uint64_t nT;          // V * ND * W
   uint64_t* T;          // states with |p| >= 2; only live vertices' segments are touched
   uint64_t* S1;         // V * ND * k singleton states T(v, {i}, d); null when C comes from a job
@@ -115,7 +115,8 @@ struct alignas(64) Slot {
   uint32_t* live;       // V: vertices that can lie on a tree cheaper than the bound
   uint8_t* lvf;         // V: live flag per vertex
   uint64_t* mask;       // 3 * V (triple-buffered dirty sub-masks)
-  uint32_t* C;          // V * k group->vertex distances (padded to even)
+  uint32_t* C;          // group->vertex distances c_i(v) = C[v * cstride + i]: the query's own
+                        // V * k table, or its k columns of its job's distance rows
   uint32_t* list;       // 3 * V frontier vertices
   uint32_t* loff;       // 3 * V arc offsets of frontier entries
   uint32_t* R;          // W * k * k route table

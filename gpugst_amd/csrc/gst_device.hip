@@ -14,8 +14,9 @@
 //              query itself (hop-bounded DP: T(v,{i},d) per hop depth d; the
 //              reference's dis0_init_1 SSSP, code/GPUGST/src/DPQ.cu:241-271)
 //              or, for unbounded queries, from a group-distance job: one
-//              multi-label SSSP shared by a batch of queries (SCATTER hands each
-//              query its table C)
+//              multi-label SSSP shared by a batch of queries, whose distance
+//              rows the queries then read as their C (STAR gives each its
+//              star bound)
 //   DIST     : compact distance table C[v][i], inter-group distances vv,
 //              route table R (host loops of code/GPUGST/src/DPQ.cu:409-483
 //              moved on the device)
@@ -110,6 +111,7 @@ struct States {
   uint64_t* T;
   uint64_t* S1;
   const uint32_t* C;
+  uint32_t cs;
   int k, ND, W;
   // address of a state that is written (never a singleton of a C-fed slot)
   __device__ __forceinline__ uint64_t* at(uint32_t v, uint32_t p, uint32_t d) const {
@@ -120,7 +122,7 @@ struct States {
     if ((p & (p - 1u)) == 0u) {
       const uint32_t i = (uint32_t)(__ffs(p) - 1);
       if (S1) return S1[((uint64_t)v * ND + d) * (uint64_t)k + i];
-      return ((uint64_t)C[(uint64_t)v * k + i] << 32) | kPredPath;
+      return ((uint64_t)C[(uint64_t)v * cs + i] << 32) | kPredPath;
     }
     return T[((uint64_t)v * ND + d) * (uint64_t)W + p];
   }
@@ -131,6 +133,7 @@ __device__ __forceinline__ States states_of(const Slot* sp) {
   st.T = sp->T;
   st.S1 = sp->S1;
   st.C = sp->C;
+  st.cs = sp->cstride;
   st.k = sp->k;
   st.ND = sp->ND;
   st.W = sp->W;
@@ -444,7 +447,7 @@ __device__ void item_round(const GraphDev& g, Slot* sp, uint32_t chunk, FrontLDS
             if (level >= 2) {
               // bound first: a target that fails it is never read, so only the
               // segments of live vertices are ever touched
-              if (lbx == 0xFFFFFFFFu) lbx = lower_bound(C + (uint64_t)u * k, k, x, R);
+              if (lbx == 0xFFFFFFFFu) lbx = lower_bound(C + (uint64_t)u * sp->cstride, k, x, R);
               const uint32_t bst = __hip_atomic_load(&sp->best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
               if (cand + lbx > bst) continue;
             }
@@ -611,30 +614,27 @@ __device__ void item_jround(const GraphDev& g, Slot* sp, uint32_t item, FrontLDS
   }
 }
 
-// Job SCATTER: each member query's distance table C (its k labels of every
-// row) and its star bound min_v sum_i c_i(v).  One lane per member.
-__device__ void item_jscatter(Slot* slots, Slot* sp, uint32_t tile, uint64_t V) {
-  const uint64_t v0 = (uint64_t)tile * kScatterTV;
-  const uint64_t v1 = v0 + kScatterTV < V ? v0 + kScatterTV : V;
+// Job STAR: the star bound min_v sum_i c_i(v) of every member query, from
+// its k labels of each distance row (one lane per member, so the wave reads
+// each row once, coalesced).  The members read their distances from these
+// rows for the rest of their DP.
+__device__ void item_jstar(Slot* slots, Slot* sp, uint32_t tile, uint64_t V) {
+  const uint64_t v0 = (uint64_t)tile * kStarTV;
+  const uint64_t v1 = v0 + kStarTV < V ? v0 + kStarTV : V;
   const uint64_t Lp = (uint64_t)sp->Lp;
   for (int mi = lane_id(); mi < sp->nmem; mi += 64) {
     Slot* q = &slots[sp->jmem[mi]];
     const int k = q->k;
     const uint32_t* src = sp->dist + q->lab;
-    uint32_t* C = q->C;
     uint32_t star = kInf;
     for (uint64_t v = v0; v < v1; v++) {
       const uint32_t* row = src + v * Lp;
       uint32_t s = 0;
-      for (int i = 0; i < k; i++) {
-        const uint32_t c = row[i];
-        C[v * k + i] = c;
-        s = sat(s + c);
-      }
+      for (int i = 0; i < k; i++) s = sat(s + row[i]);
       star = s < star ? s : star;
     }
-    // star: shortest paths from one vertex to every group form a connected
-    // cover, so their sum bounds the optimum from above
+    // shortest paths from one vertex to every group form a connected cover,
+    // so their sum bounds the optimum from above
     if (star < kInf) atomicMin(&q->best, star);
   }
 }
@@ -657,7 +657,7 @@ __device__ void item_dist(const GraphDev& g, Slot* sp, uint32_t tile) {
         const uint32_t t = (uint32_t)(row[d * k + i] >> 32);
         c = t < c ? t : c;
       }
-      sp->C[v * k + i] = c;
+      sp->C[v * sp->cstride + i] = c;
       s = sat(s + c);
     }
     star = s < star ? s : star;
@@ -691,7 +691,7 @@ __device__ void item_vvr(const GraphDev& g, Slot* sp, WaveLDS* L) {
             c = t2 < c ? t2 : c;
           }
         } else {
-          c = sp->C[(uint64_t)v * k + j];
+          c = sp->C[(uint64_t)v * sp->cstride + j];
         }
         loc[j] = c < loc[j] ? c : loc[j];
       }
@@ -766,7 +766,7 @@ __device__ void item_live(Slot* sp, uint32_t tile, uint64_t V, WaveLDS* L) {
     bool lv = false;
     if (v < v1) {
       // lower bound on any tree through v that covers every group
-      const uint32_t lbf = lower_bound(sp->C + v * k, k, (uint32_t)W - 1, sp->R);
+      const uint32_t lbf = lower_bound(sp->C + v * sp->cstride, k, (uint32_t)W - 1, sp->R);
       lv = (lbf < kInf) && (lbf <= bst);
       if (lv) sp->lvf[v] = 1;
     }
@@ -829,7 +829,7 @@ __device__ void item_merge(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* 
       uint32_t csum = 0;
       if (!hop) {
         for (int i = 0; i < k; i++)
-          if ((x >> i) & 1u) csum = sat(csum + sp->C[(uint64_t)v * k + i]);
+          if ((x >> i) & 1u) csum = sat(csum + sp->C[(uint64_t)v * sp->cstride + i]);
       }
       for (int nd = 0; nd < ND; nd++) {
         uint64_t bestpk = row[nd * W + p];
@@ -860,7 +860,7 @@ __device__ void item_merge(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* 
         }
         const uint32_t c = (uint32_t)(bestpk >> 32);
         if (c >= kInf) continue;
-        if (lbx == 0xFFFFFFFFu) lbx = lower_bound(sp->C + (uint64_t)v * k, k, x, sp->R);
+        if (lbx == 0xFFFFFFFFu) lbx = lower_bound(sp->C + (uint64_t)v * sp->cstride, k, x, sp->R);
         if (c + lbx > bst0) continue;
         sp->T[((uint64_t)v * ND + nd) * W + p] = bestpk;
         wrote = true;
@@ -981,7 +981,7 @@ __device__ void item_final(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* 
 // found by a wave-wide scan of v's row (it exists because C is exact).
 __device__ void item_recon(const GraphDev& g, Slot* sp) {
   const int lane = lane_id();
-  const int W = sp->W, ND = sp->ND, D = sp->D, k = sp->k;
+  const int W = sp->W, ND = sp->ND, D = sp->D;
   const States S = states_of(sp);
   const bool hop = D >= 0;
   const uint32_t full = (uint32_t)W - 1;
@@ -1048,7 +1048,7 @@ __device__ void item_recon(const GraphDev& g, Slot* sp) {
         bool ok = false;
         if (e < e1) {
           const uint32_t u = (uint32_t)g.col[e];
-          ok = sp->C[(uint64_t)u * k + li] + (uint32_t)g.wt[e] == lc;
+          ok = sp->C[(uint64_t)u * sp->cstride + li] + (uint32_t)g.wt[e] == lc;
         }
         const uint64_t hb = __ballot(ok);
         if (hb) {
@@ -1160,9 +1160,9 @@ __device__ void complete_step(Slot* slots, Slot* s, Ctrl* ctrl) {
       case PH_ROUND:
         s->ctr[1] += (unsigned long long)(s->app[r % 3] >> 32);
         s->ctr[2] += (unsigned long long)(uint32_t)s->app[r % 3];
-        if ((s->app[(r + 1) % 3] >> 32) == 0) s->phase = PH_SCATTER;
+        if ((s->app[(r + 1) % 3] >> 32) == 0) s->phase = PH_STAR;
         break;
-      case PH_SCATTER:
+      case PH_STAR:
         for (int m = 0; m < s->nmem; m++) slots[s->jmem[m]].jdone = 1u;
         s->phase = PH_DONE;
         break;
@@ -1237,7 +1237,7 @@ __device__ uint32_t plan_step(Slot* s, uint64_t V) {
       work = ceil_div((uint32_t)s->app[r % 3], (uint32_t)s->chunk);
       if (!job) cls = kClsHeavy;  // the per-query grow carries the lower-bound code
       break;
-    case PH_SCATTER: work = ceil_div(V, kScatterTV); break;
+    case PH_STAR: work = ceil_div(V, kStarTV); break;
     case PH_DIST: work = (s->S1 ? ceil_div(V, kDistTV) : 0u) + 1u; cls = kClsHeavy; break;
     case PH_LIVE: work = ceil_div(V, kDistTV); cls = kClsHeavy; break;
     case PH_MERGE:
@@ -1394,7 +1394,7 @@ __device__ void step_body(const GraphDev& g, Slot* slots, Ctrl* ctrl, LDS* L) {
             case PH_FILL: item_fill(sp, local, V); break;
             case PH_SEED: item_jseed(g, sp, local); break;
             case PH_ROUND: item_jround(g, sp, local, (FrontLDS*)L); break;
-            case PH_SCATTER: item_jscatter(slots, sp, local, V); break;
+            case PH_STAR: item_jstar(slots, sp, local, V); break;
             default: break;
           }
         } else {

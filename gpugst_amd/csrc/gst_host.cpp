@@ -195,7 +195,7 @@ Layout layout_for(const gst_graph* g, int k, int D, bool own) {
   L.live = off; off += align256(V * 4);
   L.lvf = off;  off += align256(((V + 7) / 8) * 8);
   L.mask = off; off += align256(3 * V * 8);
-  L.C = off;    off += align256(((V * k + 1) / 2) * 8);
+  L.C = off;    off += own ? align256(((V * k + 1) / 2) * 8) : 0;  // else the job's rows
   L.list = off; off += align256(3 * V * 4);
   L.loff = off; off += align256(3 * V * 4);
   L.istart = off; off += align256(3 * (size_t)L.istride * 4);
@@ -261,7 +261,8 @@ void fill_infeasible(gst_result* r) {
 
 struct SlotHost {
   bool used = false;
-  bool job = false;  // a group-distance job (no result of its own)
+  bool job = false;      // a group-distance job (no result of its own)
+  bool charged = false;  // job: its work has been charged to its members
   size_t off = 0, len = 0;
   uint64_t tag = 0;
   double t_submit = 0, t_admit = 0;
@@ -371,10 +372,10 @@ class Engine {
     return -1;
   }
 
-  // A query slot; job >= 0 names the slot of its group-distance job and lab
-  // its first label there.  Returns the slot index, -1 when memory or slots
-  // are exhausted.
-  int admit_query(const QueryReq& q, int job, int lab) {
+  // A query slot; job >= 0 names the slot of its group-distance job, lab its
+  // first label there and jdist / jLp the job's distance rows (the query's
+  // C).  Returns the slot index, -1 when memory or slots are exhausted.
+  int admit_query(const QueryReq& q, int job, int lab, uint32_t* jdist = nullptr, int jLp = 0) {
     const int k = (int)q.groups.size();
     const bool own = job < 0;
     const Layout L = layout_for(g_, k, q.D, own);
@@ -410,7 +411,8 @@ class Engine {
     d.live = (uint32_t*)(base + L.live);
     d.lvf = (uint8_t*)(base + L.lvf);
     d.mask = (uint64_t*)(base + L.mask);
-    d.C = (uint32_t*)(base + L.C);
+    d.C = own ? (uint32_t*)(base + L.C) : jdist + lab;
+    d.cstride = own ? (uint32_t)k : (uint32_t)jLp;
     d.list = (uint32_t*)(base + L.list);
     d.loff = (uint32_t*)(base + L.loff);
     d.istart = (uint32_t*)(base + L.istart);
@@ -434,6 +436,7 @@ class Engine {
     if (!h.job) active_--;
     h.used = false;
     h.job = false;
+    h.charged = false;
     used_slots_--;
     memset(&h_slots_[s], 0, sizeof(gst::Slot));  // PH_FREE
     while (nslots_used_ > 0 && !sh_[nslots_used_ - 1].used) nslots_used_--;
@@ -455,8 +458,9 @@ class Engine {
     nslots_used_ = std::max(nslots_used_, js + 1);
     std::vector<int> mem;
     int lab = 0;
+    uint32_t* jdist = (uint32_t*)(arena_mem_ + off + J.dist);
     for (QueryReq* q : grp) {
-      const int s = admit_query(*q, js, lab);
+      const int s = admit_query(*q, js, lab, jdist, J.Lp);
       if (s < 0) {
         for (int m : mem) release_slot(m);
         release_slot(js);
@@ -466,6 +470,8 @@ class Engine {
       lab += (int)q->groups.size();
     }
     job_members_[js] = mem;
+    job_refs_[js] = (int)mem.size();
+    for (int m : mem) member_job_[m] = js;
     gst::Slot& d = h_slots_[js];
     memset(&d, 0, sizeof(d));
     d.kind = gst::KIND_JOB;
@@ -585,19 +591,19 @@ class Engine {
     for (int i = 0; i < nslots_used_; i++) {
       if (!sh_[i].used || h_slots_[i].phase != gst::PH_DONE) continue;
       if (sh_[i].job) {
-        // the job's level-1 work is charged to its members, per label
+        if (sh_[i].charged) continue;
+        // the job's level-1 work is charged to its members, per label; its
+        // distance rows stay until the last member has been harvested
         const gst::Slot& j = h_slots_[i];
-        std::vector<uint32_t> jm(j.nmem);
-        for (int m = 0; m < j.nmem; m++) {
-          const gst::Slot& q = h_slots_[job_members_at(i, m)];
-          SlotHost& qh = sh_[job_members_at(i, m)];
+        for (int m : job_members_.at(i)) {
+          const gst::Slot& q = h_slots_[m];
+          SlotHost& qh = sh_[m];
           for (int t = 1; t <= 4; t++) qh.extra[t] += (int64_t)(j.ctr[t] * (uint64_t)q.k / (uint64_t)std::max(1, j.L));
           qh.extra[0] += (int64_t)j.ctr[0];
         }
         jobs_done_++;
         for (int t = 0; t < 6; t++) job_ctr_[t] += (int64_t)j.ctr[t];
-        job_members_.erase(i);
-        release_slot(i);
+        sh_[i].charged = true;
       } else {
         finished->push_back(i);
       }
@@ -605,7 +611,18 @@ class Engine {
     return GST_OK;
   }
 
-  int job_members_at(int js, int m) const { return job_members_.at(js)[m]; }
+  // A harvested member releases its hold on its job; the last one frees it.
+  void drop_job_ref(int member) {
+    auto it = member_job_.find(member);
+    if (it == member_job_.end()) return;
+    const int js = it->second;
+    member_job_.erase(it);
+    if (--job_refs_[js] == 0) {
+      job_refs_.erase(js);
+      job_members_.erase(js);
+      release_slot(js);
+    }
+  }
 
   // Waits for the stream with a deadline, so a launch that never finishes is
   // reported instead of blocking the caller forever.
@@ -703,6 +720,7 @@ class Engine {
     *tag = h.tag;
     for (auto& x : h.extra) x = 0;
     release_slot(s);
+    drop_job_ref(s);
     return st;
   }
 
@@ -719,6 +737,8 @@ class Engine {
     nslots_used_ = 0;
     pending_uploads_.clear();
     job_members_.clear();
+    job_refs_.clear();
+    member_job_.clear();
   }
 
   uint32_t last_steps() const { return last_steps_; }
@@ -768,6 +788,8 @@ class Engine {
   // host copies of job tables, uploaded at the start of the next epoch
   std::vector<std::pair<void*, std::vector<char>>> pending_uploads_;
   std::map<int, std::vector<int>> job_members_;
+  std::map<int, int> job_refs_;    // job slot -> members not yet harvested
+  std::map<int, int> member_job_;  // member slot -> its job slot
 };
 
 namespace {
