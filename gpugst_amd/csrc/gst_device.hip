@@ -174,6 +174,34 @@ __device__ __forceinline__ uint32_t lower_bound(const uint32_t* __restrict__ Cro
   return sat(lb1);
 }
 
+// lower_bound(Crow, k, x, R) <= lim, decided with early exits: false as soon
+// as one group of x is farther than lim, true at the first pair (i, j) whose
+// closed walk c_i + R[x][i][j] + c_j is at most 2 lim (ceil(t / 2) <= lim
+// exactly when t <= 2 lim).  Used where only the comparison matters.
+__device__ __forceinline__ bool lb_le(const uint32_t* __restrict__ Crow, int k, uint32_t x,
+                                      const uint32_t* __restrict__ R, uint32_t lim) {
+  if (lim >= kInf) return true;  // lower_bound saturates at kInf
+  uint32_t c[kMaxK];
+#pragma unroll
+  for (int i = 0; i < kMaxK; i++) {
+    c[i] = (i < k) ? Crow[i] : kInf;
+    if (i < k && ((x >> i) & 1u) && c[i] > lim) return false;
+  }
+  if (__popc(x) < 2) return true;
+  const uint32_t* Rx = R + (size_t)x * (uint32_t)(k * k);
+  const uint32_t lim2 = 2u * lim;
+#pragma unroll
+  for (int i = 0; i < kMaxK; i++) {
+    if (i >= k || !((x >> i) & 1u)) continue;
+#pragma unroll
+    for (int j = 0; j < kMaxK; j++) {
+      if (j >= k || j == i || !((x >> j) & 1u)) continue;
+      if (c[i] + Rx[i * k + j] + c[j] <= lim2) return true;
+    }
+  }
+  return false;
+}
+
 // Wave-aggregated append of vertex u (when flag) to frontier buffer `buf`:
 // one 64-bit atomicAdd of (count << 32 | degree sum) per wave gives every
 // appended vertex both its list position and its arc offset.  Each ROUND item
@@ -766,8 +794,7 @@ __device__ void item_live(Slot* sp, uint32_t tile, uint64_t V, WaveLDS* L) {
     bool lv = false;
     if (v < v1) {
       // lower bound on any tree through v that covers every group
-      const uint32_t lbf = lower_bound(sp->C + v * sp->cstride, k, (uint32_t)W - 1, sp->R);
-      lv = (lbf < kInf) && (lbf <= bst);
+      lv = lb_le(sp->C + v * sp->cstride, k, (uint32_t)W - 1, sp->R, bst < kInf ? bst : kInf - 1);
       if (lv) sp->lvf[v] = 1;
     }
     const uint64_t ball = __ballot(lv);
