@@ -78,6 +78,22 @@ def heavy_bytes(res_list, V, k, ND=1):
     return total
 
 
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` on this workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the copy calibration),
+    or None."""
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "step_kernels_rmat1.1M_k5_batch256.md")
+    try:
+        for line in open(path):
+            if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
+                return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -176,11 +192,14 @@ def main():
         dom = "gst_lean_kernel" if lean_ms >= heavy_ms else "gst_heavy_kernel"
         ms, byts = (lean_ms, lb) if dom == "gst_lean_kernel" else (heavy_ms, hb)
         achieved = byts / n_steps / (ms * 1e-3) / 1e9 if (byts and ms > 0) else 0.0
+        traffic = profiled_traffic(dom) if (args.D < 0 and args.batch == 256 and args.k == 5) else None
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
                 "algorithmic_bytes_per_launch": int(byts / n_steps) if byts else None,
                 "avg_launch_ms": round(ms, 4), "launches": int(n_steps),
                 "other_kernel_avg_ms": round(heavy_ms if dom == "gst_lean_kernel" else lean_ms, 4),
+                "traffic_source": "profiles/step_kernels_rmat1.1M_k5_batch256.md (rocprofv3 FETCH_SIZE x2.0 "
+                                  "+ WRITE_SIZE x1.0 per launch, same workload)" if traffic else None,
                 "pass": f"{len(rr)} further queries of the same workload, events around every step kernel"}
         log(f"roofline pass: {n_steps} steps, lean {lean_ms:.3f} ms/launch, heavy {heavy_ms:.3f} ms/launch, "
             f"{dom} {achieved:.1f} GB/s algorithmic")
