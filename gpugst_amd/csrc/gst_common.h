@@ -112,6 +112,9 @@ struct alignas(64) Slot {
   uint32_t* dist;       // job: V * Lp distances
   int32_t* lgrp;        // job: group id of each label
   uint32_t* jmem;       // job: member slot indices
+  uint32_t* vsrc;       // vertices LIVE examines (null: all V); job: the vertices it reached
+  uint32_t nvsrc;       // entries of vsrc
+  uint32_t pad3;
   uint32_t* live;       // V: vertices that can lie on a tree cheaper than the bound
   uint8_t* lvf;         // V: live flag per vertex
   uint64_t* mask;       // 3 * V (triple-buffered dirty sub-masks)

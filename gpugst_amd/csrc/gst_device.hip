@@ -643,28 +643,52 @@ __device__ void item_jround(const GraphDev& g, Slot* sp, uint32_t item, FrontLDS
 }
 
 // Job STAR: the star bound min_v sum_i c_i(v) of every member query, from
-// its k labels of each distance row (one lane per member, so the wave reads
-// each row once, coalesced).  The members read their distances from these
-// rows for the rest of their DP.
+// its k labels of each distance row (one lane per member, at most two members
+// per lane since a job has at most 128; the wave reads each row once,
+// coalesced), and the list of vertices where some member has every group at
+// a finite distance.  A vertex off that list cannot be live for any member,
+// so the members' LIVE passes examine only the list.
 __device__ void item_jstar(Slot* slots, Slot* sp, uint32_t tile, uint64_t V) {
+  const int lane = lane_id();
   const uint64_t v0 = (uint64_t)tile * kStarTV;
   const uint64_t v1 = v0 + kStarTV < V ? v0 + kStarTV : V;
   const uint64_t Lp = (uint64_t)sp->Lp;
-  for (int mi = lane_id(); mi < sp->nmem; mi += 64) {
-    Slot* q = &slots[sp->jmem[mi]];
-    const int k = q->k;
-    const uint32_t* src = sp->dist + q->lab;
-    uint32_t star = kInf;
-    for (uint64_t v = v0; v < v1; v++) {
-      const uint32_t* row = src + v * Lp;
+  const int nmem = sp->nmem;
+  Slot* q0 = lane < nmem ? &slots[sp->jmem[lane]] : nullptr;
+  Slot* q1 = lane + 64 < nmem ? &slots[sp->jmem[lane + 64]] : nullptr;
+  const int k0 = q0 ? q0->k : 0, k1 = q1 ? q1->k : 0;
+  const uint32_t lab0 = q0 ? (uint32_t)q0->lab : 0u, lab1 = q1 ? (uint32_t)q1->lab : 0u;
+  uint32_t star0 = kInf, star1 = kInf;
+  uint64_t reached = 0;
+  for (uint64_t v = v0; v < v1; v++) {
+    const uint32_t* row = sp->dist + v * Lp;
+    bool fin = false;
+    if (q0) {
       uint32_t s = 0;
-      for (int i = 0; i < k; i++) s = sat(s + row[i]);
-      star = s < star ? s : star;
+      for (int i = 0; i < k0; i++) s = sat(s + row[lab0 + i]);
+      star0 = s < star0 ? s : star0;
+      fin = s < kInf;
     }
-    // shortest paths from one vertex to every group form a connected cover,
-    // so their sum bounds the optimum from above
-    if (star < kInf) atomicMin(&q->best, star);
+    if (q1) {
+      uint32_t s = 0;
+      for (int i = 0; i < k1; i++) s = sat(s + row[lab1 + i]);
+      star1 = s < star1 ? s : star1;
+      fin = fin || s < kInf;
+    }
+    if (__ballot(fin)) reached |= 1ull << (v - v0);
   }
+  // shortest paths from one vertex to every group form a connected cover, so
+  // their sum bounds the optimum from above
+  if (q0 && star0 < kInf) atomicMin(&q0->best, star0);
+  if (q1 && star1 < kInf) atomicMin(&q1->best, star1);
+  const bool r = (reached >> lane) & 1ull;
+  const uint64_t ball = __ballot(r);
+  if (ball == 0) return;
+  const int leader = __ffsll((long long)ball) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(&sp->nvsrc, (uint32_t)__popcll(ball));
+  base = __shfl(base, leader, 64);
+  if (r) sp->vsrc[base + __popcll(ball & __lanemask_lt())] = (uint32_t)(v0 + (uint64_t)lane);
 }
 
 // ------------------------------------------------------ per-query DIST ----
@@ -787,30 +811,33 @@ __device__ void item_live(Slot* sp, uint32_t tile, uint64_t V, WaveLDS* L) {
   const int k = sp->k, W = sp->W, ND = sp->ND;
   const int rowlen = W * ND;
   const uint32_t bst = __hip_atomic_load(&sp->best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const uint64_t n = sp->vsrc ? (uint64_t)sp->nvsrc : V;  // candidate vertices
   const uint64_t v0 = (uint64_t)tile * kDistTV;
-  const uint64_t v1 = v0 + kDistTV < V ? v0 + kDistTV : V;
+  const uint64_t v1 = v0 + kDistTV < n ? v0 + kDistTV : n;
   for (uint64_t vb = v0; vb < v1; vb += 64) {
-    const uint64_t v = vb + lane;
+    const uint64_t idx = vb + lane;
+    uint64_t v = 0;
     bool lv = false;
-    if (v < v1) {
+    if (idx < v1) {
+      v = sp->vsrc ? (uint64_t)sp->vsrc[idx] : idx;
       // lower bound on any tree through v that covers every group
       lv = lb_le(sp->C + v * sp->cstride, k, (uint32_t)W - 1, sp->R, bst < kInf ? bst : kInf - 1);
       if (lv) sp->lvf[v] = 1;
     }
     const uint64_t ball = __ballot(lv);
     if (ball == 0) continue;
-    const int n = __popcll(ball);
+    const int nl = __popcll(ball);
     const uint32_t rank = __popcll(ball & __lanemask_lt());
     uint32_t base = 0;
     const int leader = __ffsll((long long)ball) - 1;
-    if (lane == leader) base = atomicAdd(&sp->nlive, (uint32_t)n);
+    if (lane == leader) base = atomicAdd(&sp->nlive, (uint32_t)nl);
     base = __shfl(base, leader, 64);
     if (lv) {
       sp->live[base + rank] = (uint32_t)v;
       L->f.wv[rank] = (uint32_t)v;
     }
     wave_sync();
-    for (int t = lane; t < n * rowlen; t += 64) {
+    for (int t = lane; t < nl * rowlen; t += 64) {
       const int jj = t / rowlen;
       sp->T[(uint64_t)L->f.wv[jj] * rowlen + (t - jj * rowlen)] = kInfState;
     }
@@ -1190,7 +1217,12 @@ __device__ void complete_step(Slot* slots, Slot* s, Ctrl* ctrl) {
         if ((s->app[(r + 1) % 3] >> 32) == 0) s->phase = PH_STAR;
         break;
       case PH_STAR:
-        for (int m = 0; m < s->nmem; m++) slots[s->jmem[m]].jdone = 1u;
+        for (int m = 0; m < s->nmem; m++) {
+          Slot* q = &slots[s->jmem[m]];
+          q->vsrc = s->vsrc;
+          q->nvsrc = s->nvsrc;
+          q->jdone = 1u;
+        }
         s->phase = PH_DONE;
         break;
       default:
@@ -1266,7 +1298,7 @@ __device__ uint32_t plan_step(Slot* s, uint64_t V) {
       break;
     case PH_STAR: work = ceil_div(V, kStarTV); break;
     case PH_DIST: work = (s->S1 ? ceil_div(V, kDistTV) : 0u) + 1u; cls = kClsHeavy; break;
-    case PH_LIVE: work = ceil_div(V, kDistTV); cls = kClsHeavy; break;
+    case PH_LIVE: work = ceil_div(s->vsrc ? (uint64_t)s->nvsrc : V, kDistTV); cls = kClsHeavy; break;
     case PH_MERGE:
     case PH_FINAL: work = ceil_div(s->nlive, kMergeTV); cls = kClsHeavy; break;
     case PH_RECON: work = 1; cls = kClsHeavy; break;
