@@ -108,6 +108,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--max-slots", type=int, default=0)
+    ap.add_argument("--serve", type=int, default=512, help="queries of the serving leg (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=0.0,
+                    help="offered queries/s of the serving leg (default: half the batched throughput)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -206,6 +209,41 @@ def main():
         log("job counters: " + " ".join(f"{f}={g.get_option('batch_job_' + f)}" for f in
                                         ("entries", "arcs", "relaxations", "improvements", "atomics")))
 
+    # ---------------- serving: open-loop Poisson arrivals into the server
+    serving = None
+    if args.serve > 0 and args.D < 0:
+        import threading
+        from gpugst_amd import Server
+        rate = args.serve_rate if args.serve_rate > 0 else 0.5 * batch_qps
+        n = args.serve
+        sq = data.make_queries(n, args.k, inst.meta["eligible"], 3000 + 4 + 10 * args.k + 7)
+        gaps = np.random.default_rng(11).exponential(1.0 / rate, n)
+        srv = Server(g)
+        tickets = [None] * n
+
+        def submitter():
+            t_next = time.perf_counter()
+            for i in range(n):
+                t_next += gaps[i]
+                dt = t_next - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+                tickets[i] = srv.submit(sq[i], D=-1)
+
+        ts = time.perf_counter()
+        th = threading.Thread(target=submitter)
+        th.start()
+        th.join()
+        sres = [srv.wait(t) for t in tickets]
+        span = time.perf_counter() - ts
+        srv.close()
+        slat = [r.latency_ms for r in sres]
+        serving = {"offered_qps": round(rate, 1), "queries": n, "achieved_qps": round(n / span, 1),
+                   "p50_ms": round(pct(slat, 50), 2), "p99_ms": round(pct(slat, 99), 2),
+                   "arrivals": "Poisson (seeded), open loop, gst_server_* continuous batching"}
+        log(f"serving: offered {rate:.0f} q/s, {n} queries in {span:.2f}s -> {n/span:.0f} q/s, "
+            f"p50 {serving['p50_ms']} ms p99 {serving['p99_ms']} ms")
+
     # ---------------- parity: plain vs batched on shared queries, trees valid
     all_b = bres + timed
     mism = 0
@@ -273,6 +311,7 @@ def main():
         "parity": {"plain_vs_batched_mismatches": mism, "invalid_trees": bad_trees,
                    "oracle_checked": oracle_checked, "oracle_mismatches": oracle_mism,
                    "infeasible": infeas},
+        "serving": serving,
         "roofline": roof,
         "cpu_baseline": cpu,
     }
