@@ -55,6 +55,7 @@ constexpr uint32_t kStarTV = 64;        // vertices per job STAR item
 constexpr int kClsLean = 0;   // FILL, SEED, job ROUND, STAR, CLEAR
 constexpr int kClsHeavy = 1;  // query ROUND, DIST, LIVE, MERGE, FINAL, RECON
 constexpr int kNumCls = 2;
+constexpr int kTraceSteps = 4096;  // steps of one epoch whose phase can be logged
 
 enum Phase : int32_t {
   PH_FREE = 0,
@@ -151,8 +152,10 @@ struct alignas(64) Ctrl {
   uint32_t exit_after;
   uint32_t max_steps;
   uint32_t nslots;
+  uint32_t trace_slot;  // slot whose planned phase is logged per step (~0u: none)
   uint32_t active[kNumCls][kMaxSlots];
   uint32_t prefix[kNumCls][kMaxSlots + 1];
+  uint32_t trace[kTraceSteps];  // phase | level << 8 | planned << 16, by step of the epoch
 };
 
 }  // namespace gst

@@ -140,6 +140,13 @@ __device__ __forceinline__ States states_of(const Slot* sp) {
   return st;
 }
 
+// Row of vertex v in the distance table C.  A hop query's table holds D + 1
+// cumulative rows per vertex, row h the group distances within h hops; other
+// queries have one row (the job's row for a job-fed query).
+__device__ __forceinline__ const uint32_t* crow(const Slot* sp, uint64_t v, int h) {
+  return sp->C + v * sp->cstride + (sp->D >= 0 ? (uint64_t)h * (uint64_t)sp->k : 0);
+}
+
 // Lower bound on the cost of the cheapest tree containing u that covers the
 // groups in x (x non-empty): max( max_{i in x} c_i(u),
 //   ceil( min_{i != j in x} (c_i(u) + R[x][i][j] + c_j(u)) / 2 ) ).
@@ -306,33 +313,31 @@ __device__ __forceinline__ bool mark_dirty(uint64_t* mnxt, uint32_t u, uint64_t 
 // ---------------------------------------------------------------- items ----
 
 // u64 words that FILL initialises: job [distances][masks]; query [S1][masks]
-// [C][live flags] (S1 and C only when the query computes its own level 1)
+// [live flags] (S1 only when the query computes its own level 1; its C table
+// needs no fill, DIST writes every entry)
 __device__ __forceinline__ void fill_regions(const Slot* sp, uint64_t V, uint64_t* nA, uint64_t* nM,
-                                             uint64_t* nC, uint64_t* nF) {
+                                             uint64_t* nF) {
   if (sp->kind == KIND_JOB) {
     *nA = V * (uint64_t)sp->Lp / 2;
     *nM = 3 * V;
-    *nC = 0;
     *nF = 0;
     return;
   }
   const bool own = sp->S1 != nullptr;
   *nA = own ? V * (uint64_t)sp->ND * (uint64_t)sp->k : 0;
   *nM = 3 * V;
-  *nC = own ? ((uint64_t)V * sp->k + 1) / 2 : 0;
   *nF = (V + 7) / 8;
 }
 
 __device__ void item_fill(Slot* sp, uint32_t idx, uint64_t V) {
-  uint64_t nA, nM, nC, nF;
-  fill_regions(sp, V, &nA, &nM, &nC, &nF);
-  const uint64_t total = nA + nM + nC + nF;
+  uint64_t nA, nM, nF;
+  fill_regions(sp, V, &nA, &nM, &nF);
+  const uint64_t total = nA + nM + nF;
   const uint64_t base = (uint64_t)idx * kFillWords;
   const uint64_t end = base + kFillWords < total ? base + kFillWords : total;
   const bool job = sp->kind == KIND_JOB;
   uint64_t* A = job ? (uint64_t*)sp->dist : sp->S1;
   uint64_t* M = sp->mask;
-  uint64_t* C64 = (uint64_t*)sp->C;
   uint64_t* F64 = (uint64_t*)sp->lvf;
   const uint64_t cinf = ((uint64_t)kInf << 32) | kInf;
   const uint64_t aval = job ? cinf : kInfState;
@@ -341,10 +346,8 @@ __device__ void item_fill(Slot* sp, uint32_t idx, uint64_t V) {
       A[t] = aval;
     else if (t < nA + nM)
       M[t - nA] = 0;
-    else if (t < nA + nM + nC)
-      C64[t - nA - nM] = cinf;
     else
-      F64[t - nA - nM - nC] = 0;
+      F64[t - nA - nM] = 0;
   }
 }
 
@@ -474,10 +477,17 @@ __device__ void item_round(const GraphDev& g, Slot* sp, uint32_t chunk, FrontLDS
             const int td = hop ? d + 1 : 0;
             if (level >= 2) {
               // bound first: a target that fails it is never read, so only the
-              // segments of live vertices are ever touched
-              if (lbx == 0xFFFFFFFFu) lbx = lower_bound(C + (uint64_t)u * sp->cstride, k, x, R);
+              // segments of live vertices are ever touched.  A hop state of
+              // height td reaches the rest of its tree within D - td hops.
+              uint32_t lb;
+              if (hop) {
+                lb = lower_bound(crow(sp, u, D - td), k, x, R);
+              } else {
+                if (lbx == 0xFFFFFFFFu) lbx = lower_bound(C + (uint64_t)u * sp->cstride, k, x, R);
+                lb = lbx;
+              }
               const uint32_t bst = __hip_atomic_load(&sp->best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-              if (cand + lbx > bst) continue;
+              if (cand + lb > bst) continue;
             }
             unsigned long long* tp = (unsigned long long*)S.at(u, p, td);
             const uint32_t tc = (uint32_t)(*tp >> 32);
@@ -699,25 +709,26 @@ __device__ void item_dist(const GraphDev& g, Slot* sp, uint32_t tile) {
   const int k = sp->k, ND = sp->ND;
   const bool hop = sp->D >= 0;
   const uint64_t v0 = (uint64_t)tile * kDistTV;
+  // arms of a hop star pairwise within D hops: floor(D/2) hops each
+  const int hs = hop ? sp->D / 2 : 0;
   uint32_t star = kInf;
   for (uint64_t v = v0 + lane_id(); v < v0 + kDistTV && v < V; v += 64) {
     uint32_t s = 0;
     const uint64_t* row = sp->S1 + v * (uint64_t)ND * k;
+    uint32_t* crw = sp->C + v * sp->cstride;
     for (int i = 0; i < k; i++) {
       uint32_t c = kInf;
       for (int d = 0; d < ND; d++) {
         const uint32_t t = (uint32_t)(row[d * k + i] >> 32);
         c = t < c ? t : c;
+        crw[d * k + i] = c;  // within d hops
+        if (d == hs) s = sat(s + c);
       }
-      sp->C[v * sp->cstride + i] = c;
-      s = sat(s + c);
     }
     star = s < star ? s : star;
   }
-  if (!hop) {
-    star = wave_min_u32(star);
-    if (lane_id() == 0 && star < kInf) atomicMin(&sp->best, star);
-  }
+  star = wave_min_u32(star);
+  if (lane_id() == 0 && star < kInf) atomicMin(&sp->best, star);
 }
 
 __device__ void item_vvr(const GraphDev& g, Slot* sp, WaveLDS* L) {
@@ -821,7 +832,7 @@ __device__ void item_live(Slot* sp, uint32_t tile, uint64_t V, WaveLDS* L) {
     if (idx < v1) {
       v = sp->vsrc ? (uint64_t)sp->vsrc[idx] : idx;
       // lower bound on any tree through v that covers every group
-      lv = lb_le(sp->C + v * sp->cstride, k, (uint32_t)W - 1, sp->R, bst < kInf ? bst : kInf - 1);
+      lv = lb_le(crow(sp, v, sp->D), k, (uint32_t)W - 1, sp->R, bst < kInf ? bst : kInf - 1);
       if (lv) sp->lvf[v] = 1;
     }
     const uint64_t ball = __ballot(lv);
@@ -885,6 +896,7 @@ __device__ void item_merge(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* 
         for (int i = 0; i < k; i++)
           if ((x >> i) & 1u) csum = sat(csum + sp->C[(uint64_t)v * sp->cstride + i]);
       }
+      const int nx = __popc(x);
       for (int nd = 0; nd < ND; nd++) {
         uint64_t bestpk = row[nd * W + p];
         for (uint32_t sub = (p - 1) & p; sub; sub = (sub - 1) & p) {
@@ -914,12 +926,27 @@ __device__ void item_merge(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* 
         }
         const uint32_t c = (uint32_t)(bestpk >> 32);
         if (c >= kInf) continue;
-        if (lbx == 0xFFFFFFFFu) lbx = lower_bound(sp->C + (uint64_t)v * sp->cstride, k, x, sp->R);
-        if (c + lbx > bst0) continue;
+        if (hop) {
+          // a state of height nd reaches the rest of its tree within D - nd hops
+          if (c + lower_bound(crow(sp, v, D - nd), k, x, sp->R) > bst0) continue;
+        } else {
+          if (lbx == 0xFFFFFFFFu) lbx = lower_bound(sp->C + (uint64_t)v * sp->cstride, k, x, sp->R);
+          if (c + lbx > bst0) continue;
+        }
         sp->T[((uint64_t)v * ND + nd) * W + p] = bestpk;
         wrote = true;
-        // upper bounds from complete walk-trees at v
-        if (!hop) {
+        // upper bounds from complete walk-trees at v: this state plus one
+        // shortest path per missing group (hop: paths of at most h hops, with
+        // nd + h <= D and, for two or more paths, 2h <= D)
+        if (hop) {
+          const int h = (nx >= 2 && D / 2 < D - nd) ? D / 2 : D - nd;
+          const uint32_t* cr = crow(sp, v, h);
+          uint32_t hsum = 0;
+          for (int i = 0; i < k; i++)
+            if ((x >> i) & 1u) hsum = sat(hsum + cr[i]);
+          const uint32_t ub = sat(c + hsum);
+          ubmin = ub < ubmin ? ub : ubmin;
+        } else {
           const uint32_t ub = sat(c + csum);
           ubmin = ub < ubmin ? ub : ubmin;
         }
@@ -1286,9 +1313,9 @@ __device__ uint32_t plan_step(Slot* s, uint64_t V) {
   const bool job = s->kind == KIND_JOB;
   switch (s->phase) {
     case PH_FILL: {
-      uint64_t nA, nM, nC, nF;
-      fill_regions(s, V, &nA, &nM, &nC, &nF);
-      work = ceil_div(nA + nM + nC + nF, kFillWords);
+      uint64_t nA, nM, nF;
+      fill_regions(s, V, &nA, &nM, &nF);
+      work = ceil_div(nA + nM + nF, kFillWords);
       break;
     }
     case PH_SEED: work = job ? (uint32_t)s->L : (uint32_t)s->k; break;
@@ -1350,6 +1377,9 @@ __device__ void phase_b(Slot* slots, Ctrl* ctrl, uint64_t V, uint32_t (*sh_items
     }
     sh_items[kClsLean][i] = lean;
     sh_items[kClsHeavy][i] = heavy;
+    if ((uint32_t)i == ctrl->trace_slot && !leave)
+      ctrl->trace[(ctrl->steps - 1u) & (uint32_t)(kTraceSteps - 1)] =
+          (uint32_t)s->phase | ((uint32_t)s->level << 8) | ((lean + heavy) ? 0x10000u : 0u);
   }
   __syncthreads();
   // per class: exclusive scan over slots with work, compacting them into the

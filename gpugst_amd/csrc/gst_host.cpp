@@ -121,6 +121,7 @@ struct Options {
   int64_t graph = 0;  // replay chunks of engine steps as a captured hipGraph
   int64_t group_pass = 1;  // unbounded queries take level 1 from shared group-distance jobs
   int64_t time_steps = 0;  // time each step's lean and heavy kernels with events
+  int64_t trace_slot = 0;  // with time_steps: slot whose phases the step times are binned by
 };
 
 }  // namespace
@@ -195,7 +196,7 @@ Layout layout_for(const gst_graph* g, int k, int D, bool own) {
   L.live = off; off += align256(V * 4);
   L.lvf = off;  off += align256(((V + 7) / 8) * 8);
   L.mask = off; off += align256(3 * V * 8);
-  L.C = off;    off += own ? align256(((V * k + 1) / 2) * 8) : 0;  // else the job's rows
+  L.C = off;    off += own ? align256(V * ND * k * 4) : 0;  // rows within 0..D hops; else the job's rows
   L.list = off; off += align256(3 * V * 4);
   L.loff = off; off += align256(3 * V * 4);
   L.istart = off; off += align256(3 * (size_t)L.istride * 4);
@@ -413,7 +414,7 @@ class Engine {
     d.lvf = (uint8_t*)(base + L.lvf);
     d.mask = (uint64_t*)(base + L.mask);
     d.C = own ? (uint32_t*)(base + L.C) : jdist + lab;
-    d.cstride = own ? (uint32_t)k : (uint32_t)jLp;
+    d.cstride = own ? (uint32_t)(k * d.ND) : (uint32_t)jLp;
     d.list = (uint32_t*)(base + L.list);
     d.loff = (uint32_t*)(base + L.loff);
     d.istart = (uint32_t*)(base + L.istart);
@@ -525,7 +526,9 @@ class Engine {
     c.exit_after = std::max(1u, exit_after);
     c.max_steps = max_steps;
     c.nslots = (uint32_t)(use_graph ? max_slots_ : nslots_used_);
+    c.trace_slot = timed ? (uint32_t)g_->opt.trace_slot : ~0u;
     const size_t hdr = offsetof(gst::Ctrl, active);
+    pair_ns_.clear();
     HIP_TRY(hipMemcpyAsync(d_ctrl_, h_ctrl_, hdr, hipMemcpyHostToDevice, stream_));
     const int grid_l = grid_size(gst::kClsLean), grid_h = grid_size(gst::kClsHeavy);
     const gst::GraphDev gd = g_->dev();
@@ -567,6 +570,7 @@ class Engine {
           cls_ns_[0] += (int64_t)((double)a * 1e6);
           cls_ns_[1] += (int64_t)((double)b * 1e6);
           cls_launches_ += 1;
+          pair_ns_.push_back((int64_t)((double)(a + b) * 1e6));
         }
       }
       if (h_ctrl_->exit_flag) break;
@@ -577,6 +581,22 @@ class Engine {
     HIP_TRY(hipMemcpyAsync(h_slots_, d_slots_, sizeof(gst::Slot) * nslots_used_, hipMemcpyDeviceToHost, stream_));
     gst_status st = wait_stream();
     if (st != GST_OK) return st;
+    if (timed && c.trace_slot != ~0u) {
+      // step j of the epoch ran the plan that the trace logged at index j
+      const size_t n = std::min<size_t>({pair_ns_.size(), (size_t)h_ctrl_->steps, (size_t)gst::kTraceSteps});
+      std::vector<uint32_t> tr(n);
+      if (n) {
+        HIP_TRY(hipMemcpyAsync(tr.data(), d_ctrl_->trace, n * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
+        st = wait_stream();
+        if (st != GST_OK) return st;
+      }
+      for (size_t j = 0; j < n; j++) {
+        if (!(tr[j] & 0x10000u)) continue;
+        auto& e = phase_ns_[tr[j] & 0xFFFFu];
+        e.first += pair_ns_[j];
+        e.second += 1;
+      }
+    }
     float ms = 0;
     HIP_TRY(hipEventElapsedTime(&ms, ev0_, ev1_));
     kernel_ns_ += (int64_t)((double)ms * 1e6);
@@ -751,7 +771,15 @@ class Engine {
   int64_t cls_launches() const { return cls_launches_; }
   int64_t jobs_done() const { return jobs_done_; }
   int64_t job_ctr(int t) const { return job_ctr_[t]; }
+  // summed step time (lean + heavy) and step count of the traced slot's
+  // steps in `phase` at `level` (options time_steps and trace_slot)
+  int64_t phase_stat(int phase, int level, bool count) const {
+    auto it = phase_ns_.find((uint32_t)phase | ((uint32_t)level << 8));
+    if (it == phase_ns_.end()) return 0;
+    return count ? it->second.second : it->second.first;
+  }
   void reset_stats() {
+    phase_ns_.clear();
     kernel_ns_ = launches_ = steps_total_ = 0;
     cls_ns_[0] = cls_ns_[1] = cls_launches_ = jobs_done_ = 0;
     for (auto& x : job_ctr_) x = 0;
@@ -787,6 +815,8 @@ class Engine {
   int64_t kernel_ns_ = 0, launches_ = 0, steps_total_ = 0;
   int64_t cls_ns_[2] = {0, 0}, cls_launches_ = 0, jobs_done_ = 0;
   int64_t job_ctr_[6] = {0, 0, 0, 0, 0, 0};
+  std::vector<int64_t> pair_ns_;  // per step of the current epoch (time_steps)
+  std::map<uint32_t, std::pair<int64_t, int64_t>> phase_ns_;
   // host copies of job tables, uploaded at the start of the next epoch
   std::vector<std::pair<void*, std::vector<char>>> pending_uploads_;
   std::map<int, std::vector<int>> job_members_;
@@ -1025,6 +1055,7 @@ static int64_t* option_ref(gst_graph* g, const std::string& k) {
   if (k == "graph") return &g->opt.graph;
   if (k == "group_pass") return &g->opt.group_pass;
   if (k == "time_steps") return &g->opt.time_steps;
+  if (k == "trace_slot") return &g->opt.trace_slot;
   return nullptr;
 }
 
@@ -1043,7 +1074,7 @@ gst_status gst_set_option(gst_graph* g, const char* key, int64_t value) {
     return fail(GST_INVALID_ARG, "max_slots outside [1,%d]", gst::kMaxSlots);
   if (k == "chunk" && (value < 64 || value > (1 << 20))) return fail(GST_INVALID_ARG, "chunk outside [64, 2^20]");
   if (k == "edge_cap" && (value < 64 || value > (1 << 26))) return fail(GST_INVALID_ARG, "edge_cap outside [64, 2^26]");
-  if ((k == "budget_mb" || k == "blocks_per_cu" || k == "timeout_ms") && value < 0)
+  if ((k == "budget_mb" || k == "blocks_per_cu" || k == "timeout_ms" || k == "trace_slot") && value < 0)
     return fail(GST_INVALID_ARG, "negative value");
   std::lock_guard<std::mutex> lk(g->dev_mu);
   *r = value;
@@ -1070,6 +1101,16 @@ gst_status gst_get_option(gst_graph* g, const char* key, int64_t* value) {
     if (f == "job_relaxations") { *value = e ? e->job_ctr(3) : 0; return GST_OK; }
     if (f == "job_improvements") { *value = e ? e->job_ctr(4) : 0; return GST_OK; }
     if (f == "job_atomics") { *value = e ? e->job_ctr(5) : 0; return GST_OK; }
+    // phase_ns:<phase>:<level>, phase_steps:<phase>:<level>
+    int ph = 0, lv = 0;
+    if (sscanf(f.c_str(), "phase_ns:%d:%d", &ph, &lv) == 2) {
+      *value = e ? e->phase_stat(ph, lv, false) : 0;
+      return GST_OK;
+    }
+    if (sscanf(f.c_str(), "phase_steps:%d:%d", &ph, &lv) == 2) {
+      *value = e ? e->phase_stat(ph, lv, true) : 0;
+      return GST_OK;
+    }
   }
   auto grid_of = [&](int c) {
     const int64_t b = g->opt.blocks_per_cu > 0 ? std::min<int64_t>(g->opt.blocks_per_cu, g->occ[c]) : g->occ[c];

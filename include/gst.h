@@ -135,11 +135,15 @@ void gst_result_free(gst_result *r);
 /* Tunables: "max_slots", "budget_mb", "blocks_per_cu", "chunk",
  * "edge_cap", "timeout_ms", "graph" (replay engine steps as a hipGraph),
  * "group_pass" (1: unbounded queries share group-distance jobs; 0: each
- * query runs its own level 1), "time_steps" (time every step's kernels).
+ * query runs its own level 1), "time_steps" (time every step's kernels),
+ * "trace_slot" (with time_steps: the engine slot whose phase each step's
+ * time is binned by; slot 0 is the query of a one-query call).
  * Read-only: "num_cu", "occupancy", "occupancy_heavy", "grid",
  * "grid_heavy", "{plain,batch}_{kernel_ns,launches,steps,lean_ns,heavy_ns,
  * timed_steps,jobs,job_entries,job_arcs,job_relaxations,job_improvements,
- * job_atomics}".
+ * job_atomics}", "{plain,batch}_phase_ns:<phase>:<level>" and
+ * "{plain,batch}_phase_steps:<phase>:<level>" (step time and step count of
+ * the traced slot per engine phase).
  * "reset_stats" (set) clears the statistics.  Unknown keys return
  * GST_INVALID_ARG. */
 gst_status gst_set_option(gst_graph *g, const char *key, int64_t value);

@@ -7,6 +7,8 @@ at tests/cpu_sim/_build/libgst_cpu.so.  Prints one JSON line of results.
            validated
   jobs   : one batch of random k=5 queries (duplicates allowed) over several
            group-distance jobs, against the oracle and against group_pass=0
+  hop    : one batch of random k=3..5 queries with hop bounds 2..4 (depth-aware
+           bounds and hop star / path upper bounds), against the oracle
 """
 import json
 import os
@@ -58,19 +60,25 @@ def check_tiny(stride):
     return {"checked": n, "bad": bad}
 
 
-def check_jobs(V, nq, seed):
+def random_instance(V, seed, deg=3, gmax=4):
     rng = random.Random(seed)
     edges = set()
     for v in range(1, V):
         edges.add((rng.randrange(v), v))
-    while len(edges) < 3 * V:
+    while len(edges) < deg * V:
         a, b = rng.randrange(V), rng.randrange(V)
         if a != b:
             edges.add((min(a, b), max(a, b)))
     E = [(a, b, rng.randint(1, 20)) for a, b in sorted(edges)]
     rp, col, wt = data.csr_from_edges(V, E)
     G = 40
-    groups = [sorted(rng.sample(range(V), rng.randint(1, 4))) for _ in range(G)]
+    groups = [sorted(rng.sample(range(V), rng.randint(1, gmax))) for _ in range(G)]
+    return rng, rp, col, wt, groups
+
+
+def check_jobs(V, nq, seed):
+    rng, rp, col, wt, groups = random_instance(V, seed)
+    G = len(groups)
     qs = [[rng.randrange(G) for _ in range(5)] for _ in range(nq)]
     g = Graph(rp, col, wt)
     g.set_groups(*groups_csr(groups))
@@ -89,10 +97,33 @@ def check_jobs(V, nq, seed):
     return {"checked": nq, "bad": bad, "labels": 5 * nq}
 
 
+def check_hop(V, nq, seed):
+    rng, rp, col, wt, groups = random_instance(V, seed, deg=4, gmax=8)
+    G = len(groups)
+    qs = [rng.sample(range(G), rng.randint(3, 5)) for _ in range(nq)]
+    Ds = [rng.randint(2, 4) for _ in range(nq)]
+    g = Graph(rp, col, wt)
+    g.set_groups(*groups_csr(groups))
+    rs = g.solve_batch(qs, Ds)
+    bad, feas = [], 0
+    for i, q in enumerate(qs):
+        gl = [groups[x] for x in q]
+        exp = O.dpbf(rp, col, wt, gl, D=Ds[i])[0]
+        ok = rs[i].cost == exp
+        if ok and rs[i].feasible:
+            feas += 1
+            ok = validate_tree(rp, col, wt, gl, rs[i].cost, rs[i].root, rs[i].edges, D=Ds[i])["ok"]
+        if not ok:
+            bad.append([i, Ds[i], rs[i].cost, exp])
+    g.close()
+    return {"checked": nq, "feasible": feas, "bad": bad}
+
+
 def main():
     stride = int(sys.argv[1]) if len(sys.argv) > 1 else 3
     nq = int(sys.argv[2]) if len(sys.argv) > 2 else 60
-    out = {"tiny": check_tiny(stride), "jobs": check_jobs(150, nq, 7)}
+    nh = int(sys.argv[3]) if len(sys.argv) > 3 else 24
+    out = {"tiny": check_tiny(stride), "jobs": check_jobs(150, nq, 7), "hop": check_hop(150, nh, 11)}
     print(json.dumps(out), flush=True)
 
 

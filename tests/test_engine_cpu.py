@@ -3,8 +3,9 @@
 No GPU is needed: tests/cpu_sim compiles the HIP device code and the host
 engine for the host (wave lanes as threads, host memory as device memory).
 This pins the engine's logic -- group-distance jobs (multi-label level 1,
-SCATTER, shortest-path reconstruction from C), the hop-bounded per-query level
-1, batching and admission over several jobs -- against the golden fixtures and
+STAR, shortest-path reconstruction from C), the hop-bounded per-query level 1
+and its pruning (bounds per state height, hop star and path upper bounds),
+batching and admission over several jobs -- against the golden fixtures and
 the oracle.  The GPU parity tests check the same properties on the MI355X.
 """
 import json
@@ -26,8 +27,9 @@ def cpu_results():
     except (OSError, subprocess.CalledProcessError) as e:  # no g++
         pytest.skip(f"CPU build unavailable: {e}")
     env = dict(os.environ, GST_LIB=lib)
-    # every 6th tiny case (all D), and 54 k=5 queries = 270 labels = two jobs
-    r = subprocess.run([sys.executable, os.path.join(ROOT, "tests", "cpu_sim", "run_checks.py"), "6", "54"],
+    # every 6th tiny case (all D), 54 k=5 queries = 270 labels = two jobs, and
+    # 24 hop-bounded queries
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "tests", "cpu_sim", "run_checks.py"), "6", "54", "24"],
                        env=env, capture_output=True, text=True, timeout=900)
     assert r.returncode == 0, r.stderr[-2000:]
     return json.loads(r.stdout.strip().splitlines()[-1])
@@ -43,3 +45,9 @@ def test_group_distance_jobs_on_cpu_build(cpu_results):
     j = cpu_results["jobs"]
     assert j["labels"] > 256  # more than one job
     assert j["bad"] == [], j["bad"]
+
+
+def test_hop_queries_on_cpu_build(cpu_results):
+    h = cpu_results["hop"]
+    assert h["feasible"] >= h["checked"] // 2
+    assert h["bad"] == [], h["bad"]
