@@ -856,9 +856,11 @@ __device__ void item_live(Slot* sp, uint32_t tile, uint64_t V, WaveLDS* L) {
   }
 }
 
-// Level-L merge: every |p| = L subset of every live vertex from the final
-// lower levels.  One wave lane per subset p (W <= 64), 64 / W vertices per
-// pass, their W * ND segments staged in LDS.
+// Level-L merge: T(v,p,nd) for every |p| = L subset p and height nd of every
+// live vertex, from the final lower levels.  The W * ND segments of up to VPW
+// live vertices are staged in LDS; the wave's lanes then take the (height,
+// vertex, subset) tuples of the staged vertices in turn, height-major so that
+// neighbouring lanes run the same height-pair loops.
 __device__ void item_merge(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* L) {
   const int lane = lane_id();
   const uint64_t V = (uint64_t)g.V;
@@ -870,9 +872,13 @@ __device__ void item_merge(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* 
   const int nxt = (sp->step + 1) % 3;
   uint64_t* mnxt = sp->mask + (uint64_t)nxt * V;
   const int rowlen = W * ND;
-  const int VPW = 64 / W;
-  const int j = lane / W;
-  const uint32_t p = (uint32_t)(lane % W);
+  const int VPW = (kStageWords / rowlen) < 64 ? (kStageWords / rowlen) : 64;
+  // this level's subsets, listed in LDS
+  uint32_t* subs = L->f.pad;
+  const bool mine = lane < W && __popc((uint32_t)lane) == level;
+  const uint64_t sball = __ballot(mine);
+  if (mine) subs[__popcll(sball & __lanemask_lt())] = (uint32_t)lane;
+  const int nL = __popcll(sball);
   const uint64_t nl = sp->nlive;
   const uint64_t v0 = (uint64_t)tile * kMergeTV;
   const uint64_t v1 = v0 + kMergeTV < nl ? v0 + kMergeTV : nl;
@@ -882,88 +888,78 @@ __device__ void item_merge(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* 
   for (uint64_t vb = v0; vb < v1; vb += VPW) {
     const int nv = (int)((v1 - vb) < (uint64_t)VPW ? (v1 - vb) : (uint64_t)VPW);
     stage_rows(sp, sp->live + vb, nv, rowlen, W, L->stage);
+    if (lane < nv) {
+      L->f.wv[lane] = sp->live[vb + lane];
+      L->f.wm[lane] = 0;
+    }
     wave_sync();
-    const uint32_t v = (j < nv) ? sp->live[vb + j] : 0u;
-    const bool act = (j < nv) && p != 0 && __popc(p) == level;
-    bool wrote = false;
-    if (act) {
-      const uint64_t* row = L->stage + j * rowlen;
+    const int per_h = nv * nL;  // tuples of one height
+    for (int t = lane; t < per_h * ND; t += 64) {
+      const int nd = t / per_h;
+      const int r = t - nd * per_h;
+      const int jj = r / nL;
+      const uint32_t p = subs[r - jj * nL];
+      const uint32_t v = L->f.wv[jj];
+      const uint64_t* row = L->stage + jj * rowlen;
       const uint32_t x = full ^ p;
       const uint32_t low = p & (0u - p);
-      uint32_t lbx = 0xFFFFFFFFu;
-      uint32_t csum = 0;
-      if (!hop) {
-        for (int i = 0; i < k; i++)
-          if ((x >> i) & 1u) csum = sat(csum + sp->C[(uint64_t)v * sp->cstride + i]);
-      }
-      const int nx = __popc(x);
-      for (int nd = 0; nd < ND; nd++) {
-        uint64_t bestpk = row[nd * W + p];
-        for (uint32_t sub = (p - 1) & p; sub; sub = (sub - 1) & p) {
-          if (!(sub & low)) continue;
-          const uint32_t sub2 = p ^ sub;
-          // pairs (d1, d2) with max(d1, d2) == nd and d1 + d2 <= Dlim
-          const int lim = Dlim - nd;
-          if (lim < 0) continue;
-          const int dm = nd < lim ? nd : lim;
-          for (int d2 = 0; d2 <= dm; d2++) {  // d1 = nd
-            const uint64_t a = row[nd * W + sub], b = row[d2 * W + sub2];
-            const uint32_t c = sat((uint32_t)(a >> 32) + (uint32_t)(b >> 32));
-            n_eval++;
-            if (c >= kInf) continue;
-            const uint64_t pk = ((uint64_t)c << 32) | (kPredMerge | sub | ((uint32_t)nd << 16) | ((uint32_t)d2 << 23));
-            bestpk = pk < bestpk ? pk : bestpk;
-          }
-          const int dm1 = (nd - 1) < lim ? (nd - 1) : lim;
-          for (int d1 = 0; d1 <= dm1; d1++) {  // d2 = nd, d1 < nd
-            const uint64_t a = row[d1 * W + sub], b = row[nd * W + sub2];
-            const uint32_t c = sat((uint32_t)(a >> 32) + (uint32_t)(b >> 32));
-            n_eval++;
-            if (c >= kInf) continue;
-            const uint64_t pk = ((uint64_t)c << 32) | (kPredMerge | sub | ((uint32_t)d1 << 16) | ((uint32_t)nd << 23));
-            bestpk = pk < bestpk ? pk : bestpk;
-          }
+      uint64_t bestpk = row[nd * W + p];
+      for (uint32_t sub = (p - 1) & p; sub; sub = (sub - 1) & p) {
+        if (!(sub & low)) continue;
+        const uint32_t sub2 = p ^ sub;
+        // pairs (d1, d2) with max(d1, d2) == nd and d1 + d2 <= Dlim
+        const int lim = Dlim - nd;
+        if (lim < 0) continue;
+        const int dm = nd < lim ? nd : lim;
+        for (int d2 = 0; d2 <= dm; d2++) {  // d1 = nd
+          const uint64_t a = row[nd * W + sub], b = row[d2 * W + sub2];
+          const uint32_t c = sat((uint32_t)(a >> 32) + (uint32_t)(b >> 32));
+          n_eval++;
+          if (c >= kInf) continue;
+          const uint64_t pk = ((uint64_t)c << 32) | (kPredMerge | sub | ((uint32_t)nd << 16) | ((uint32_t)d2 << 23));
+          bestpk = pk < bestpk ? pk : bestpk;
         }
-        const uint32_t c = (uint32_t)(bestpk >> 32);
-        if (c >= kInf) continue;
-        if (hop) {
-          // a state of height nd reaches the rest of its tree within D - nd hops
-          if (c + lower_bound(crow(sp, v, D - nd), k, x, sp->R) > bst0) continue;
-        } else {
-          if (lbx == 0xFFFFFFFFu) lbx = lower_bound(sp->C + (uint64_t)v * sp->cstride, k, x, sp->R);
-          if (c + lbx > bst0) continue;
-        }
-        sp->T[((uint64_t)v * ND + nd) * W + p] = bestpk;
-        wrote = true;
-        // upper bounds from complete walk-trees at v: this state plus one
-        // shortest path per missing group (hop: paths of at most h hops, with
-        // nd + h <= D and, for two or more paths, 2h <= D)
-        if (hop) {
-          const int h = (nx >= 2 && D / 2 < D - nd) ? D / 2 : D - nd;
-          const uint32_t* cr = crow(sp, v, h);
-          uint32_t hsum = 0;
-          for (int i = 0; i < k; i++)
-            if ((x >> i) & 1u) hsum = sat(hsum + cr[i]);
-          const uint32_t ub = sat(c + hsum);
-          ubmin = ub < ubmin ? ub : ubmin;
-        } else {
-          const uint32_t ub = sat(c + csum);
-          ubmin = ub < ubmin ? ub : ubmin;
-        }
-        if (__popc(x) < level) {
-          const int d2m = Dlim - nd;
-          for (int d2 = 0; d2 <= d2m; d2++) {
-            const uint32_t ub = sat(c + (uint32_t)(row[d2 * W + x] >> 32));
-            ubmin = ub < ubmin ? ub : ubmin;
-          }
+        const int dm1 = (nd - 1) < lim ? (nd - 1) : lim;
+        for (int d1 = 0; d1 <= dm1; d1++) {  // d2 = nd, d1 < nd
+          const uint64_t a = row[d1 * W + sub], b = row[nd * W + sub2];
+          const uint32_t c = sat((uint32_t)(a >> 32) + (uint32_t)(b >> 32));
+          n_eval++;
+          if (c >= kInf) continue;
+          const uint64_t pk = ((uint64_t)c << 32) | (kPredMerge | sub | ((uint32_t)d1 << 16) | ((uint32_t)nd << 23));
+          bestpk = pk < bestpk ? pk : bestpk;
         }
       }
+      const uint32_t c = (uint32_t)(bestpk >> 32);
+      if (c >= kInf) continue;
+      // a state of height nd reaches the rest of its tree within D - nd hops
+      if (c + lower_bound(crow(sp, v, hop ? D - nd : 0), k, x, sp->R) > bst0) continue;
+      sp->T[((uint64_t)v * ND + nd) * W + p] = bestpk;
+      atomicOr((unsigned long long*)&L->f.wm[jj], 1ull << p);
+      // upper bounds from complete walk-trees at v: this state plus one
+      // shortest path per missing group (hop: paths of at most h hops, with
+      // nd + h <= D and, for two or more paths, 2h <= D) ...
+      const int h = hop ? ((__popc(x) >= 2 && D / 2 < D - nd) ? D / 2 : D - nd) : 0;
+      const uint32_t* cr = crow(sp, v, h);
+      uint32_t psum = 0;
+      for (int i = 0; i < k; i++)
+        if ((x >> i) & 1u) psum = sat(psum + cr[i]);
+      uint32_t ub = sat(c + psum);
+      // ... or the complement state at v, when its level is final
+      if (__popc(x) < level) {
+        const int d2m = Dlim - nd;
+        for (int d2 = 0; d2 <= d2m; d2++) {
+          const uint32_t u2 = sat(c + (uint32_t)(row[d2 * W + x] >> 32));
+          ub = u2 < ub ? u2 : ub;
+        }
+      }
+      ubmin = ub < ubmin ? ub : ubmin;
     }
-    const uint64_t ball = __ballot(wrote);
+    wave_sync();
     bool app = false;
-    if (j < nv && p == 0) {
-      const uint64_t segmask = (W == 64) ? ~0ull : ((1ull << W) - 1);
-      const uint64_t m = (ball >> (j * W)) & segmask;
+    uint32_t v = 0;
+    if (lane < nv) {
+      v = L->f.wv[lane];
+      const uint64_t m = L->f.wm[lane];
       if (m) {
         mnxt[v] = m;
         app = true;
@@ -1521,7 +1517,11 @@ __global__ __launch_bounds__(kBlock) void gst_lean_kernel(GraphDev g, Slot* slot
   step_body<kClsLean>(g, slots, ctrl, &lds[threadIdx.x >> 6]);
 }
 
-__global__ __launch_bounds__(kBlock) void gst_heavy_kernel(GraphDev g, Slot* slots, Ctrl* ctrl) {
+// Three waves per SIMD is what its LDS allows (3 blocks of 4 waves per CU);
+// the register budget is held to match (one loop-invariant 16-byte value is
+// kept in scratch).
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void gst_heavy_kernel(GraphDev g, Slot* slots,
+                                                                                               Ctrl* ctrl) {
   __shared__ WaveLDS lds[kWaves];
   step_body<kClsHeavy>(g, slots, ctrl, &lds[threadIdx.x >> 6]);
 }

@@ -122,6 +122,7 @@ struct Options {
   int64_t group_pass = 1;  // unbounded queries take level 1 from shared group-distance jobs
   int64_t time_steps = 0;  // time each step's lean and heavy kernels with events
   int64_t trace_slot = 0;  // with time_steps: slot whose phases the step times are binned by
+  int64_t job_min_queries = 2;  // fewer unbounded queries admitted together run their own level 1
 };
 
 }  // namespace
@@ -354,6 +355,14 @@ class Engine {
         if (used_slots_ + (int)grp.size() + 2 > max_slots_) break;
         grp.push_back(*it);
         L += k;
+      }
+      if ((int64_t)grp.size() < g_->opt.job_min_queries) {
+        // a job pays off from about two queries on: alone, the query's own
+        // level 1 is faster
+        if (used_slots_ >= max_slots_ || admit_query(*q0, -1, 0) < 0) break;
+        pend.pop_front();
+        n++;
+        continue;
       }
       const size_t want = grp.size();
       while (!grp.empty() && !admit_job_group(grp, L)) {
@@ -867,7 +876,7 @@ gst_status engine_for(gst_graph* g, bool batch, size_t need, Engine** out) {
 // Device bytes one query needs while admitted, with the share of its
 // group-distance job when it has one (a job of its own when alone).
 size_t single_query_bytes(const gst_graph* g, int k, int D) {
-  if (!uses_job(g, k, D)) return layout_for(g, k, D, true).total;
+  if (!uses_job(g, k, D) || g->opt.job_min_queries > 1) return layout_for(g, k, D, true).total;
   return layout_for(g, k, D, false).total + job_layout(g, k, 1).total;
 }
 
@@ -1056,6 +1065,7 @@ static int64_t* option_ref(gst_graph* g, const std::string& k) {
   if (k == "group_pass") return &g->opt.group_pass;
   if (k == "time_steps") return &g->opt.time_steps;
   if (k == "trace_slot") return &g->opt.trace_slot;
+  if (k == "job_min_queries") return &g->opt.job_min_queries;
   return nullptr;
 }
 
@@ -1076,6 +1086,8 @@ gst_status gst_set_option(gst_graph* g, const char* key, int64_t value) {
   if (k == "edge_cap" && (value < 64 || value > (1 << 26))) return fail(GST_INVALID_ARG, "edge_cap outside [64, 2^26]");
   if ((k == "budget_mb" || k == "blocks_per_cu" || k == "timeout_ms" || k == "trace_slot") && value < 0)
     return fail(GST_INVALID_ARG, "negative value");
+  if (k == "job_min_queries" && (value < 1 || value > gst::kJobLabels))
+    return fail(GST_INVALID_ARG, "job_min_queries outside [1,%d]", gst::kJobLabels);
   std::lock_guard<std::mutex> lk(g->dev_mu);
   *r = value;
   if (k == "max_slots" || k == "budget_mb") g->batch.reset();  // rebuilt on next use

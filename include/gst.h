@@ -135,7 +135,9 @@ void gst_result_free(gst_result *r);
 /* Tunables: "max_slots", "budget_mb", "blocks_per_cu", "chunk",
  * "edge_cap", "timeout_ms", "graph" (replay engine steps as a hipGraph),
  * "group_pass" (1: unbounded queries share group-distance jobs; 0: each
- * query runs its own level 1), "time_steps" (time every step's kernels),
+ * query runs its own level 1), "job_min_queries" (default 2: fewer
+ * unbounded queries admitted together run their own level 1 instead of a
+ * job), "time_steps" (time every step's kernels),
  * "trace_slot" (with time_steps: the engine slot whose phase each step's
  * time is binned by; slot 0 is the query of a one-query call).
  * Read-only: "num_cu", "occupancy", "occupancy_heavy", "grid",
