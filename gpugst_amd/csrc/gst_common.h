@@ -45,9 +45,10 @@ constexpr int kStageWords = 1152;       // per-wave LDS staging (u64): 64*ND row
 // Labels are (query, group) pairs; the distance row of a vertex holds all of
 // them (u32, padded to whole 4-label chunks), so an arc relaxation reads and
 // writes coalesced 16-byte chunks shared by every query of the job.
-constexpr int kJobLabels = 256;         // labels per job: 64 chunks, one dirty bit each
+constexpr int kJobChunkLabels = 4;     // labels per chunk of a job row (one dirty bit)
+constexpr int kJobLabels = 64 * kJobChunkLabels;  // labels per job: 64 chunks
 constexpr uint32_t kJobArcs = 256;      // arcs per job ROUND item
-constexpr int kJobChunksPerLane = 4;    // 4-label chunks one lane relaxes per arc
+constexpr int kJobChunksPerLane = 4;    // chunks one lane relaxes per arc
 constexpr uint32_t kStarTV = 64;        // vertices per job STAR item
 
 // work classes: each has its own kernel, so the arc-parallel items do not run
@@ -120,7 +121,8 @@ struct alignas(64) Slot {
   uint8_t* lvf;         // V: live flag per vertex
   uint64_t* mask;       // 3 * V (triple-buffered dirty sub-masks)
   uint32_t* C;          // group->vertex distances c_i(v) = C[v * cstride + i]: the query's own
-                        // V * k table, or its k columns of its job's distance rows
+                        // V * ND * k table (row h: within h hops), or its k columns of its
+                        // job's distance rows
   uint32_t* list;       // 3 * V frontier vertices
   uint32_t* loff;       // 3 * V arc offsets of frontier entries
   uint32_t* R;          // W * k * k route table

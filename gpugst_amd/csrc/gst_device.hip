@@ -104,17 +104,32 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t x) {
 
 __device__ __forceinline__ uint32_t sat(uint32_t a) { return a < kInf ? a : kInf; }
 
+// Device tables are reached through pointers held in the slot table, which
+// the compiler cannot place in an address space.  Cast to the global one,
+// their loads and atomics are global_* instructions instead of flat ones
+// (whose waits also cover LDS traffic).
+#ifdef GST_CPU_SIM
+#define GST_AS_GLOBAL
+#else
+#define GST_AS_GLOBAL __attribute__((address_space(1)))
+#endif
+typedef uint32_t jchunk __attribute__((vector_size(4 * kJobChunkLabels)));
+typedef GST_AS_GLOBAL const jchunk g_cchunk;
+typedef GST_AS_GLOBAL uint32_t g_u32;
+typedef GST_AS_GLOBAL const uint32_t g_cu32;
+typedef GST_AS_GLOBAL unsigned long long g_u64;
+
 // State (v, p, d): singletons in the compact S1 table or, when the slot's
 // distances come from a group-distance job, synthesised from C with the
 // kPredPath predecessor; every other subset in the vertex-major T segments.
 struct States {
-  uint64_t* T;
-  uint64_t* S1;
-  const uint32_t* C;
+  g_u64* T;
+  g_u64* S1;
+  g_cu32* C;
   uint32_t cs;
   int k, ND, W;
   // address of a state that is written (never a singleton of a C-fed slot)
-  __device__ __forceinline__ uint64_t* at(uint32_t v, uint32_t p, uint32_t d) const {
+  __device__ __forceinline__ g_u64* at(uint32_t v, uint32_t p, uint32_t d) const {
     if ((p & (p - 1u)) == 0u) return S1 + ((uint64_t)v * ND + d) * (uint64_t)k + (uint32_t)(__ffs(p) - 1);
     return T + ((uint64_t)v * ND + d) * (uint64_t)W + p;
   }
@@ -130,9 +145,9 @@ struct States {
 
 __device__ __forceinline__ States states_of(const Slot* sp) {
   States st;
-  st.T = sp->T;
-  st.S1 = sp->S1;
-  st.C = sp->C;
+  st.T = (g_u64*)sp->T;
+  st.S1 = (g_u64*)sp->S1;
+  st.C = (g_cu32*)sp->C;
   st.cs = sp->cstride;
   st.k = sp->k;
   st.ND = sp->ND;
@@ -143,8 +158,8 @@ __device__ __forceinline__ States states_of(const Slot* sp) {
 // Row of vertex v in the distance table C.  A hop query's table holds D + 1
 // cumulative rows per vertex, row h the group distances within h hops; other
 // queries have one row (the job's row for a job-fed query).
-__device__ __forceinline__ const uint32_t* crow(const Slot* sp, uint64_t v, int h) {
-  return sp->C + v * sp->cstride + (sp->D >= 0 ? (uint64_t)h * (uint64_t)sp->k : 0);
+__device__ __forceinline__ g_cu32* crow(const Slot* sp, uint64_t v, int h) {
+  return (g_cu32*)sp->C + v * sp->cstride + (sp->D >= 0 ? (uint64_t)h * (uint64_t)sp->k : 0);
 }
 
 // Lower bound on the cost of the cheapest tree containing u that covers the
@@ -152,8 +167,8 @@ __device__ __forceinline__ const uint32_t* crow(const Slot* sp, uint64_t v, int 
 //   ceil( min_{i != j in x} (c_i(u) + R[x][i][j] + c_j(u)) / 2 ) ).
 // The second term halves a closed walk from u through every group of x; R is
 // the cheapest group-to-group route visiting all of x from i to j.
-__device__ __forceinline__ uint32_t lower_bound(const uint32_t* __restrict__ Crow, int k,
-                                                uint32_t x, const uint32_t* __restrict__ R) {
+__device__ __forceinline__ uint32_t lower_bound(g_cu32* __restrict__ Crow, int k, uint32_t x,
+                                                g_cu32* __restrict__ R) {
   uint32_t c[kMaxK];
   uint32_t lb1 = 0;
 #pragma unroll
@@ -163,7 +178,7 @@ __device__ __forceinline__ uint32_t lower_bound(const uint32_t* __restrict__ Cro
   }
   if (lb1 >= kInf) return kInf;
   if (__popc(x) >= 2) {
-    const uint32_t* Rx = R + (size_t)x * (uint32_t)(k * k);
+    g_cu32* Rx = R + (size_t)x * (uint32_t)(k * k);
     uint32_t tour = 3u * kInf;
 #pragma unroll
     for (int i = 0; i < kMaxK; i++) {
@@ -185,8 +200,8 @@ __device__ __forceinline__ uint32_t lower_bound(const uint32_t* __restrict__ Cro
 // as one group of x is farther than lim, true at the first pair (i, j) whose
 // closed walk c_i + R[x][i][j] + c_j is at most 2 lim (ceil(t / 2) <= lim
 // exactly when t <= 2 lim).  Used where only the comparison matters.
-__device__ __forceinline__ bool lb_le(const uint32_t* __restrict__ Crow, int k, uint32_t x,
-                                      const uint32_t* __restrict__ R, uint32_t lim) {
+__device__ __forceinline__ bool lb_le(g_cu32* __restrict__ Crow, int k, uint32_t x, g_cu32* __restrict__ R,
+                                      uint32_t lim) {
   if (lim >= kInf) return true;  // lower_bound saturates at kInf
   uint32_t c[kMaxK];
 #pragma unroll
@@ -195,7 +210,7 @@ __device__ __forceinline__ bool lb_le(const uint32_t* __restrict__ Crow, int k, 
     if (i < k && ((x >> i) & 1u) && c[i] > lim) return false;
   }
   if (__popc(x) < 2) return true;
-  const uint32_t* Rx = R + (size_t)x * (uint32_t)(k * k);
+  g_cu32* Rx = R + (size_t)x * (uint32_t)(k * k);
   const uint32_t lim2 = 2u * lim;
 #pragma unroll
   for (int i = 0; i < kMaxK; i++) {
@@ -437,8 +452,8 @@ __device__ void item_round(const GraphDev& g, Slot* sp, uint32_t chunk, FrontLDS
   const uint64_t* mcur = sp->mask + (uint64_t)cur * V;
   uint64_t* mnxt = sp->mask + (uint64_t)nxt * V;
   const States S = states_of(sp);
-  const uint32_t* C = sp->C;
-  const uint32_t* R = sp->R;
+  g_cu32* C = (g_cu32*)sp->C;
+  g_cu32* R = (g_cu32*)sp->R;
 
   uint64_t n_relax = 0, n_impr = 0;
   uint32_t apn = 0;
@@ -489,12 +504,12 @@ __device__ void item_round(const GraphDev& g, Slot* sp, uint32_t chunk, FrontLDS
               const uint32_t bst = __hip_atomic_load(&sp->best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
               if (cand + lb > bst) continue;
             }
-            unsigned long long* tp = (unsigned long long*)S.at(u, p, td);
+            g_u64* tp = S.at(u, p, td);
             const uint32_t tc = (uint32_t)(*tp >> 32);
             if (cand >= tc) continue;
             if (revarc == 0xFFFFFFFFu) revarc = (uint32_t)g.rev[e];
             const unsigned long long nv = ((unsigned long long)cand << 32) | (kPredGrow | revarc);
-            const unsigned long long old = atomicMin(tp, nv);
+            const unsigned long long old = __hip_atomic_fetch_min(tp, nv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if ((uint32_t)(old >> 32) > cand) {
               improved |= 1ull << p;
               n_impr++;
@@ -548,12 +563,13 @@ __device__ void item_jseed(const GraphDev& g, Slot* sp, uint32_t l) {
     if (t < end) {
       v = (uint32_t)g.gmem[t];
       sp->dist[(uint64_t)v * Lp + l] = 0u;
-      const unsigned long long old = atomicOr((unsigned long long*)&mn[v], 1ull << (l >> 2));
+      const unsigned long long old = atomicOr((unsigned long long*)&mn[v], 1ull << (l / kJobChunkLabels));
       app = (old == 0);
     }
     wave_append(app, v, sp, nxt, g, V);
   }
 }
+
 
 // One label of a job relaxation: target word tw held t when read, the
 // candidate is s + w.  The atomicMin returns nothing, so the wave never waits
@@ -562,7 +578,7 @@ __device__ void item_jseed(const GraphDev& g, Slot* sp, uint32_t l) {
 // another arc lowered further in the meantime: it is relaxed once more in the
 // next round, with its current value.  Every mark follows a strict decrease
 // of an integer, so the rounds still end.
-__device__ __forceinline__ bool jrelax(uint32_t s, uint32_t t, uint32_t w, uint32_t* tw, uint32_t& n_at) {
+__device__ __forceinline__ bool jrelax(uint32_t s, uint32_t t, uint32_t w, g_u32* tw, uint32_t& n_at) {
   if (s >= kInf) return false;
   const uint32_t cand = s + w;
   if (cand >= t || cand >= kInf) return false;
@@ -573,8 +589,8 @@ __device__ __forceinline__ bool jrelax(uint32_t s, uint32_t t, uint32_t w, uint3
 
 // Job ROUND: one chunk of arcs of the job's frontier.  G lanes share an arc
 // (64 / G arcs per wave iteration); lane r of the group relaxes the dirty
-// 4-label chunks r, r + G, ... of the source row into the target row (16-byte
-// loads, coalesced over the group).
+// kJobChunkLabels-label chunks r, r + G, ... of the source row into the target
+// row (one vector load per chunk, coalesced over the group).
 __device__ void item_jround(const GraphDev& g, Slot* sp, uint32_t item, FrontLDS* L) {
   const int lane = lane_id();
   const uint64_t V = (uint64_t)g.V;
@@ -592,7 +608,7 @@ __device__ void item_jround(const GraphDev& g, Slot* sp, uint32_t item, FrontLDS
   uint64_t* mnxt = sp->mask + (uint64_t)nxt * V;
   uint32_t* dist = sp->dist;
   const uint64_t Lp = (uint64_t)sp->Lp;
-  const int nch = sp->Lp >> 2;
+  const int nch = sp->Lp / kJobChunkLabels;
   const int G = sp->G;
   const int APW = 64 / G;
   const int sub = lane / G, rr = lane - sub * G;
@@ -615,18 +631,18 @@ __device__ void item_jround(const GraphDev& g, Slot* sp, uint32_t item, FrontLDS
         const uint32_t e = L->wrs[o] + (pos - L->woff[o]);
         u = (uint32_t)g.col[e];
         const uint32_t w = (uint32_t)g.wt[e];
-        const uint4* srow = (const uint4*)(dist + (uint64_t)vs * Lp);
-        const uint4* trow = (const uint4*)(dist + (uint64_t)u * Lp);
-        uint32_t* tw = dist + (uint64_t)u * Lp;
+        g_cchunk* srow = (g_cchunk*)(dist + (uint64_t)vs * Lp);
+        g_cchunk* trow = (g_cchunk*)(dist + (uint64_t)u * Lp);
+        g_u32* tw = (g_u32*)(dist + (uint64_t)u * Lp);
         for (int c = rr; c < nch; c += G) {
           if (!((m >> c) & 1ull)) continue;
-          const uint4 sv = srow[c];
-          const uint4 tv = trow[c];
-          bool im = jrelax(sv.x, tv.x, w, tw + 4 * c, n_at);
-          im |= jrelax(sv.y, tv.y, w, tw + 4 * c + 1, n_at);
-          im |= jrelax(sv.z, tv.z, w, tw + 4 * c + 2, n_at);
-          im |= jrelax(sv.w, tv.w, w, tw + 4 * c + 3, n_at);
-          n_relax += 4;
+          const jchunk sv = srow[c];
+          const jchunk tv = trow[c];
+          bool im = false;
+#pragma unroll
+          for (int q = 0; q < kJobChunkLabels; q++)
+            im |= jrelax(sv[q], tv[q], w, tw + kJobChunkLabels * c + q, n_at);
+          n_relax += kJobChunkLabels;
           if (im) {
             imp |= 1ull << c;
             n_impr++;
@@ -704,31 +720,51 @@ __device__ void item_jstar(Slot* slots, Slot* sp, uint32_t tile, uint64_t V) {
 // ------------------------------------------------------ per-query DIST ----
 
 // Distance table, inter-group distances and route table after level 1.
+// The query's own C rows are as long as its S1 rows (cstride == ND * k), so a
+// tile of vertices is one contiguous run of both: lanes walk it element by
+// element (vertex, height, group), coalesced, each taking the minimum over the
+// heights up to its own.
 __device__ void item_dist(const GraphDev& g, Slot* sp, uint32_t tile) {
+  const int lane = lane_id();
   const uint64_t V = (uint64_t)g.V;
   const int k = sp->k, ND = sp->ND;
   const bool hop = sp->D >= 0;
   const uint64_t v0 = (uint64_t)tile * kDistTV;
-  // arms of a hop star pairwise within D hops: floor(D/2) hops each
+  const uint64_t v1 = v0 + kDistTV < V ? v0 + kDistTV : V;
+  const uint32_t rl = (uint32_t)(ND * k);
+  const uint64_t* S1t = sp->S1 + v0 * rl;
+  uint32_t* Ct = sp->C + v0 * rl;
+  const uint32_t nE = (uint32_t)(v1 - v0) * rl;
+  for (uint32_t e = (uint32_t)lane; e < nE; e += 64) {
+    const uint32_t vr = e / rl, rem = e - vr * rl;
+    const uint32_t d = rem / (uint32_t)k, i = rem - d * (uint32_t)k;
+    const uint64_t* col = S1t + vr * rl + i;
+    uint32_t c = kInf;
+    for (uint32_t dd = 0; dd <= d; dd++) {
+      const uint32_t t = (uint32_t)(col[dd * (uint32_t)k] >> 32);
+      c = t < c ? t : c;
+    }
+    Ct[e] = c;  // within d hops
+  }
+  // star upper bound; the arms of a hop star are pairwise within D hops when
+  // each has at most floor(D/2)
   const int hs = hop ? sp->D / 2 : 0;
   uint32_t star = kInf;
-  for (uint64_t v = v0 + lane_id(); v < v0 + kDistTV && v < V; v += 64) {
+  for (uint64_t v = v0 + (uint64_t)lane; v < v1; v += 64) {
+    const uint64_t* row = sp->S1 + v * rl;
     uint32_t s = 0;
-    const uint64_t* row = sp->S1 + v * (uint64_t)ND * k;
-    uint32_t* crw = sp->C + v * sp->cstride;
     for (int i = 0; i < k; i++) {
       uint32_t c = kInf;
-      for (int d = 0; d < ND; d++) {
+      for (int d = 0; d <= hs; d++) {
         const uint32_t t = (uint32_t)(row[d * k + i] >> 32);
         c = t < c ? t : c;
-        crw[d * k + i] = c;  // within d hops
-        if (d == hs) s = sat(s + c);
       }
+      s = sat(s + c);
     }
     star = s < star ? s : star;
   }
   star = wave_min_u32(star);
-  if (lane_id() == 0 && star < kInf) atomicMin(&sp->best, star);
+  if (lane == 0 && star < kInf) atomicMin(&sp->best, star);
 }
 
 __device__ void item_vvr(const GraphDev& g, Slot* sp, WaveLDS* L) {
@@ -832,7 +868,7 @@ __device__ void item_live(Slot* sp, uint32_t tile, uint64_t V, WaveLDS* L) {
     if (idx < v1) {
       v = sp->vsrc ? (uint64_t)sp->vsrc[idx] : idx;
       // lower bound on any tree through v that covers every group
-      lv = lb_le(crow(sp, v, sp->D), k, (uint32_t)W - 1, sp->R, bst < kInf ? bst : kInf - 1);
+      lv = lb_le(crow(sp, v, sp->D), k, (uint32_t)W - 1, (g_cu32*)sp->R, bst < kInf ? bst : kInf - 1);
       if (lv) sp->lvf[v] = 1;
     }
     const uint64_t ball = __ballot(lv);
@@ -932,14 +968,14 @@ __device__ void item_merge(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* 
       const uint32_t c = (uint32_t)(bestpk >> 32);
       if (c >= kInf) continue;
       // a state of height nd reaches the rest of its tree within D - nd hops
-      if (c + lower_bound(crow(sp, v, hop ? D - nd : 0), k, x, sp->R) > bst0) continue;
+      if (c + lower_bound(crow(sp, v, hop ? D - nd : 0), k, x, (g_cu32*)sp->R) > bst0) continue;
       sp->T[((uint64_t)v * ND + nd) * W + p] = bestpk;
       atomicOr((unsigned long long*)&L->f.wm[jj], 1ull << p);
       // upper bounds from complete walk-trees at v: this state plus one
       // shortest path per missing group (hop: paths of at most h hops, with
       // nd + h <= D and, for two or more paths, 2h <= D) ...
       const int h = hop ? ((__popc(x) >= 2 && D / 2 < D - nd) ? D / 2 : D - nd) : 0;
-      const uint32_t* cr = crow(sp, v, h);
+      g_cu32* cr = crow(sp, v, h);
       uint32_t psum = 0;
       for (int i = 0; i < k; i++)
         if ((x >> i) & 1u) psum = sat(psum + cr[i]);

@@ -220,11 +220,13 @@ struct JobLayout {
 JobLayout job_layout(const gst_graph* g, int L, int nmem) {
   const size_t V = (size_t)g->V;
   JobLayout J;
-  J.Lp = (L + 3) & ~3;
+  // rows of whole chunks, and of whole u64 words (FILL writes pairs of labels)
+  const int unit = gst::kJobChunkLabels > 2 ? gst::kJobChunkLabels : 2;
+  J.Lp = (L + unit - 1) / unit * unit;
   // lanes per arc: the smallest power of two that gives each lane at most
-  // kJobChunksPerLane 4-label chunks (256 labels: 16 lanes, 4 arcs per wave
-  // iteration; 5 labels: 1 lane, 64 arcs)
-  const int nch = J.Lp / 4;
+  // kJobChunksPerLane chunks (64 chunks: 16 lanes, 4 arcs per wave iteration;
+  // one or two chunks: 1 lane, 64 arcs)
+  const int nch = J.Lp / gst::kJobChunkLabels;
   J.G = 1;
   while (J.G * gst::kJobChunksPerLane < nch) J.G <<= 1;
   J.chunk = (int)gst::kJobArcs;
