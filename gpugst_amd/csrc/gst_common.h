@@ -111,7 +111,7 @@ struct alignas(64) Slot {
   uint64_t* T;          // states with |p| >= 2; only live vertices' segments are touched
   uint64_t* S1;         // V * ND * k singleton states T(v, {i}, d); null when C comes from a job
   uint32_t* istart;     // 3 * istride: frontier index holding the first arc of each ROUND item
-  uint32_t* dist;       // job: V * Lp distances
+  uint32_t* dist;       // job: V * ND * Lp distances (per vertex, one row of Lp labels per height)
   int32_t* lgrp;        // job: group id of each label
   uint32_t* jmem;       // job: member slot indices
   uint32_t* vsrc;       // vertices LIVE examines (null: all V); job: the vertices it reached
@@ -121,8 +121,11 @@ struct alignas(64) Slot {
   uint8_t* lvf;         // V: live flag per vertex
   uint64_t* mask;       // 3 * V (triple-buffered dirty sub-masks)
   uint32_t* C;          // group->vertex distances c_i(v) = C[v * cstride + i]: the query's own
-                        // V * ND * k table (row h: within h hops), or its k columns of its
-                        // job's distance rows
+                        // V * ND * k table (row h: within h hops), or, for an unbounded
+                        // job-fed query, its k columns of its job's distance rows
+  uint32_t* J;          // job-fed query: its first label in the job's rows (null otherwise);
+                        // singleton T(v, {i}, d) costs J[v * jvs + d * jds + i]
+  uint32_t jvs, jds;    // job-fed query: row stride per vertex and per height
   uint32_t* list;       // 3 * V frontier vertices
   uint32_t* loff;       // 3 * V arc offsets of frontier entries
   uint32_t* R;          // W * k * k route table

@@ -125,8 +125,8 @@ typedef GST_AS_GLOBAL unsigned long long g_u64;
 struct States {
   g_u64* T;
   g_u64* S1;
-  g_cu32* C;
-  uint32_t cs;
+  g_cu32* J;
+  uint32_t jvs, jds;
   int k, ND, W;
   // address of a state that is written (never a singleton of a C-fed slot)
   __device__ __forceinline__ g_u64* at(uint32_t v, uint32_t p, uint32_t d) const {
@@ -137,9 +137,13 @@ struct States {
     if ((p & (p - 1u)) == 0u) {
       const uint32_t i = (uint32_t)(__ffs(p) - 1);
       if (S1) return S1[((uint64_t)v * ND + d) * (uint64_t)k + i];
-      return ((uint64_t)C[(uint64_t)v * cs + i] << 32) | kPredPath;
+      return ((uint64_t)jcost(v, i, d) << 32) | kPredPath;
     }
     return T[((uint64_t)v * ND + d) * (uint64_t)W + p];
+  }
+  // cost of the singleton T(v, {i}, d) of a job-fed query
+  __device__ __forceinline__ uint32_t jcost(uint64_t v, uint32_t i, uint32_t d) const {
+    return J[v * jvs + (uint64_t)d * jds + i];
   }
 };
 
@@ -147,8 +151,9 @@ __device__ __forceinline__ States states_of(const Slot* sp) {
   States st;
   st.T = (g_u64*)sp->T;
   st.S1 = (g_u64*)sp->S1;
-  st.C = (g_cu32*)sp->C;
-  st.cs = sp->cstride;
+  st.J = (g_cu32*)sp->J;
+  st.jvs = sp->jvs;
+  st.jds = sp->jds;
   st.k = sp->k;
   st.ND = sp->ND;
   st.W = sp->W;
@@ -333,7 +338,7 @@ __device__ __forceinline__ bool mark_dirty(uint64_t* mnxt, uint32_t u, uint64_t 
 __device__ __forceinline__ void fill_regions(const Slot* sp, uint64_t V, uint64_t* nA, uint64_t* nM,
                                              uint64_t* nF) {
   if (sp->kind == KIND_JOB) {
-    *nA = V * (uint64_t)sp->Lp / 2;
+    *nA = V * (uint64_t)sp->ND * (uint64_t)sp->Lp / 2;
     *nM = 3 * V;
     *nF = 0;
     return;
@@ -555,14 +560,14 @@ __device__ void item_jseed(const GraphDev& g, Slot* sp, uint32_t l) {
   const int32_t grp = sp->lgrp[l];
   const int64_t beg = g.gptr[grp], end = g.gptr[grp + 1];
   uint64_t* mn = sp->mask + (uint64_t)nxt * V;
-  const uint64_t Lp = (uint64_t)sp->Lp;
+  const uint64_t rs = (uint64_t)sp->ND * (uint64_t)sp->Lp;  // row stride per vertex
   for (int64_t base = beg; base < end; base += 64) {
     const int64_t t = base + lane_id();
     bool app = false;
     uint32_t v = 0;
     if (t < end) {
       v = (uint32_t)g.gmem[t];
-      sp->dist[(uint64_t)v * Lp + l] = 0u;
+      sp->dist[(uint64_t)v * rs + l] = 0u;  // height 0
       const unsigned long long old = atomicOr((unsigned long long*)&mn[v], 1ull << (l / kJobChunkLabels));
       app = (old == 0);
     }
@@ -590,7 +595,10 @@ __device__ __forceinline__ bool jrelax(uint32_t s, uint32_t t, uint32_t w, g_u32
 // Job ROUND: one chunk of arcs of the job's frontier.  G lanes share an arc
 // (64 / G arcs per wave iteration); lane r of the group relaxes the dirty
 // kJobChunkLabels-label chunks r, r + G, ... of the source row into the target
-// row (one vector load per chunk, coalesced over the group).
+// row (one vector load per chunk, coalesced over the group).  Unbounded jobs
+// relax within the one row per vertex until no chunk is dirty; hop jobs relax
+// the rows of height level - 1 into those of height level (walks of exactly
+// level arcs), one round per height.
 __device__ void item_jround(const GraphDev& g, Slot* sp, uint32_t item, FrontLDS* L) {
   const int lane = lane_id();
   const uint64_t V = (uint64_t)g.V;
@@ -606,8 +614,11 @@ __device__ void item_jround(const GraphDev& g, Slot* sp, uint32_t item, FrontLDS
   const uint32_t* lof = sp->loff + (uint64_t)cur * V;
   const uint64_t* mcur = sp->mask + (uint64_t)cur * V;
   uint64_t* mnxt = sp->mask + (uint64_t)nxt * V;
-  uint32_t* dist = sp->dist;
   const uint64_t Lp = (uint64_t)sp->Lp;
+  const uint64_t rs = (uint64_t)sp->ND * Lp;  // row stride per vertex
+  const bool hop = sp->D >= 0;
+  const uint32_t* sdist = sp->dist + (hop ? (uint64_t)(sp->level - 1) * Lp : 0);
+  uint32_t* tdist = sp->dist + (hop ? (uint64_t)sp->level * Lp : 0);
   const int nch = sp->Lp / kJobChunkLabels;
   const int G = sp->G;
   const int APW = 64 / G;
@@ -631,9 +642,9 @@ __device__ void item_jround(const GraphDev& g, Slot* sp, uint32_t item, FrontLDS
         const uint32_t e = L->wrs[o] + (pos - L->woff[o]);
         u = (uint32_t)g.col[e];
         const uint32_t w = (uint32_t)g.wt[e];
-        g_cchunk* srow = (g_cchunk*)(dist + (uint64_t)vs * Lp);
-        g_cchunk* trow = (g_cchunk*)(dist + (uint64_t)u * Lp);
-        g_u32* tw = (g_u32*)(dist + (uint64_t)u * Lp);
+        g_cchunk* srow = (g_cchunk*)(sdist + (uint64_t)vs * rs);
+        g_cchunk* trow = (g_cchunk*)(tdist + (uint64_t)u * rs);
+        g_u32* tw = (g_u32*)(tdist + (uint64_t)u * rs);
         for (int c = rr; c < nch; c += G) {
           if (!((m >> c) & 1ull)) continue;
           const jchunk sv = srow[c];
@@ -679,6 +690,10 @@ __device__ void item_jstar(Slot* slots, Slot* sp, uint32_t tile, uint64_t V) {
   const uint64_t v0 = (uint64_t)tile * kStarTV;
   const uint64_t v1 = v0 + kStarTV < V ? v0 + kStarTV : V;
   const uint64_t Lp = (uint64_t)sp->Lp;
+  const int ND = sp->ND;
+  const uint64_t rs = (uint64_t)ND * Lp;
+  // hop: arms of at most floor(D/2) hops for the star, every height for reach
+  const int hs = sp->D >= 0 ? sp->D / 2 : 0;
   const int nmem = sp->nmem;
   Slot* q0 = lane < nmem ? &slots[sp->jmem[lane]] : nullptr;
   Slot* q1 = lane + 64 < nmem ? &slots[sp->jmem[lane + 64]] : nullptr;
@@ -687,24 +702,32 @@ __device__ void item_jstar(Slot* slots, Slot* sp, uint32_t tile, uint64_t V) {
   uint32_t star0 = kInf, star1 = kInf;
   uint64_t reached = 0;
   for (uint64_t v = v0; v < v1; v++) {
-    const uint32_t* row = sp->dist + v * Lp;
+    const uint32_t* row = sp->dist + v * rs;
     bool fin = false;
-    if (q0) {
-      uint32_t s = 0;
-      for (int i = 0; i < k0; i++) s = sat(s + row[lab0 + i]);
-      star0 = s < star0 ? s : star0;
-      fin = s < kInf;
-    }
-    if (q1) {
-      uint32_t s = 0;
-      for (int i = 0; i < k1; i++) s = sat(s + row[lab1 + i]);
-      star1 = s < star1 ? s : star1;
-      fin = fin || s < kInf;
+    for (int m = 0; m < 2; m++) {
+      const int km = m ? k1 : k0;
+      const uint32_t lab = m ? lab1 : lab0;
+      if (km == 0) continue;
+      uint32_t s = 0, sr = 0;
+      for (int i = 0; i < km; i++) {
+        uint32_t ch = kInf, cr = kInf;
+        for (int d = 0; d < ND; d++) {
+          const uint32_t c = row[(uint64_t)d * Lp + lab + i];
+          cr = c < cr ? c : cr;
+          if (d == hs) ch = cr;
+        }
+        s = sat(s + ch);
+        sr = sat(sr + cr);
+      }
+      if (m) star1 = s < star1 ? s : star1;
+      else star0 = s < star0 ? s : star0;
+      fin = fin || sr < kInf;
     }
     if (__ballot(fin)) reached |= 1ull << (v - v0);
   }
   // shortest paths from one vertex to every group form a connected cover, so
-  // their sum bounds the optimum from above
+  // their sum bounds the optimum from above (for a hop query, paths of at most
+  // floor(D/2) hops: every two of them are within D hops of each other)
   if (q0 && star0 < kInf) atomicMin(&q0->best, star0);
   if (q1 && star1 < kInf) atomicMin(&q1->best, star1);
   const bool r = (reached >> lane) & 1ull;
@@ -723,7 +746,8 @@ __device__ void item_jstar(Slot* slots, Slot* sp, uint32_t tile, uint64_t V) {
 // The query's own C rows are as long as its S1 rows (cstride == ND * k), so a
 // tile of vertices is one contiguous run of both: lanes walk it element by
 // element (vertex, height, group), coalesced, each taking the minimum over the
-// heights up to its own.
+// heights up to its own.  A job-fed hop query reads its singleton costs from
+// the job's rows instead (its star bound came from the job's STAR).
 __device__ void item_dist(const GraphDev& g, Slot* sp, uint32_t tile) {
   const int lane = lane_id();
   const uint64_t V = (uint64_t)g.V;
@@ -732,19 +756,34 @@ __device__ void item_dist(const GraphDev& g, Slot* sp, uint32_t tile) {
   const uint64_t v0 = (uint64_t)tile * kDistTV;
   const uint64_t v1 = v0 + kDistTV < V ? v0 + kDistTV : V;
   const uint32_t rl = (uint32_t)(ND * k);
-  const uint64_t* S1t = sp->S1 + v0 * rl;
   uint32_t* Ct = sp->C + v0 * rl;
   const uint32_t nE = (uint32_t)(v1 - v0) * rl;
-  for (uint32_t e = (uint32_t)lane; e < nE; e += 64) {
-    const uint32_t vr = e / rl, rem = e - vr * rl;
-    const uint32_t d = rem / (uint32_t)k, i = rem - d * (uint32_t)k;
-    const uint64_t* col = S1t + vr * rl + i;
-    uint32_t c = kInf;
-    for (uint32_t dd = 0; dd <= d; dd++) {
-      const uint32_t t = (uint32_t)(col[dd * (uint32_t)k] >> 32);
-      c = t < c ? t : c;
+  if (sp->S1) {
+    const uint64_t* S1t = sp->S1 + v0 * rl;
+    for (uint32_t e = (uint32_t)lane; e < nE; e += 64) {
+      const uint32_t vr = e / rl, rem = e - vr * rl;
+      const uint32_t d = rem / (uint32_t)k, i = rem - d * (uint32_t)k;
+      const uint64_t* col = S1t + vr * rl + i;
+      uint32_t c = kInf;
+      for (uint32_t dd = 0; dd <= d; dd++) {
+        const uint32_t t = (uint32_t)(col[dd * (uint32_t)k] >> 32);
+        c = t < c ? t : c;
+      }
+      Ct[e] = c;  // within d hops
     }
-    Ct[e] = c;  // within d hops
+  } else {
+    const States S = states_of(sp);
+    for (uint32_t e = (uint32_t)lane; e < nE; e += 64) {
+      const uint32_t vr = e / rl, rem = e - vr * rl;
+      const uint32_t d = rem / (uint32_t)k, i = rem - d * (uint32_t)k;
+      uint32_t c = kInf;
+      for (uint32_t dd = 0; dd <= d; dd++) {
+        const uint32_t t = S.jcost(v0 + vr, i, dd);
+        c = t < c ? t : c;
+      }
+      Ct[e] = c;
+    }
+    return;
   }
   // star upper bound; the arms of a hop star are pairwise within D hops when
   // each has at most floor(D/2)
@@ -770,6 +809,7 @@ __device__ void item_dist(const GraphDev& g, Slot* sp, uint32_t tile) {
 __device__ void item_vvr(const GraphDev& g, Slot* sp, WaveLDS* L) {
   const int lane = lane_id();
   const int k = sp->k, W = sp->W, ND = sp->ND;
+  const States S = states_of(sp);
   uint32_t* vvl = (uint32_t*)L->f.wv;  // k*k <= 36 words
   for (int i = 0; i < k; i++) {
     const int32_t grp = sp->qg[i];
@@ -790,7 +830,11 @@ __device__ void item_vvr(const GraphDev& g, Slot* sp, WaveLDS* L) {
             c = t2 < c ? t2 : c;
           }
         } else {
-          c = sp->C[(uint64_t)v * sp->cstride + j];
+          // the job's rows (for an unbounded query they are its C)
+          for (int d = 0; d < ND; d++) {
+            const uint32_t t2 = S.jcost(v, (uint32_t)j, (uint32_t)d);
+            c = t2 < c ? t2 : c;
+          }
         }
         loc[j] = c < loc[j] ? c : loc[j];
       }
@@ -1089,9 +1133,10 @@ __device__ void item_final(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* 
 
 // Predecessor chase from the optimal root: breadth-first over the witness,
 // one lane per state; edges get their parent edge index so that a tree-like
-// walk (hop DP) keeps its shape.  A singleton read from C (kPredPath) has no
-// stored predecessor: its next edge is an arc v->u with C[u][i] + w == C[v][i],
-// found by a wave-wide scan of v's row (it exists because C is exact).
+// walk (hop DP) keeps its shape.  A singleton read from a job's rows
+// (kPredPath) has no stored predecessor: its next edge is an arc v->u with
+// J[u][i] + w == J[v][i] (for a hop job, J at height d - 1 against height d),
+// found by a wave-wide scan of v's row (it exists because the rows are exact).
 __device__ void item_recon(const GraphDev& g, Slot* sp) {
   const int lane = lane_id();
   const int W = sp->W, ND = sp->ND, D = sp->D;
@@ -1154,6 +1199,7 @@ __device__ void item_recon(const GraphDev& g, Slot* sp) {
       const uint32_t lv = __shfl(v, l, 64);
       const uint32_t li = (uint32_t)(__ffs(__shfl(p, l, 64)) - 1);
       const uint32_t lc = __shfl(sc, l, 64);
+      const uint32_t ld = __shfl(d, l, 64);
       const uint32_t e0 = (uint32_t)g.rowptr[lv], e1 = (uint32_t)g.rowptr[lv + 1];
       uint32_t found = 0xFFFFFFFFu;
       for (uint32_t b = e0; b < e1; b += 64) {
@@ -1161,7 +1207,7 @@ __device__ void item_recon(const GraphDev& g, Slot* sp) {
         bool ok = false;
         if (e < e1) {
           const uint32_t u = (uint32_t)g.col[e];
-          ok = sp->C[(uint64_t)u * sp->cstride + li] + (uint32_t)g.wt[e] == lc;
+          ok = S.jcost(u, li, hop ? ld - 1u : 0u) + (uint32_t)g.wt[e] == lc;
         }
         const uint64_t hb = __ballot(ok);
         if (hb) {
@@ -1268,12 +1314,15 @@ __device__ void complete_step(Slot* slots, Slot* s, Ctrl* ctrl) {
         s->phase = PH_SEED;
         break;
       case PH_SEED:
-        s->phase = PH_ROUND;
+        // a hop job computes heights 1..D, one round each
+        s->level = 1;
+        s->phase = (s->D == 0) ? PH_STAR : PH_ROUND;
         break;
       case PH_ROUND:
         s->ctr[1] += (unsigned long long)(s->app[r % 3] >> 32);
         s->ctr[2] += (unsigned long long)(uint32_t)s->app[r % 3];
-        if ((s->app[(r + 1) % 3] >> 32) == 0) s->phase = PH_STAR;
+        if ((s->app[(r + 1) % 3] >> 32) == 0 || (s->D >= 0 && s->level >= s->D)) s->phase = PH_STAR;
+        else if (s->D >= 0) s->level++;
         break;
       case PH_STAR:
         for (int m = 0; m < s->nmem; m++) {
@@ -1356,7 +1405,7 @@ __device__ uint32_t plan_step(Slot* s, uint64_t V) {
       if (!job) cls = kClsHeavy;  // the per-query grow carries the lower-bound code
       break;
     case PH_STAR: work = ceil_div(V, kStarTV); break;
-    case PH_DIST: work = (s->S1 ? ceil_div(V, kDistTV) : 0u) + 1u; cls = kClsHeavy; break;
+    case PH_DIST: work = ((s->S1 || s->D >= 0) ? ceil_div(V, kDistTV) : 0u) + 1u; cls = kClsHeavy; break;
     case PH_LIVE: work = ceil_div(s->vsrc ? (uint64_t)s->nvsrc : V, kDistTV); cls = kClsHeavy; break;
     case PH_MERGE:
     case PH_FINAL: work = ceil_div(s->nlive, kMergeTV); cls = kClsHeavy; break;

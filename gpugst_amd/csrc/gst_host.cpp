@@ -178,11 +178,13 @@ uint32_t istride_for(const gst_graph* g, int64_t chunk) {
   return (uint32_t)(g->n_arcs / std::max<int64_t>(chunk, 1) + 2);
 }
 
-// True when the query takes its level 1 from a group-distance job: unbounded
-// DP with k >= 2 (k = 1 is answered at SEED).
-bool uses_job(const gst_graph* g, int k, int D) { return g->opt.group_pass != 0 && D < 0 && k >= 2; }
+// True when the query can take its level 1 from a group-distance job (k >= 2;
+// k = 1 is answered at SEED).  A job serves queries of one hop bound.
+bool uses_job(const gst_graph* g, int k, int D) { return g->opt.group_pass != 0 && k >= 2; }
 
 // Device memory of one query; `own` = it computes its own level 1 (S1 table).
+// Every query with its own level 1 or a hop bound has its own C table (rows
+// within 0..D hops); an unbounded job-fed query reads C from the job's rows.
 Layout layout_for(const gst_graph* g, int k, int D, bool own) {
   const size_t V = (size_t)g->V;
   const size_t W = size_t(1) << k;
@@ -197,7 +199,7 @@ Layout layout_for(const gst_graph* g, int k, int D, bool own) {
   L.live = off; off += align256(V * 4);
   L.lvf = off;  off += align256(((V + 7) / 8) * 8);
   L.mask = off; off += align256(3 * V * 8);
-  L.C = off;    off += own ? align256(V * ND * k * 4) : 0;  // rows within 0..D hops; else the job's rows
+  L.C = off;    off += (own || D >= 0) ? align256(V * ND * k * 4) : 0;
   L.list = off; off += align256(3 * V * 4);
   L.loff = off; off += align256(3 * V * 4);
   L.istart = off; off += align256(3 * (size_t)L.istride * 4);
@@ -217,7 +219,7 @@ struct JobLayout {
   uint32_t istride;
 };
 
-JobLayout job_layout(const gst_graph* g, int L, int nmem) {
+JobLayout job_layout(const gst_graph* g, int L, int nmem, int D) {
   const size_t V = (size_t)g->V;
   JobLayout J;
   // rows of whole chunks, and of whole u64 words (FILL writes pairs of labels)
@@ -232,7 +234,8 @@ JobLayout job_layout(const gst_graph* g, int L, int nmem) {
   J.chunk = (int)gst::kJobArcs;
   J.istride = istride_for(g, J.chunk);
   size_t off = 0;
-  J.dist = off;   off += align256(V * (size_t)J.Lp * 4);
+  const size_t ND = D < 0 ? 1 : (size_t)D + 1;
+  J.dist = off;   off += align256(V * ND * (size_t)J.Lp * 4);  // one row per vertex and height
   J.mask = off;   off += align256(3 * V * 8);
   J.list = off;   off += align256(3 * V * 4);
   J.loff = off;   off += align256(3 * V * 4);
@@ -353,7 +356,7 @@ class Engine {
       int L = 0;
       for (auto it = pend.begin(); it != pend.end(); ++it) {
         const int k = (int)(*it)->groups.size();
-        if (!uses_job(g_, k, (*it)->D) || L + k > gst::kJobLabels) break;
+        if (!uses_job(g_, k, (*it)->D) || (*it)->D != q0->D || L + k > gst::kJobLabels) break;
         if (used_slots_ + (int)grp.size() + 2 > max_slots_) break;
         grp.push_back(*it);
         L += k;
@@ -389,6 +392,7 @@ class Engine {
   // first label there and jdist / jLp the job's distance rows (the query's
   // C).  Returns the slot index, -1 when memory or slots are exhausted.
   int admit_query(const QueryReq& q, int job, int lab, uint32_t* jdist = nullptr, int jLp = 0) {
+    // jdist: the job's rows, ND rows of jLp labels per vertex
     const int k = (int)q.groups.size();
     const bool own = job < 0;
     const Layout L = layout_for(g_, k, q.D, own);
@@ -424,8 +428,14 @@ class Engine {
     d.live = (uint32_t*)(base + L.live);
     d.lvf = (uint8_t*)(base + L.lvf);
     d.mask = (uint64_t*)(base + L.mask);
-    d.C = own ? (uint32_t*)(base + L.C) : jdist + lab;
-    d.cstride = own ? (uint32_t)(k * d.ND) : (uint32_t)jLp;
+    const bool cown = own || q.D >= 0;
+    d.C = cown ? (uint32_t*)(base + L.C) : jdist + lab;
+    d.cstride = cown ? (uint32_t)(k * d.ND) : (uint32_t)jLp;
+    if (!own) {
+      d.J = jdist + lab;
+      d.jvs = (uint32_t)(d.ND * jLp);
+      d.jds = q.D >= 0 ? (uint32_t)jLp : 0u;
+    }
     d.list = (uint32_t*)(base + L.list);
     d.loff = (uint32_t*)(base + L.loff);
     d.istart = (uint32_t*)(base + L.istart);
@@ -457,7 +467,8 @@ class Engine {
 
   // One job over the labels of `grp` plus its member queries; all or nothing.
   bool admit_job_group(const std::vector<QueryReq*>& grp, int L) {
-    const JobLayout J = job_layout(g_, L, (int)grp.size());
+    const int D = grp.front()->D;
+    const JobLayout J = job_layout(g_, L, (int)grp.size(), D);
     const int js = free_slot();
     if (js < 0) return false;
     size_t off = 0;
@@ -489,6 +500,8 @@ class Engine {
     memset(&d, 0, sizeof(d));
     d.kind = gst::KIND_JOB;
     d.job = -1;
+    d.D = D;
+    d.ND = D < 0 ? 1 : D + 1;
     d.L = L;
     d.Lp = J.Lp;
     d.G = J.G;
@@ -879,7 +892,7 @@ gst_status engine_for(gst_graph* g, bool batch, size_t need, Engine** out) {
 // group-distance job when it has one (a job of its own when alone).
 size_t single_query_bytes(const gst_graph* g, int k, int D) {
   if (!uses_job(g, k, D) || g->opt.job_min_queries > 1) return layout_for(g, k, D, true).total;
-  return layout_for(g, k, D, false).total + job_layout(g, k, 1).total;
+  return layout_for(g, k, D, false).total + job_layout(g, k, 1, D).total;
 }
 
 // Drives an engine over a list of queries: admits while memory and slots

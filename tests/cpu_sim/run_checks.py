@@ -8,7 +8,9 @@ at tests/cpu_sim/_build/libgst_cpu.so.  Prints one JSON line of results.
   jobs   : one batch of random k=5 queries (duplicates allowed) over several
            group-distance jobs, against the oracle and against group_pass=0
   hop    : one batch of random k=3..5 queries with hop bounds 2..4 (depth-aware
-           bounds and hop star / path upper bounds), against the oracle
+           bounds and hop star / path upper bounds), then the same queries
+           ordered by bound (runs of one bound share hop group-distance jobs),
+           against the oracle
 """
 import json
 import os
@@ -105,16 +107,23 @@ def check_hop(V, nq, seed):
     g = Graph(rp, col, wt)
     g.set_groups(*groups_csr(groups))
     rs = g.solve_batch(qs, Ds)
+    # the same queries ordered by bound: runs of one D share hop jobs
+    order = sorted(range(nq), key=lambda i: Ds[i])
+    ro = g.solve_batch([qs[i] for i in order], [Ds[i] for i in order])
+    rj = [None] * nq
+    for pos, i in enumerate(order):
+        rj[i] = ro[pos]
     bad, feas = [], 0
     for i, q in enumerate(qs):
         gl = [groups[x] for x in q]
         exp = O.dpbf(rp, col, wt, gl, D=Ds[i])[0]
-        ok = rs[i].cost == exp
-        if ok and rs[i].feasible:
-            feas += 1
-            ok = validate_tree(rp, col, wt, gl, rs[i].cost, rs[i].root, rs[i].edges, D=Ds[i])["ok"]
+        ok = rs[i].cost == exp and rj[i].cost == exp
+        for r in (rs[i], rj[i]):
+            if ok and r.feasible:
+                ok = validate_tree(rp, col, wt, gl, r.cost, r.root, r.edges, D=Ds[i])["ok"]
+        feas += int(ok and rs[i].feasible)
         if not ok:
-            bad.append([i, Ds[i], rs[i].cost, exp])
+            bad.append([i, Ds[i], rs[i].cost, rj[i].cost, exp])
     g.close()
     return {"checked": nq, "feasible": feas, "bad": bad}
 
