@@ -119,6 +119,8 @@ struct alignas(64) Slot {
   uint32_t pad3;
   uint32_t* live;       // V: vertices that can lie on a tree cheaper than the bound
   uint8_t* lvf;         // V: live flag per vertex
+  uint32_t* hold;       // V bits: the vertex holds a finite state with |p| >= 2
+  uint32_t* act;        // V: vertices with their hold bit set, in order of setting
   uint64_t* mask;       // 3 * V (triple-buffered dirty sub-masks)
   uint32_t* C;          // group->vertex distances c_i(v) = C[v * cstride + i]: the query's own
                         // V * ND * k table (row h: within h hops), or, for an unbounded
@@ -140,7 +142,7 @@ struct alignas(64) Slot {
   uint32_t best;        // upper bound on the optimum (cost of a real walk-tree)
   uint32_t status;
   uint32_t nlive;
-  uint32_t pad1;
+  uint32_t nact;        // entries of act
   uint64_t ans;         // (cost << 32) | root vertex
   uint32_t vv[kMaxK * kMaxK];
   int32_t out_cost, out_root, out_nedges, out_d;

@@ -330,11 +330,33 @@ __device__ __forceinline__ bool mark_dirty(uint64_t* mnxt, uint32_t u, uint64_t 
   return om == 0;
 }
 
+// Sets v's hold bit (flag: v received a finite state with |p| >= 2) and
+// appends v to the active list when the bit is new.  Wave-collective: every
+// lane calls it, with flag false where it has nothing.  A merge of a subset of
+// three or more groups at v needs a part of two or more groups at v, so only
+// vertices in the active list can hold such a merge.
+__device__ __forceinline__ void mark_hold(bool flag, uint32_t v, Slot* sp) {
+  bool fresh = false;
+  if (flag) {
+    uint32_t* w = sp->hold + (v >> 5);
+    const uint32_t bit = 1u << (v & 31u);
+    if (!(__hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & bit))
+      fresh = (atomicOr(w, bit) & bit) == 0u;
+  }
+  const uint64_t ball = __ballot(fresh);
+  if (ball == 0) return;
+  const int leader = __ffsll((long long)ball) - 1;
+  uint32_t base = 0;
+  if (lane_id() == leader) base = atomicAdd(&sp->nact, (uint32_t)__popcll(ball));
+  base = __shfl(base, leader, 64);
+  if (fresh) sp->act[base + __popcll(ball & __lanemask_lt())] = v;
+}
+
 // ---------------------------------------------------------------- items ----
 
 // u64 words that FILL initialises: job [distances][masks]; query [S1][masks]
-// [live flags] (S1 only when the query computes its own level 1; its C table
-// needs no fill, DIST writes every entry)
+// [live flags][hold bits] (S1 only when the query computes its own level 1;
+// its C table needs no fill, DIST writes every entry)
 __device__ __forceinline__ void fill_regions(const Slot* sp, uint64_t V, uint64_t* nA, uint64_t* nM,
                                              uint64_t* nF) {
   if (sp->kind == KIND_JOB) {
@@ -346,7 +368,7 @@ __device__ __forceinline__ void fill_regions(const Slot* sp, uint64_t V, uint64_
   const bool own = sp->S1 != nullptr;
   *nA = own ? V * (uint64_t)sp->ND * (uint64_t)sp->k : 0;
   *nM = 3 * V;
-  *nF = (V + 7) / 8;
+  *nF = (V + 7) / 8 + (V + 63) / 64;  // live flags, then hold bits
 }
 
 __device__ void item_fill(Slot* sp, uint32_t idx, uint64_t V) {
@@ -470,7 +492,7 @@ __device__ void item_round(const GraphDev& g, Slot* sp, uint32_t chunk, FrontLDS
     const uint32_t hia = a1 < wend ? a1 : wend;
     for (uint32_t base = a; base < hia; base += 64) {
       const uint32_t pos = base + lane;
-      bool app = false;
+      bool app = false, held = false;
       uint32_t u = 0;
       if (pos < hia) {
         const int o = find_entry(L, pos);
@@ -536,7 +558,9 @@ __device__ void item_round(const GraphDev& g, Slot* sp, uint32_t chunk, FrontLDS
           }
         }
         if (improved) app = mark_dirty(mnxt, u, improved);
+        held = improved != 0;
       }
+      if (level >= 2) mark_hold(held, u, sp);
       stage_append(app, u, L, apn, sp, nxt, g, V);
     }
     a = hia;
@@ -952,14 +976,19 @@ __device__ void item_merge(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* 
   const int nxt = (sp->step + 1) % 3;
   uint64_t* mnxt = sp->mask + (uint64_t)nxt * V;
   const int rowlen = W * ND;
-  const int VPW = (kStageWords / rowlen) < 64 ? (kStageWords / rowlen) : 64;
+  // each staged vertex also brings its C rows (ND * k words, in pairs)
+  const int crl = (ND * k + 1) / 2;
+  const int VPW = (kStageWords / (rowlen + crl)) < 64 ? (kStageWords / (rowlen + crl)) : 64;
   // this level's subsets, listed in LDS
   uint32_t* subs = L->f.pad;
   const bool mine = lane < W && __popc((uint32_t)lane) == level;
   const uint64_t sball = __ballot(mine);
   if (mine) subs[__popcll(sball & __lanemask_lt())] = (uint32_t)lane;
   const int nL = __popcll(sball);
-  const uint64_t nl = sp->nlive;
+  // level 2 merges singletons of every live vertex; a higher level needs a part
+  // of two or more groups at the vertex, so only active vertices can hold one
+  const uint32_t* vl = level >= 3 ? sp->act : sp->live;
+  const uint64_t nl = level >= 3 ? sp->nact : sp->nlive;
   const uint64_t v0 = (uint64_t)tile * kMergeTV;
   const uint64_t v1 = v0 + kMergeTV < nl ? v0 + kMergeTV : nl;
   const uint32_t bst0 = __hip_atomic_load(&sp->best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -967,9 +996,14 @@ __device__ void item_merge(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* 
   uint64_t n_eval = 0;
   for (uint64_t vb = v0; vb < v1; vb += VPW) {
     const int nv = (int)((v1 - vb) < (uint64_t)VPW ? (v1 - vb) : (uint64_t)VPW);
-    stage_rows(sp, sp->live + vb, nv, rowlen, W, L->stage);
+    stage_rows(sp, vl + vb, nv, rowlen, W, L->stage);
+    uint32_t* cst = (uint32_t*)(L->stage + nv * rowlen);  // C rows of the staged vertices
+    for (int t = lane; t < nv * ND * k; t += 64) {
+      const int jj = t / (ND * k);
+      cst[t] = crow(sp, vl[vb + jj], 0)[t - jj * ND * k];
+    }
     if (lane < nv) {
-      L->f.wv[lane] = sp->live[vb + lane];
+      L->f.wv[lane] = vl[vb + lane];
       L->f.wm[lane] = 0;
     }
     wave_sync();
@@ -982,6 +1016,20 @@ __device__ void item_merge(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* 
       const uint32_t v = L->f.wv[jj];
       const uint64_t* row = L->stage + jj * rowlen;
       const uint32_t x = full ^ p;
+      // The subtree part reaches every group of p within nd hops and the rest
+      // every group of ~p within D - nd hops, on disjoint edges: when the two
+      // farthest groups already cost more than the upper bound, the state
+      // cannot pass the prune below and its walk-trees cannot lower the bound.
+      {
+        const uint32_t* ct = cst + jj * ND * k + (hop ? nd * k : 0);
+        const uint32_t* cr = cst + jj * ND * k + (hop ? (D - nd) * k : 0);
+        uint32_t mt = 0, mr = 0;
+        for (int i = 0; i < k; i++) {
+          if ((p >> i) & 1u) mt = ct[i] > mt ? ct[i] : mt;
+          else mr = cr[i] > mr ? cr[i] : mr;
+        }
+        if (sat(mt + mr) > bst0) continue;
+      }
       const uint32_t low = p & (0u - p);
       uint64_t bestpk = row[nd * W + p];
       for (uint32_t sub = (p - 1) & p; sub; sub = (sub - 1) & p) {
@@ -1045,6 +1093,7 @@ __device__ void item_merge(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* 
         app = true;
       }
     }
+    if (level == 2) mark_hold(app, v, sp);  // above level 2 the vertex is active already
     wave_append(app, v, sp, nxt, g, V);
     wave_sync();
   }
@@ -1059,34 +1108,52 @@ __device__ void item_merge(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* 
 // Full-set merge: W/2 lanes per vertex, one lane per split (sub contains bit 0).
 __device__ void item_final(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* L) {
   const int lane = lane_id();
-  const int W = sp->W, ND = sp->ND, D = sp->D;
+  const int k = sp->k, W = sp->W, ND = sp->ND, D = sp->D;
   const bool hop = D >= 0;
   const int Dlim = hop ? D : 0;
   const uint32_t full = (uint32_t)W - 1;
   const int rowlen = W * ND;
   const int half = W / 2;  // W >= 2
+  const int crl = (ND * k + 1) / 2;  // C rows of a staged vertex, in u64 words
   int VPF = 64 / half;
-  const int cap = kStageWords / rowlen;
+  const int cap = kStageWords / (rowlen + crl);
   if (VPF > cap) VPF = cap;
+  const uint32_t bst0 = __hip_atomic_load(&sp->best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   const int j = lane / half;
   const int s = lane % half;
   const uint32_t sub = 2u * (uint32_t)s + 1u;
   const uint32_t sub2 = full ^ sub;
-  const uint64_t nl = sp->nlive;
+  // with k >= 3 one part of every split has two or more groups: active vertices
+  const uint32_t* vl = k >= 3 ? sp->act : sp->live;
+  const uint64_t nl = k >= 3 ? sp->nact : sp->nlive;
   const uint64_t v0 = (uint64_t)tile * kMergeTV;
   const uint64_t v1 = v0 + kMergeTV < nl ? v0 + kMergeTV : nl;
   uint64_t ansmin = ~0ull;
   uint64_t n_eval = 0;
   for (uint64_t vb = v0; vb < v1; vb += VPF) {
     const int nv = (int)((v1 - vb) < (uint64_t)VPF ? (v1 - vb) : (uint64_t)VPF);
-    stage_rows(sp, sp->live + vb, nv, rowlen, W, L->stage);
+    stage_rows(sp, vl + vb, nv, rowlen, W, L->stage);
+    uint32_t* cst = (uint32_t*)(L->stage + nv * rowlen);
+    for (int t = lane; t < nv * ND * k; t += 64) {
+      const int jj = t / (ND * k);
+      cst[t] = crow(sp, vl[vb + jj], 0)[t - jj * ND * k];
+    }
     wave_sync();
     const bool vact = j < nv;
-    const uint32_t v = vact ? sp->live[vb + j] : 0u;
+    const uint32_t v = vact ? vl[vb + j] : 0u;
     const uint64_t* row = L->stage + (vact ? j : 0) * rowlen;
-    const bool sact = vact && sub != full;
     for (int nd = 0; nd < ND; nd++) {
-      uint64_t bestpk = (vact && s == 0) ? row[nd * W + full] : kInfState;
+      // a tree of height nd at v reaches every group within nd hops: when the
+      // farthest one costs more than the upper bound it cannot be the answer
+      bool vok = vact;
+      if (vok) {
+        const uint32_t* ct = cst + j * ND * k + (hop ? nd * k : 0);
+        uint32_t mt = 0;
+        for (int i = 0; i < k; i++) mt = ct[i] > mt ? ct[i] : mt;
+        vok = mt <= bst0;
+      }
+      const bool sact = vok && sub != full;
+      uint64_t bestpk = (vok && s == 0) ? row[nd * W + full] : kInfState;
       if (sact) {
         const int lim = Dlim - nd;
         if (lim >= 0) {
@@ -1407,8 +1474,8 @@ __device__ uint32_t plan_step(Slot* s, uint64_t V) {
     case PH_STAR: work = ceil_div(V, kStarTV); break;
     case PH_DIST: work = ((s->S1 || s->D >= 0) ? ceil_div(V, kDistTV) : 0u) + 1u; cls = kClsHeavy; break;
     case PH_LIVE: work = ceil_div(s->vsrc ? (uint64_t)s->nvsrc : V, kDistTV); cls = kClsHeavy; break;
-    case PH_MERGE:
-    case PH_FINAL: work = ceil_div(s->nlive, kMergeTV); cls = kClsHeavy; break;
+    case PH_MERGE: work = ceil_div(s->level >= 3 ? s->nact : s->nlive, kMergeTV); cls = kClsHeavy; break;
+    case PH_FINAL: work = ceil_div(s->k >= 3 ? s->nact : s->nlive, kMergeTV); cls = kClsHeavy; break;
     case PH_RECON: work = 1; cls = kClsHeavy; break;
     default: return (s->n_work = s->n_clr = 0);
   }

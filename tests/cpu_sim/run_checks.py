@@ -7,6 +7,8 @@ at tests/cpu_sim/_build/libgst_cpu.so.  Prints one JSON line of results.
            validated
   jobs   : one batch of random k=5 queries (duplicates allowed) over several
            group-distance jobs, against the oracle and against group_pass=0
+  branching: a vertex that holds states of two or more groups only through
+           grows, where the optimum needs a merge
   hop    : one batch of random k=3..5 queries with hop bounds 2..4 (depth-aware
            bounds and hop star / path upper bounds), then the same queries
            ordered by bound (runs of one bound share hop group-distance jobs),
@@ -128,11 +130,35 @@ def check_hop(V, nq, seed):
     return {"checked": nq, "feasible": feas, "bad": bad}
 
 
+def check_branching():
+    """A vertex x where three two-group branches meet, holding its level-2
+    states only through grows.  Each group also has a member in a second
+    component, joined to one hub by edges of weight 7 (one of 6), so the star
+    bound is 41 from the start; that bound prunes the single-group merges at x
+    (22 + a rest bound of 20), while the branches grown into x pass (12 + 20).
+    The optimum, 36, needs the four-group merge at x: without x among the
+    merge vertices the DP answers the hub's star, 41."""
+    E = []
+    for b in range(3):
+        E += [(0, 1 + b, 10), (1 + b, 4 + 2 * b, 1), (1 + b, 5 + 2 * b, 1)]
+    E += [(10, 11 + i, 7 if i < 5 else 6) for i in range(6)]
+    rp, col, wt = data.csr_from_edges(17, E)
+    groups = [[4 + i, 11 + i] for i in range(6)]
+    exp = O.dpbf(rp, col, wt, groups)[0]
+    g = Graph(rp, col, wt)
+    g.set_groups(*groups_csr(groups))
+    r = g.solve(list(range(6)))
+    ok = r.cost == exp and validate_tree(rp, col, wt, groups, r.cost, r.root, r.edges)["ok"]
+    g.close()
+    return {"cost": r.cost, "expect": exp, "ok": bool(ok)}
+
+
 def main():
     stride = int(sys.argv[1]) if len(sys.argv) > 1 else 3
     nq = int(sys.argv[2]) if len(sys.argv) > 2 else 60
     nh = int(sys.argv[3]) if len(sys.argv) > 3 else 24
-    out = {"tiny": check_tiny(stride), "jobs": check_jobs(150, nq, 7), "hop": check_hop(150, nh, 11)}
+    out = {"tiny": check_tiny(stride), "jobs": check_jobs(150, nq, 7), "hop": check_hop(150, nh, 11),
+           "branching": check_branching()}
     print(json.dumps(out), flush=True)
 
 

@@ -51,3 +51,8 @@ def test_hop_queries_on_cpu_build(cpu_results):
     h = cpu_results["hop"]
     assert h["feasible"] >= h["checked"] // 2
     assert h["bad"] == [], h["bad"]
+
+
+def test_grown_branch_vertex_on_cpu_build(cpu_results):
+    b = cpu_results["branching"]
+    assert b["ok"] and b["cost"] == b["expect"] == 36, b

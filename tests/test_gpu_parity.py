@@ -119,6 +119,24 @@ def test_infeasible_and_edge_cases():
     g.close()
 
 
+def test_vertex_reached_only_by_grows():
+    # x = 0 joins three two-group branches; the star bound through the second
+    # component (41) prunes the single-group merges at x, so x holds states of
+    # two or more groups only through grows, and the optimum 36 needs the
+    # four-group merge there (the same instance as the CPU-build check)
+    E = []
+    for b in range(3):
+        E += [(0, 1 + b, 10), (1 + b, 4 + 2 * b, 1), (1 + b, 5 + 2 * b, 1)]
+    E += [(10, 11 + i, 7 if i < 5 else 6) for i in range(6)]
+    rp, col, wt = data.csr_from_edges(17, E)
+    groups = [[4 + i, 11 + i] for i in range(6)]
+    assert O.dpbf(rp, col, wt, groups)[0] == 36
+    g = _graph_from(rp, col, wt, groups)
+    _check(rp, col, wt, groups, g.solve(list(range(6))), 36)
+    _check(rp, col, wt, groups, g.solve_batch([list(range(6))] * 3)[1], 36)
+    g.close()
+
+
 def test_batch_matches_plain():
     inst = data.make_config(2, scale=0.02, nq=6)
     g = Graph(inst.rowptr, inst.col, inst.wt)
