@@ -975,10 +975,15 @@ __device__ void item_merge(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* 
   const uint32_t full = (uint32_t)W - 1;
   const int nxt = (sp->step + 1) % 3;
   uint64_t* mnxt = sp->mask + (uint64_t)nxt * V;
-  const int rowlen = W * ND;
+  // Level 2 reads only singletons (every state of two or more groups is still
+  // INF), so it stages the ND * k singletons of a vertex instead of its whole
+  // segment: row[d * k + i] instead of row[d * W + p].
+  const bool l2 = level == 2;
+  const int rowlen = l2 ? ND * k : W * ND;
   // each staged vertex also brings its C rows (ND * k words, in pairs)
   const int crl = (ND * k + 1) / 2;
   const int VPW = (kStageWords / (rowlen + crl)) < 64 ? (kStageWords / (rowlen + crl)) : 64;
+  const States S = states_of(sp);
   // this level's subsets, listed in LDS
   uint32_t* subs = L->f.pad;
   const bool mine = lane < W && __popc((uint32_t)lane) == level;
@@ -996,7 +1001,15 @@ __device__ void item_merge(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* 
   uint64_t n_eval = 0;
   for (uint64_t vb = v0; vb < v1; vb += VPW) {
     const int nv = (int)((v1 - vb) < (uint64_t)VPW ? (v1 - vb) : (uint64_t)VPW);
-    stage_rows(sp, vl + vb, nv, rowlen, W, L->stage);
+    if (l2) {
+      for (int t = lane; t < nv * rowlen; t += 64) {
+        const int jj = t / rowlen, rem = t - jj * rowlen;
+        const int d = rem / k;
+        L->stage[t] = S.get(vl[vb + jj], 1u << (rem - d * k), (uint32_t)d);
+      }
+    } else {
+      stage_rows(sp, vl + vb, nv, rowlen, W, L->stage);
+    }
     uint32_t* cst = (uint32_t*)(L->stage + nv * rowlen);  // C rows of the staged vertices
     for (int t = lane; t < nv * ND * k; t += 64) {
       const int jj = t / (ND * k);
@@ -1030,8 +1043,12 @@ __device__ void item_merge(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* 
         }
         if (sat(mt + mr) > bst0) continue;
       }
+      // state (d, q) of the staged vertex, in either layout
+      auto st = [&](int d, uint32_t q) -> uint64_t {
+        return l2 ? row[d * k + (__ffs(q) - 1)] : row[d * W + q];
+      };
       const uint32_t low = p & (0u - p);
-      uint64_t bestpk = row[nd * W + p];
+      uint64_t bestpk = l2 ? kInfState : row[nd * W + p];
       for (uint32_t sub = (p - 1) & p; sub; sub = (sub - 1) & p) {
         if (!(sub & low)) continue;
         const uint32_t sub2 = p ^ sub;
@@ -1040,7 +1057,7 @@ __device__ void item_merge(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* 
         if (lim < 0) continue;
         const int dm = nd < lim ? nd : lim;
         for (int d2 = 0; d2 <= dm; d2++) {  // d1 = nd
-          const uint64_t a = row[nd * W + sub], b = row[d2 * W + sub2];
+          const uint64_t a = st(nd, sub), b = st(d2, sub2);
           const uint32_t c = sat((uint32_t)(a >> 32) + (uint32_t)(b >> 32));
           n_eval++;
           if (c >= kInf) continue;
@@ -1049,7 +1066,7 @@ __device__ void item_merge(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* 
         }
         const int dm1 = (nd - 1) < lim ? (nd - 1) : lim;
         for (int d1 = 0; d1 <= dm1; d1++) {  // d2 = nd, d1 < nd
-          const uint64_t a = row[d1 * W + sub], b = row[nd * W + sub2];
+          const uint64_t a = st(d1, sub), b = st(nd, sub2);
           const uint32_t c = sat((uint32_t)(a >> 32) + (uint32_t)(b >> 32));
           n_eval++;
           if (c >= kInf) continue;
@@ -1076,7 +1093,7 @@ __device__ void item_merge(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* 
       if (__popc(x) < level) {
         const int d2m = Dlim - nd;
         for (int d2 = 0; d2 <= d2m; d2++) {
-          const uint32_t u2 = sat(c + (uint32_t)(row[d2 * W + x] >> 32));
+          const uint32_t u2 = sat(c + (uint32_t)(st(d2, x) >> 32));
           ub = u2 < ub ? u2 : ub;
         }
       }
