@@ -13,6 +13,10 @@ throughput, latency percentiles, parity checks (plain vs batched costs, tree
 validation, the CPU oracle on a bounded sample), the roofline of the dominant
 step kernel and the CPU baseline ride along in the same object.
 
+Schedule: the batched engine's options (work-item size, grid, hipGraph
+replay, admitted slots) come from the autotuner's results file when it holds
+this graph and query shape (``--tune`` re-tunes on a sample batch first).
+
 Roofline: one extra batch (not in the timed steps) runs with every step's
 lean and heavy kernels bracketed by HIP events on the engine stream; the
 dominant kernel's average launch time and the algorithmic bytes its items
@@ -111,6 +115,10 @@ def main():
     ap.add_argument("--serve", type=int, default=512, help="queries of the serving leg (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=0.0,
                     help="offered queries/s of the serving leg (default: half the batched throughput)")
+    ap.add_argument("--tune", action="store_true",
+                    help="run the schedule autotuner on a sample batch before the batched leg")
+    ap.add_argument("--tune-db", default=None,
+                    help="autotuner results file (default: gpugst_amd/tuned.json)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -153,6 +161,18 @@ def main():
         log(f"plain: {args.plain} queries {plain_s:.3f}s -> {plain_qps:.1f} q/s "
             f"p50 {pct(plain_lat, 50):.2f} ms p99 {pct(plain_lat, 99):.2f} ms")
 
+    # ---------------- schedule of the batched engine: the autotuner's choice
+    from gpugst_amd import autotune
+    db = args.tune_db or autotune.DEFAULT_DB
+    if args.tune:
+        sample = list(queries[:args.batch])
+        schedule = autotune.tune(g, sample, D=args.D, path=db, log=log)["best"]
+    else:
+        schedule = autotune.apply(g, args.k, args.D, path=db)
+    if args.max_slots:
+        g.set_option("max_slots", args.max_slots)
+    log(f"schedule: {schedule if schedule else 'engine defaults (no tuned entry for this graph and shape)'}")
+
     # ---------------- batched: warmup + timed steps
     bres = []
     for w in range(args.warmup):
@@ -184,9 +204,12 @@ def main():
         a = (args.warmup + args.steps) * args.batch
         qs = queries[a:a + args.batch]
         g.set_option("reset_stats", 0)
+        use_graph = g.get_option("graph")
+        g.set_option("graph", 0)  # events bracket eager launches
         g.set_option("time_steps", 1)
         rr = g.solve_batch(list(qs), Ds[:len(qs)])
         g.set_option("time_steps", 0)
+        g.set_option("graph", use_graph)
         n_steps = max(1, g.get_option("batch_timed_steps"))
         lean_ms = g.get_option("batch_lean_ns") / 1e6 / n_steps
         heavy_ms = g.get_option("batch_heavy_ns") / 1e6 / n_steps
@@ -302,7 +325,8 @@ def main():
         "config": {"workload": f"rmat-1.1M k={args.k} {'unbounded' if args.D < 0 else f'D={args.D}'} "
                                f"batched, {args.batch} queries/step",
                    "V": V, "arcs": int(inst.col.shape[0]), "groups": int(inst.gptr.shape[0] - 1),
-                   "k": args.k, "D": args.D, "batch": args.batch, "scale": args.scale},
+                   "k": args.k, "D": args.D, "batch": args.batch, "scale": args.scale,
+                   "schedule": schedule},
         "plain_qps": round(plain_qps, 2) if plain_qps else None,
         "batched_vs_plain": round(batch_qps / plain_qps, 2) if plain_qps else None,
         "latency_ms": {"plain_p50": pct(plain_lat, 50), "plain_p99": pct(plain_lat, 99),

@@ -1,9 +1,12 @@
 """Schedule autotuner for the persistent engine.
 
 Tunables (all runtime options of the C ABI):
-* ``chunk``          -- arcs per frontier work item (load-balance granularity);
-* ``blocks_per_cu``  -- resident workgroups per CU of the persistent grid;
-* ``max_slots``      -- queries admitted concurrently (batch width).
+* ``chunk``          -- arcs per work item of a query's own frontier rounds
+                        (load-balance granularity);
+* ``blocks_per_cu``  -- workgroups per CU of the step kernels' grids (0: the
+                        occupancy maximum of each kernel);
+* ``graph``          -- replay chunks of engine steps from a captured hipGraph;
+* ``max_slots``      -- queries and group-distance jobs admitted at once.
 
 ``tune()`` times a sample of queries through ``solve_batch`` for every point of
 a small grid and returns the fastest; results are kept per (graph, k, D)
@@ -22,10 +25,12 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_DB = os.path.join(HERE, "tuned.json")
 
 GRID = {
-    "chunk": (256, 512, 1024, 2048),
-    "blocks_per_cu": (1, 2),
-    "max_slots": (64, 128, 256, 512),
+    "chunk": (256, 512, 1024),
+    "blocks_per_cu": (0, 2, 3),
+    "graph": (0, 1),
+    "max_slots": (256, 512),
 }
+KEYS = ("chunk", "blocks_per_cu", "graph", "max_slots")
 
 
 def bucket(V: int, arcs: int, k: int, D: int) -> str:
@@ -46,7 +51,7 @@ def apply(graph, k: int, D: int = -1, path: str = DEFAULT_DB) -> Optional[dict]:
     b = db.get(bucket(graph.V, graph.n_arcs, k, D))
     if not b:
         return None
-    for key in ("chunk", "blocks_per_cu", "max_slots"):
+    for key in KEYS:
         if key in b["best"]:
             graph.set_option(key, int(b["best"][key]))
     return b["best"]
@@ -61,8 +66,6 @@ def tune(graph, queries: Sequence, D: int = -1, grid: Optional[dict] = None,
     ref_costs = None
     for combo in itertools.product(*(grid[k] for k in keys)):
         opts = dict(zip(keys, combo))
-        if opts["max_slots"] > len(queries) * 2:
-            continue
         for key, val in opts.items():
             graph.set_option(key, val)
         graph.solve_batch(list(queries[: min(len(queries), 8)]), [D] * min(len(queries), 8))

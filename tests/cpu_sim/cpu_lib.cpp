@@ -7,6 +7,7 @@
 #define GST_CPU_SIM_HOST 1
 #include "gst_device.hip"
 
+#include <functional>
 #include <thread>
 #include <vector>
 
@@ -20,7 +21,15 @@ thread_local ThreadIdx grid_dim = {1};
 thread_local uint64_t collectives = 0;
 }  // namespace gsim
 
-extern "C" hipError_t gst_launch_plan(gst::Slot* slots, gst::Ctrl* ctrl, uint64_t V, hipStream_t) {
+// stream capture (mock runtime): record the launch instead of running it
+bool gsim_capturing();
+void gsim_record(std::function<void()> f);
+
+extern "C" hipError_t gst_launch_plan(gst::Slot* slots, gst::Ctrl* ctrl, uint64_t V, hipStream_t s) {
+  if (gsim_capturing()) {
+    gsim_record([=]() { gst_launch_plan(slots, ctrl, V, s); });
+    return hipSuccess;
+  }
   if (ctrl->exit_flag) return hipSuccess;
   static uint32_t sh_items[gst::kNumCls][gst::kMaxSlots];
   static uint32_t sh_scan[gst::kNumCls][2][gst::kBlock];
@@ -62,7 +71,12 @@ static void run_class(const gst::GraphDev* g, gst::Slot* slots, gst::Ctrl* ctrl,
 }
 
 extern "C" hipError_t gst_launch_step(int cls, const gst::GraphDev* g, gst::Slot* slots, gst::Ctrl* ctrl,
-                                      int grid, hipStream_t) {
+                                      int grid, hipStream_t s) {
+  if (gsim_capturing()) {
+    const gst::GraphDev gd = *g;  // kernel arguments are copied at capture
+    gsim_record([=]() { gst_launch_step(cls, &gd, slots, ctrl, grid, s); });
+    return hipSuccess;
+  }
   if (cls == gst::kClsLean)
     run_class<gst::kClsLean, gst::FrontLDS>(g, slots, ctrl, grid);
   else

@@ -5,6 +5,8 @@
 #include <chrono>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
+#include <vector>
 
 struct ihipEvent {
   double t;
@@ -39,11 +41,37 @@ hipError_t hipEventCreate(hipEvent_t* e) { *e = new ihipEvent{0}; return hipSucc
 hipError_t hipEventDestroy(hipEvent_t e) { delete e; return hipSuccess; }
 hipError_t hipEventRecord(hipEvent_t e, hipStream_t) { e->t = now_ms(); return hipSuccess; }
 hipError_t hipEventElapsedTime(float* ms, hipEvent_t a, hipEvent_t b) { *ms = (float)(b->t - a->t); return hipSuccess; }
-hipError_t hipStreamBeginCapture(hipStream_t, int) { return hipErrorNotSupported; }
-hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t*) { return hipErrorNotSupported; }
-hipError_t hipGraphInstantiate(hipGraphExec_t*, hipGraph_t, void*, char*, size_t) { return hipErrorNotSupported; }
-hipError_t hipGraphLaunch(hipGraphExec_t, hipStream_t) { return hipErrorNotSupported; }
-hipError_t hipGraphDestroy(hipGraph_t) { return hipSuccess; }
-hipError_t hipGraphExecDestroy(hipGraphExec_t) { return hipSuccess; }
+// Stream capture: launches made while capturing are recorded (with their
+// arguments copied) instead of run; a graph launch runs them in order.
+struct ihipGraph {
+  std::vector<std::function<void()>> ops;
+};
+struct ihipGraphExec {
+  std::vector<std::function<void()>> ops;
+};
+static thread_local ihipGraph* g_capture = nullptr;
+bool gsim_capturing() { return g_capture != nullptr; }
+void gsim_record(std::function<void()> f) { g_capture->ops.push_back(std::move(f)); }
+hipError_t hipStreamBeginCapture(hipStream_t, int) {
+  if (g_capture) return hipErrorNotSupported;
+  g_capture = new ihipGraph;
+  return hipSuccess;
+}
+hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t* g) {
+  if (!g_capture) return hipErrorNotSupported;
+  *g = g_capture;
+  g_capture = nullptr;
+  return hipSuccess;
+}
+hipError_t hipGraphInstantiate(hipGraphExec_t* e, hipGraph_t g, void*, char*, size_t) {
+  *e = new ihipGraphExec{g->ops};
+  return hipSuccess;
+}
+hipError_t hipGraphLaunch(hipGraphExec_t e, hipStream_t) {
+  for (auto& f : e->ops) f();
+  return hipSuccess;
+}
+hipError_t hipGraphDestroy(hipGraph_t g) { delete g; return hipSuccess; }
+hipError_t hipGraphExecDestroy(hipGraphExec_t e) { delete e; return hipSuccess; }
 hipError_t hipGetLastError() { return hipSuccess; }
 const char* hipGetErrorString(hipError_t e) { return e ? "mock runtime error" : "no error"; }

@@ -7,6 +7,7 @@ at tests/cpu_sim/_build/libgst_cpu.so.  Prints one JSON line of results.
            validated
   jobs   : one batch of random k=5 queries (duplicates allowed) over several
            group-distance jobs, against the oracle and against group_pass=0
+  graph  : engine steps replayed from a captured graph against eager launches
   branching: a vertex that holds states of two or more groups only through
            grows, where the optimum needs a merge
   hop    : one batch of random k=3..5 queries with hop bounds 2..4 (depth-aware
@@ -153,12 +154,29 @@ def check_branching():
     return {"cost": r.cost, "expect": exp, "ok": bool(ok)}
 
 
+def check_graph():
+    """Batches with engine steps replayed from a captured graph (graph=1, the
+    mock runtime records the launches) against eager launches and the oracle."""
+    inst = data.make_config(2, scale=0.002, nq=4)
+    g = Graph(inst.rowptr, inst.col, inst.wt)
+    g.set_groups(inst.gptr, inst.gmem)
+    qs = list(inst.queries)
+    Ds = [3 if i % 4 == 1 else -1 for i in range(len(qs))]
+    g.set_option("graph", 0)
+    eager = [r.cost for r in g.solve_batch(qs, Ds)]
+    g.set_option("graph", 1)
+    rep = [[r.cost for r in g.solve_batch(qs, Ds)] for _ in range(2)]
+    g.close()
+    exp = [O.dpbf(inst.rowptr, inst.col, inst.wt, inst.query_groups(i), D=Ds[i])[0] for i in range(len(qs))]
+    return {"expect": exp, "eager": eager, "graph": rep}
+
+
 def main():
     stride = int(sys.argv[1]) if len(sys.argv) > 1 else 3
     nq = int(sys.argv[2]) if len(sys.argv) > 2 else 60
     nh = int(sys.argv[3]) if len(sys.argv) > 3 else 24
     out = {"tiny": check_tiny(stride), "jobs": check_jobs(150, nq, 7), "hop": check_hop(150, nh, 11),
-           "branching": check_branching()}
+           "branching": check_branching(), "graph": check_graph()}
     print(json.dumps(out), flush=True)
 
 

@@ -56,3 +56,9 @@ def test_hop_queries_on_cpu_build(cpu_results):
 def test_grown_branch_vertex_on_cpu_build(cpu_results):
     b = cpu_results["branching"]
     assert b["ok"] and b["cost"] == b["expect"] == 36, b
+
+
+def test_graph_replay_on_cpu_build(cpu_results):
+    r = cpu_results["graph"]
+    assert r["eager"] == r["expect"]
+    assert all(x == r["expect"] for x in r["graph"]), r
