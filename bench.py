@@ -41,7 +41,7 @@ def pct(x, q):
     return float(np.percentile(np.asarray(x, dtype=np.float64), q)) if len(x) else None
 
 
-def lean_bytes(g, res_list, V, k):
+def lean_bytes(g, res_list, V, ks):
     """Algorithmic bytes of the lean kernel's items for these queries (one
     group-distance job per up to 256 labels):
       job FILL        4 B per (vertex, label) + 24 B per vertex (3 masks)
@@ -51,7 +51,7 @@ def lean_bytes(g, res_list, V, k):
                       improved chunk 16 B (write back)
       job SCATTER     4 B per (vertex, label) read + 4 B per (vertex, label) written
       query FILL      24 B per vertex (3 masks) + 1/8 B live flags"""
-    labels = k * len(res_list)
+    labels = int(sum(ks))
     jobs = max(1, g.get_option("batch_jobs"))
     lp = labels + 3 * jobs  # rows padded to whole 4-label chunks
     ent = g.get_option("batch_job_entries")
@@ -65,16 +65,16 @@ def lean_bytes(g, res_list, V, k):
     return b
 
 
-def heavy_bytes(res_list, V, k, ND=1):
+def heavy_bytes(res_list, V, ks, ND=1):
     """Algorithmic bytes of the heavy kernel's items: per query the live test
     (C row, 4 B per vertex and group), T segments of live vertices (8 B per
     state: initialised once, read by every merge level and FINAL), and the
     level >= 2 grow rounds (24 B per frontier entry, 8 B per arc, 16 B per
     relaxation candidate, 8 B per improvement) -- the counters of a query fed
     by a job hold only these rounds."""
-    W = 1 << k
     total = 0
-    for r in res_list:
+    for r, k in zip(res_list, ks):
+        W = 1 << k
         c = r.counters
         nl = c.get("live_vertices", 0)
         total += 4 * V * k + nl * W * ND * 8 * (1 + max(k - 1, 1))
@@ -106,6 +106,8 @@ def main():
     ap.add_argument("--batch", type=int, default=256, help="queries per step")
     ap.add_argument("--plain", type=int, default=32, help="queries answered one at a time")
     ap.add_argument("--k", type=int, default=5)
+    ap.add_argument("--kmix", action="store_true",
+                    help="k drawn uniformly from {3,4,5,6} per query (the mixed-k batched workload)")
     ap.add_argument("--D", type=int, default=-1)
     ap.add_argument("--scale", type=float, default=1.0, help="shrink the graph (tests only)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU oracle sample budget")
@@ -135,9 +137,16 @@ def main():
     n_roof = 0 if args.no_roofline else args.batch
     n_batched = (args.warmup + args.steps) * args.batch + n_roof
     nq = max(n_batched, args.plain)
-    queries = data.make_queries(nq, args.k, inst.meta["eligible"], 3000 + 4 + 10 * args.k + 1)
+    if args.kmix:
+        # the mixed-k workload's query stream: k uniform in {3..6}, one seeded draw per query
+        ks = data.uniform_ints(nq, 3, 6, 4004)
+        queries = [data.make_queries(1, int(ks[i]), inst.meta["eligible"], 4004 * 1_000_003 + i)[0]
+                   for i in range(nq)]
+    else:
+        queries = data.make_queries(nq, args.k, inst.meta["eligible"], 3000 + 4 + 10 * args.k + 1)
+    kdesc = "k in {3..6}" if args.kmix else f"k={args.k}"
     log(f"graph V={V} arcs={inst.col.shape[0]} groups={inst.gptr.shape[0]-1} "
-        f"queries={nq} k={args.k} D={args.D} gen {time.time()-t0:.1f}s")
+        f"queries={nq} {kdesc} D={args.D} gen {time.time()-t0:.1f}s")
 
     g = Graph(inst.rowptr, inst.col, inst.wt)
     g.set_groups(inst.gptr, inst.gmem)
@@ -213,12 +222,14 @@ def main():
         n_steps = max(1, g.get_option("batch_timed_steps"))
         lean_ms = g.get_option("batch_lean_ns") / 1e6 / n_steps
         heavy_ms = g.get_option("batch_heavy_ns") / 1e6 / n_steps
-        lb = lean_bytes(g, rr, V, args.k) if args.D < 0 else None
-        hb = heavy_bytes(rr, V, args.k, 1 if args.D < 0 else args.D + 1)
+        rk = [len(q) for q in qs]
+        lb = lean_bytes(g, rr, V, rk) if args.D < 0 else None
+        hb = heavy_bytes(rr, V, rk, 1 if args.D < 0 else args.D + 1)
         dom = "gst_lean_kernel" if lean_ms >= heavy_ms else "gst_heavy_kernel"
         ms, byts = (lean_ms, lb) if dom == "gst_lean_kernel" else (heavy_ms, hb)
         achieved = byts / n_steps / (ms * 1e-3) / 1e9 if (byts and ms > 0) else 0.0
-        traffic = profiled_traffic(dom) if (args.D < 0 and args.batch == 256 and args.k == 5) else None
+        traffic = profiled_traffic(dom) if (args.D < 0 and args.batch == 256 and args.k == 5
+                                            and not args.kmix) else None
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
                 "algorithmic_bytes_per_launch": int(byts / n_steps) if byts else None,
@@ -303,7 +314,7 @@ def main():
         algo = ("exact pruned best-first DPBF (orc_dpbf_pruned, the exact-mode PrunedDP++ "
                 "restatement)" if args.D < 0 else "exact hop-bounded DP (orc_dpbf, HOP_cpu restatement)")
         cpu = {"value": n / cs, "unit": "queries/s", "cores": 1, "kind": "port",
-               "sample": f"first {n} of the same k={args.k} queries, {algo} in oracle/gst_oracle.c, "
+               "sample": f"first {n} of the same {kdesc} queries, {algo} in oracle/gst_oracle.c, "
                          f"single thread, {cs:.1f} s"}
         log(f"cpu oracle: {n} queries in {cs:.1f}s, mismatches {oracle_mism}")
 
@@ -322,10 +333,10 @@ def main():
         "vs_baseline": None,
         "dtype": "int32",
         "data": "synthetic",
-        "config": {"workload": f"rmat-1.1M k={args.k} {'unbounded' if args.D < 0 else f'D={args.D}'} "
+        "config": {"workload": f"rmat-1.1M {kdesc} {'unbounded' if args.D < 0 else f'D={args.D}'} "
                                f"batched, {args.batch} queries/step",
                    "V": V, "arcs": int(inst.col.shape[0]), "groups": int(inst.gptr.shape[0] - 1),
-                   "k": args.k, "D": args.D, "batch": args.batch, "scale": args.scale,
+                   "k": "3..6" if args.kmix else args.k, "D": args.D, "batch": args.batch, "scale": args.scale,
                    "schedule": schedule},
         "plain_qps": round(plain_qps, 2) if plain_qps else None,
         "batched_vs_plain": round(batch_qps / plain_qps, 2) if plain_qps else None,
