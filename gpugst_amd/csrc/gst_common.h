@@ -118,7 +118,7 @@ struct alignas(64) Slot {
   uint32_t nvsrc;       // entries of vsrc
   uint32_t pad3;
   uint32_t* live;       // V: vertices that can lie on a tree cheaper than the bound
-  uint8_t* lvf;         // V: live flag per vertex
+  uint32_t* lvb;        // V bits: live flag per vertex
   uint32_t* hold;       // V bits: the vertex holds a finite state with |p| >= 2
   uint32_t* act;        // V: vertices with their hold bit set, in order of setting
   uint64_t* mask;       // 3 * V (triple-buffered dirty sub-masks)

@@ -355,7 +355,7 @@ __device__ __forceinline__ void mark_hold(bool flag, uint32_t v, Slot* sp) {
 // ---------------------------------------------------------------- items ----
 
 // u64 words that FILL initialises: job [distances][masks]; query [S1][masks]
-// [live flags][hold bits] (S1 only when the query computes its own level 1;
+// [live bits][hold bits] (S1 only when the query computes its own level 1;
 // its C table needs no fill, DIST writes every entry)
 __device__ __forceinline__ void fill_regions(const Slot* sp, uint64_t V, uint64_t* nA, uint64_t* nM,
                                              uint64_t* nF) {
@@ -368,7 +368,7 @@ __device__ __forceinline__ void fill_regions(const Slot* sp, uint64_t V, uint64_
   const bool own = sp->S1 != nullptr;
   *nA = own ? V * (uint64_t)sp->ND * (uint64_t)sp->k : 0;
   *nM = 3 * V;
-  *nF = (V + 7) / 8 + (V + 63) / 64;  // live flags, then hold bits
+  *nF = 2 * ((V + 63) / 64);  // live bits, then hold bits
 }
 
 __device__ void item_fill(Slot* sp, uint32_t idx, uint64_t V) {
@@ -380,7 +380,7 @@ __device__ void item_fill(Slot* sp, uint32_t idx, uint64_t V) {
   const bool job = sp->kind == KIND_JOB;
   uint64_t* A = job ? (uint64_t*)sp->dist : sp->S1;
   uint64_t* M = sp->mask;
-  uint64_t* F64 = (uint64_t*)sp->lvf;
+  uint64_t* F64 = (uint64_t*)sp->lvb;
   const uint64_t cinf = ((uint64_t)kInf << 32) | kInf;
   const uint64_t aval = job ? cinf : kInfState;
   for (uint64_t t = base + lane_id(); t < end; t += 64) {
@@ -504,7 +504,7 @@ __device__ void item_round(const GraphDev& g, Slot* sp, uint32_t chunk, FrontLDS
         uint32_t revarc = 0xFFFFFFFFu;
         uint64_t improved = 0;
         // above level 1 only live targets can hold useful states
-        uint64_t mm = (level >= 2 && !sp->lvf[u]) ? 0ull : m;
+        uint64_t mm = (level >= 2 && !((sp->lvb[u >> 5] >> (u & 31u)) & 1u)) ? 0ull : m;
         while (mm) {
           const uint32_t p = (uint32_t)(__ffsll((long long)mm) - 1);
           mm &= mm - 1;
@@ -937,7 +937,7 @@ __device__ void item_live(Slot* sp, uint32_t tile, uint64_t V, WaveLDS* L) {
       v = sp->vsrc ? (uint64_t)sp->vsrc[idx] : idx;
       // lower bound on any tree through v that covers every group
       lv = lb_le(crow(sp, v, sp->D), k, (uint32_t)W - 1, (g_cu32*)sp->R, bst < kInf ? bst : kInf - 1);
-      if (lv) sp->lvf[v] = 1;
+      if (lv) atomicOr(&sp->lvb[v >> 5], 1u << (uint32_t)(v & 31u));
     }
     const uint64_t ball = __ballot(lv);
     if (ball == 0) continue;

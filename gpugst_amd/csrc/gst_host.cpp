@@ -168,7 +168,7 @@ struct QueryReq {
 };
 
 struct Layout {
-  size_t T, S1, live, lvf, act, mask, C, list, loff, istart, R, edges, rq, total;
+  size_t T, S1, live, lvb, act, mask, C, list, loff, istart, R, edges, rq, total;
   int edge_cap;
   uint32_t istride;
 };
@@ -197,7 +197,7 @@ Layout layout_for(const gst_graph* g, int k, int D, bool own) {
   L.T = off;    off += align256(V * ND * W * 8);
   L.S1 = off;   off += own ? align256(V * ND * k * 8) : 0;
   L.live = off; off += align256(V * 4);
-  L.lvf = off;  off += align256(((V + 7) / 8) * 8 + ((V + 63) / 64) * 8);  // flags, then hold bits
+  L.lvb = off;  off += align256(2 * ((V + 63) / 64) * 8);  // live bits, then hold bits
   L.act = off;  off += align256(V * 4);
   L.mask = off; off += align256(3 * V * 8);
   L.C = off;    off += (own || D >= 0) ? align256(V * ND * k * 4) : 0;
@@ -427,8 +427,8 @@ class Engine {
     d.T = (uint64_t*)(base + L.T);
     d.S1 = own ? (uint64_t*)(base + L.S1) : nullptr;
     d.live = (uint32_t*)(base + L.live);
-    d.lvf = (uint8_t*)(base + L.lvf);
-    d.hold = (uint32_t*)(base + L.lvf + ((size_t)g_->V + 7) / 8 * 8);
+    d.lvb = (uint32_t*)(base + L.lvb);
+    d.hold = (uint32_t*)(base + L.lvb + ((size_t)g_->V + 63) / 64 * 8);
     d.act = (uint32_t*)(base + L.act);
     d.mask = (uint64_t*)(base + L.mask);
     const bool cown = own || q.D >= 0;
