@@ -137,6 +137,29 @@ def test_vertex_reached_only_by_grows():
     g.close()
 
 
+@pytest.mark.parametrize("D", [-1, 2, 3])
+def test_batch_jobs_vs_oracle(D):
+    # one batch of 16 queries of one bound: they share a group-distance job
+    # (for D >= 0 a hop job with one row per height); every cost against the
+    # oracle, every tree validated
+    rng = random.Random(77 + D)
+    V = 2500
+    u, v = data.chung_lu_edges(V, 4 * V, 2.2, 900 + D)
+    rp, col, wt, _ = data.weighted_csr(V, u, v, 1, 50, 901 + D)
+    G = 60
+    gptr, gmem = data.make_groups(V, G, 4, 40, 902 + D)
+    g = Graph(rp, col, wt)
+    g.set_groups(gptr, gmem)
+    qs = [rng.sample(range(G), rng.randint(3, 5)) for _ in range(16)]
+    _say("batch of 16, D", D)
+    rs = g.solve_batch(qs, [D] * len(qs))
+    assert g.get_option("batch_jobs") >= 1
+    for q, r in zip(qs, rs):
+        groups = [gmem[gptr[x]:gptr[x + 1]] for x in q]
+        _check(rp, col, wt, groups, r, O.dpbf(rp, col, wt, groups, D=D)[0], D=D)
+    g.close()
+
+
 def test_batch_matches_plain():
     inst = data.make_config(2, scale=0.02, nq=6)
     g = Graph(inst.rowptr, inst.col, inst.wt)
