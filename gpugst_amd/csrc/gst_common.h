@@ -47,8 +47,8 @@ constexpr int kStageWords = 1152;       // per-wave LDS staging (u64): 64*ND row
 // writes coalesced 16-byte chunks shared by every query of the job.
 constexpr int kJobChunkLabels = 4;     // labels per chunk of a job row (one dirty bit)
 constexpr int kJobLabels = 64 * kJobChunkLabels;  // labels per job: 64 chunks
-constexpr uint32_t kJobArcs = 256;      // arcs per job ROUND item
-constexpr int kJobChunksPerLane = 4;    // chunks one lane relaxes per arc
+constexpr uint32_t kJobArcs = 512;      // arcs per job ROUND item (swept: 128 / 256 / 512 / 1024)
+constexpr int kJobChunksPerLane = 8;    // chunks one lane relaxes per arc (swept: 2 / 4 / 8)
 constexpr uint32_t kStarTV = 64;        // vertices per job STAR item
 
 // work classes: each has its own kernel, so the arc-parallel items do not run

@@ -227,7 +227,7 @@ JobLayout job_layout(const gst_graph* g, int L, int nmem, int D) {
   const int unit = gst::kJobChunkLabels > 2 ? gst::kJobChunkLabels : 2;
   J.Lp = (L + unit - 1) / unit * unit;
   // lanes per arc: the smallest power of two that gives each lane at most
-  // kJobChunksPerLane chunks (64 chunks: 16 lanes, 4 arcs per wave iteration;
+  // kJobChunksPerLane chunks (64 chunks: 8 lanes, 8 arcs per wave iteration;
   // one or two chunks: 1 lane, 64 arcs)
   const int nch = J.Lp / gst::kJobChunkLabels;
   J.G = 1;
