@@ -106,7 +106,8 @@ struct alignas(64) Slot {
   uint32_t istride;     // entries per buffer of istart
   int32_t L, Lp, G;     // job: labels, padded row length, lanes per arc (power of two)
   int32_t nmem;         // job: member queries
-  uint32_t cstride;     // words per vertex row of C (k, or the job's Lp)
+  uint32_t cstride;     // words per vertex of C (ND * k for an own table, the job's row stride)
+  uint32_t chs;         // words between the heights of one vertex in C (k, or the job's Lp)
   uint64_t nT;          // V * ND * W
   uint64_t* T;          // states with |p| >= 2; only live vertices' segments are touched
   uint64_t* S1;         // V * ND * k singleton states T(v, {i}, d); null when C comes from a job
@@ -122,9 +123,9 @@ struct alignas(64) Slot {
   uint32_t* hold;       // V bits: the vertex holds a finite state with |p| >= 2
   uint32_t* act;        // V: vertices with their hold bit set, in order of setting
   uint64_t* mask;       // 3 * V (triple-buffered dirty sub-masks)
-  uint32_t* C;          // group->vertex distances c_i(v) = C[v * cstride + i]: the query's own
-                        // V * ND * k table (row h: within h hops), or, for an unbounded
-                        // job-fed query, its k columns of its job's distance rows
+  uint32_t* C;          // group->vertex distances within h hops, C[v * cstride + h * chs + i]:
+                        // the query's own V * ND * k table, or its k columns of its job's
+                        // rows (a hop job turns them cumulative at STAR)
   uint32_t* J;          // job-fed query: its first label in the job's rows (null otherwise);
                         // singleton T(v, {i}, d) costs J[v * jvs + d * jds + i]
   uint32_t jvs, jds;    // job-fed query: row stride per vertex and per height

@@ -161,10 +161,11 @@ __device__ __forceinline__ States states_of(const Slot* sp) {
 }
 
 // Row of vertex v in the distance table C.  A hop query's table holds D + 1
-// cumulative rows per vertex, row h the group distances within h hops; other
-// queries have one row (the job's row for a job-fed query).
+// cumulative rows per vertex, row h the group distances within h hops (its own
+// table, or its job's rows made cumulative at STAR); an unbounded query has
+// one row (the job's row for a job-fed query).
 __device__ __forceinline__ g_cu32* crow(const Slot* sp, uint64_t v, int h) {
-  return (g_cu32*)sp->C + v * sp->cstride + (sp->D >= 0 ? (uint64_t)h * (uint64_t)sp->k : 0);
+  return (g_cu32*)sp->C + v * sp->cstride + (sp->D >= 0 ? (uint64_t)h * (uint64_t)sp->chs : 0);
 }
 
 // Lower bound on the cost of the cheapest tree containing u that covers the
@@ -725,6 +726,37 @@ __device__ void item_jstar(Slot* slots, Slot* sp, uint32_t tile, uint64_t V) {
   const uint32_t lab0 = q0 ? (uint32_t)q0->lab : 0u, lab1 = q1 ? (uint32_t)q1->lab : 0u;
   uint32_t star0 = kInf, star1 = kInf;
   uint64_t reached = 0;
+  if (ND > 1) {
+    // a hop job's rows become cumulative: height d holds the distance within d
+    // hops (walks of at most d arcs), which is what the members read as C and
+    // as their singleton states
+    const int nch = (int)(Lp / kJobChunkLabels);
+    for (uint64_t v = v0; v < v1; v++) {
+      g_u32* row = (g_u32*)(sp->dist + v * rs);
+      for (int c = lane; c < nch; c += 64) {
+        jchunk run = *(g_cchunk*)(row + (uint64_t)c * kJobChunkLabels);
+        for (int d = 1; d < ND; d++) {
+          g_u32* w = row + (uint64_t)d * Lp + (uint64_t)c * kJobChunkLabels;
+          jchunk x = *(g_cchunk*)w;
+          bool lower = false;
+#pragma unroll
+          for (int q = 0; q < kJobChunkLabels; q++) {
+            if (run[q] < x[q]) {
+              x[q] = run[q];
+              lower = true;
+            }
+          }
+          if (lower) {
+#pragma unroll
+            for (int q = 0; q < kJobChunkLabels; q++) w[q] = x[q];
+          }
+          run = x;
+        }
+      }
+    }
+    __threadfence();  // the rows just written are read below by other lanes
+    wave_sync();
+  }
   for (uint64_t v = v0; v < v1; v++) {
     const uint32_t* row = sp->dist + v * rs;
     bool fin = false;
@@ -734,14 +766,8 @@ __device__ void item_jstar(Slot* slots, Slot* sp, uint32_t tile, uint64_t V) {
       if (km == 0) continue;
       uint32_t s = 0, sr = 0;
       for (int i = 0; i < km; i++) {
-        uint32_t ch = kInf, cr = kInf;
-        for (int d = 0; d < ND; d++) {
-          const uint32_t c = row[(uint64_t)d * Lp + lab + i];
-          cr = c < cr ? c : cr;
-          if (d == hs) ch = cr;
-        }
-        s = sat(s + ch);
-        sr = sat(sr + cr);
+        s = sat(s + row[(uint64_t)hs * Lp + lab + i]);
+        sr = sat(sr + row[(uint64_t)(ND - 1) * Lp + lab + i]);
       }
       if (m) star1 = s < star1 ? s : star1;
       else star0 = s < star0 ? s : star0;
@@ -770,8 +796,8 @@ __device__ void item_jstar(Slot* slots, Slot* sp, uint32_t tile, uint64_t V) {
 // The query's own C rows are as long as its S1 rows (cstride == ND * k), so a
 // tile of vertices is one contiguous run of both: lanes walk it element by
 // element (vertex, height, group), coalesced, each taking the minimum over the
-// heights up to its own.  A job-fed hop query reads its singleton costs from
-// the job's rows instead (its star bound came from the job's STAR).
+// heights up to its own.  Job-fed queries have no DIST tiles: their C is the
+// job's rows.
 __device__ void item_dist(const GraphDev& g, Slot* sp, uint32_t tile) {
   const int lane = lane_id();
   const uint64_t V = (uint64_t)g.V;
@@ -782,32 +808,17 @@ __device__ void item_dist(const GraphDev& g, Slot* sp, uint32_t tile) {
   const uint32_t rl = (uint32_t)(ND * k);
   uint32_t* Ct = sp->C + v0 * rl;
   const uint32_t nE = (uint32_t)(v1 - v0) * rl;
-  if (sp->S1) {
-    const uint64_t* S1t = sp->S1 + v0 * rl;
-    for (uint32_t e = (uint32_t)lane; e < nE; e += 64) {
-      const uint32_t vr = e / rl, rem = e - vr * rl;
-      const uint32_t d = rem / (uint32_t)k, i = rem - d * (uint32_t)k;
-      const uint64_t* col = S1t + vr * rl + i;
-      uint32_t c = kInf;
-      for (uint32_t dd = 0; dd <= d; dd++) {
-        const uint32_t t = (uint32_t)(col[dd * (uint32_t)k] >> 32);
-        c = t < c ? t : c;
-      }
-      Ct[e] = c;  // within d hops
+  const uint64_t* S1t = sp->S1 + v0 * rl;
+  for (uint32_t e = (uint32_t)lane; e < nE; e += 64) {
+    const uint32_t vr = e / rl, rem = e - vr * rl;
+    const uint32_t d = rem / (uint32_t)k, i = rem - d * (uint32_t)k;
+    const uint64_t* col = S1t + vr * rl + i;
+    uint32_t c = kInf;
+    for (uint32_t dd = 0; dd <= d; dd++) {
+      const uint32_t t = (uint32_t)(col[dd * (uint32_t)k] >> 32);
+      c = t < c ? t : c;
     }
-  } else {
-    const States S = states_of(sp);
-    for (uint32_t e = (uint32_t)lane; e < nE; e += 64) {
-      const uint32_t vr = e / rl, rem = e - vr * rl;
-      const uint32_t d = rem / (uint32_t)k, i = rem - d * (uint32_t)k;
-      uint32_t c = kInf;
-      for (uint32_t dd = 0; dd <= d; dd++) {
-        const uint32_t t = S.jcost(v0 + vr, i, dd);
-        c = t < c ? t : c;
-      }
-      Ct[e] = c;
-    }
-    return;
+    Ct[e] = c;  // within d hops
   }
   // star upper bound; the arms of a hop star are pairwise within D hops when
   // each has at most floor(D/2)
@@ -1012,8 +1023,9 @@ __device__ void item_merge(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* 
     }
     uint32_t* cst = (uint32_t*)(L->stage + nv * rowlen);  // C rows of the staged vertices
     for (int t = lane; t < nv * ND * k; t += 64) {
-      const int jj = t / (ND * k);
-      cst[t] = crow(sp, vl[vb + jj], 0)[t - jj * ND * k];
+      const int jj = t / (ND * k), rem = t - jj * ND * k;
+      const int d = rem / k;
+      cst[t] = crow(sp, vl[vb + jj], d)[rem - d * k];
     }
     if (lane < nv) {
       L->f.wv[lane] = vl[vb + lane];
@@ -1152,8 +1164,9 @@ __device__ void item_final(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* 
     stage_rows(sp, vl + vb, nv, rowlen, W, L->stage);
     uint32_t* cst = (uint32_t*)(L->stage + nv * rowlen);
     for (int t = lane; t < nv * ND * k; t += 64) {
-      const int jj = t / (ND * k);
-      cst[t] = crow(sp, vl[vb + jj], 0)[t - jj * ND * k];
+      const int jj = t / (ND * k), rem = t - jj * ND * k;
+      const int d = rem / k;
+      cst[t] = crow(sp, vl[vb + jj], d)[rem - d * k];
     }
     wave_sync();
     const bool vact = j < nv;
@@ -1489,7 +1502,7 @@ __device__ uint32_t plan_step(Slot* s, uint64_t V) {
       if (!job) cls = kClsHeavy;  // the per-query grow carries the lower-bound code
       break;
     case PH_STAR: work = ceil_div(V, kStarTV); break;
-    case PH_DIST: work = ((s->S1 || s->D >= 0) ? ceil_div(V, kDistTV) : 0u) + 1u; cls = kClsHeavy; break;
+    case PH_DIST: work = (s->S1 ? ceil_div(V, kDistTV) : 0u) + 1u; cls = kClsHeavy; break;
     case PH_LIVE: work = ceil_div(s->vsrc ? (uint64_t)s->nvsrc : V, kDistTV); cls = kClsHeavy; break;
     case PH_MERGE: work = ceil_div(s->level >= 3 ? s->nact : s->nlive, kMergeTV); cls = kClsHeavy; break;
     case PH_FINAL: work = ceil_div(s->k >= 3 ? s->nact : s->nlive, kMergeTV); cls = kClsHeavy; break;
@@ -1681,7 +1694,10 @@ __device__ void step_body(const GraphDev& g, Slot* slots, Ctrl* ctrl, LDS* L) {
 // One engine step of each class: every wave pulls items until the class's
 // plan is exhausted.  There is no waiting between blocks; the step ends when
 // the kernels end, and the next plan kernel sees everything they wrote.
-__global__ __launch_bounds__(kBlock) void gst_lean_kernel(GraphDev g, Slot* slots, Ctrl* ctrl) {
+// Five waves per SIMD (96 VGPRs) without spills: held there so that code
+// added to rare items cannot lower the job rounds' occupancy.
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) void gst_lean_kernel(GraphDev g, Slot* slots,
+                                                                                              Ctrl* ctrl) {
   __shared__ FrontLDS lds[kWaves];
   step_body<kClsLean>(g, slots, ctrl, &lds[threadIdx.x >> 6]);
 }

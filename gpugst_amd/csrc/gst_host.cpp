@@ -182,9 +182,8 @@ uint32_t istride_for(const gst_graph* g, int64_t chunk) {
 // k = 1 is answered at SEED).  A job serves queries of one hop bound.
 bool uses_job(const gst_graph* g, int k, int D) { return g->opt.group_pass != 0 && k >= 2; }
 
-// Device memory of one query; `own` = it computes its own level 1 (S1 table).
-// Every query with its own level 1 or a hop bound has its own C table (rows
-// within 0..D hops); an unbounded job-fed query reads C from the job's rows.
+// Device memory of one query; `own` = it computes its own level 1 (S1 and C
+// tables); a job-fed query reads both from its job's rows.
 Layout layout_for(const gst_graph* g, int k, int D, bool own) {
   const size_t V = (size_t)g->V;
   const size_t W = size_t(1) << k;
@@ -200,7 +199,7 @@ Layout layout_for(const gst_graph* g, int k, int D, bool own) {
   L.lvb = off;  off += align256(2 * ((V + 63) / 64) * 8);  // live bits, then hold bits
   L.act = off;  off += align256(V * 4);
   L.mask = off; off += align256(3 * V * 8);
-  L.C = off;    off += (own || D >= 0) ? align256(V * ND * k * 4) : 0;
+  L.C = off;    off += own ? align256(V * ND * k * 4) : 0;
   L.list = off; off += align256(3 * V * 4);
   L.loff = off; off += align256(3 * V * 4);
   L.istart = off; off += align256(3 * (size_t)L.istride * 4);
@@ -431,9 +430,9 @@ class Engine {
     d.hold = (uint32_t*)(base + L.lvb + ((size_t)g_->V + 63) / 64 * 8);
     d.act = (uint32_t*)(base + L.act);
     d.mask = (uint64_t*)(base + L.mask);
-    const bool cown = own || q.D >= 0;
-    d.C = cown ? (uint32_t*)(base + L.C) : jdist + lab;
-    d.cstride = cown ? (uint32_t)(k * d.ND) : (uint32_t)jLp;
+    d.C = own ? (uint32_t*)(base + L.C) : jdist + lab;
+    d.cstride = own ? (uint32_t)(k * d.ND) : (uint32_t)(d.ND * jLp);
+    d.chs = own ? (uint32_t)k : (uint32_t)jLp;
     if (!own) {
       d.J = jdist + lab;
       d.jvs = (uint32_t)(d.ND * jLp);
