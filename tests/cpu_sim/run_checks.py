@@ -157,7 +157,7 @@ def check_branching():
 def check_graph():
     """Batches with engine steps replayed from a captured graph (graph=1, the
     mock runtime records the launches) against eager launches and the oracle."""
-    inst = data.make_config(2, scale=0.002, nq=4)
+    inst = data.make_config(2, scale=0.002, nq=2)
     g = Graph(inst.rowptr, inst.col, inst.wt)
     g.set_groups(inst.gptr, inst.gmem)
     qs = list(inst.queries)

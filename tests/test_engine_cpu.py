@@ -27,9 +27,9 @@ def cpu_results():
     except (OSError, subprocess.CalledProcessError) as e:  # no g++
         pytest.skip(f"CPU build unavailable: {e}")
     env = dict(os.environ, GST_LIB=lib)
-    # every 6th tiny case (all D), 54 k=5 queries = 270 labels = two jobs, and
-    # 24 hop-bounded queries
-    r = subprocess.run([sys.executable, os.path.join(ROOT, "tests", "cpu_sim", "run_checks.py"), "6", "54", "24"],
+    # every 20th tiny case (all D; the GPU tests take all of them), 54 k=5
+    # queries = 270 labels = two jobs, and 16 hop-bounded queries
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "tests", "cpu_sim", "run_checks.py"), "20", "54", "16"],
                        env=env, capture_output=True, text=True, timeout=900)
     assert r.returncode == 0, r.stderr[-2000:]
     return json.loads(r.stdout.strip().splitlines()[-1])
@@ -37,7 +37,7 @@ def cpu_results():
 
 def test_tiny_cases_on_cpu_build(cpu_results):
     t = cpu_results["tiny"]
-    assert t["checked"] > 100
+    assert t["checked"] > 30
     assert t["bad"] == [], t["bad"]
 
 
