@@ -95,6 +95,26 @@ inline uint64_t ekey(uint32_t a, uint32_t b) {
   return ((uint64_t)a << 32) | b;
 }
 
+int worker_count() { return (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency())); }
+
+// f(i) for i in [0, n) on up to worker_count() threads, in contiguous ranges
+template <class F>
+void parallel_range(int64_t n, F f) {
+  const int nt = (int)std::min<int64_t>(worker_count(), std::max<int64_t>(1, n / 4096));
+  if (nt <= 1) {
+    for (int64_t i = 0; i < n; i++) f(i);
+    return;
+  }
+  std::vector<std::thread> th;
+  const int64_t per = (n + nt - 1) / nt;
+  for (int t = 0; t < nt; t++)
+    th.emplace_back([&, t]() {
+      const int64_t a = t * per, b = std::min(n, (t + 1) * per);
+      for (int64_t i = a; i < b; i++) f(i);
+    });
+  for (auto& x : th) x.join();
+}
+
 }  // namespace
 
 extern "C" {
@@ -113,16 +133,27 @@ int64_t gstd_chung_lu(int64_t V, int64_t m, double gamma, uint64_t seed, int32_t
   int64_t n = 0;
   uint64_t tries = 0;
   const uint64_t max_tries = (uint64_t)m * 200 + 1000;
+  // Candidates in blocks: the draws stay in one sequential stream, the
+  // endpoint searches run in parallel, and the candidates are accepted in
+  // stream order, so the edge list is the same as drawing one at a time.
+  const int64_t B = (int64_t)1 << 20;
+  std::vector<double> xs(2 * B);
+  std::vector<int64_t> ends(2 * B);
   while (n < m && tries < max_tries) {
-    tries++;
-    const double x = rng.unit() * acc, y = rng.unit() * acc;
-    const int64_t a = std::upper_bound(cum.begin(), cum.end(), x) - cum.begin();
-    const int64_t b = std::upper_bound(cum.begin(), cum.end(), y) - cum.begin();
-    if (a >= V || b >= V || a == b) continue;
-    if (!set.insert(ekey((uint32_t)a, (uint32_t)b))) continue;
-    eu[n] = (int32_t)std::min(a, b);
-    ev[n] = (int32_t)std::max(a, b);
-    n++;
+    const int64_t nb = (int64_t)std::min<uint64_t>((uint64_t)B, max_tries - tries);
+    for (int64_t i = 0; i < 2 * nb; i++) xs[i] = rng.unit() * acc;
+    parallel_range(2 * nb, [&](int64_t i) {
+      ends[i] = std::upper_bound(cum.begin(), cum.end(), xs[i]) - cum.begin();
+    });
+    for (int64_t i = 0; i < nb && n < m; i++) {
+      tries++;
+      const int64_t a = ends[2 * i], b = ends[2 * i + 1];
+      if (a >= V || b >= V || a == b) continue;
+      if (!set.insert(ekey((uint32_t)a, (uint32_t)b))) continue;
+      eu[n] = (int32_t)std::min(a, b);
+      ev[n] = (int32_t)std::max(a, b);
+      n++;
+    }
   }
   return n;
 }
@@ -177,7 +208,25 @@ int64_t gstd_rmat(int64_t V, int64_t m, int32_t scale, double a, double b, doubl
 void gstd_sort_edges(int64_t m, int32_t* eu, int32_t* ev) {
   std::vector<uint64_t> k(m);
   for (int64_t i = 0; i < m; i++) k[i] = ekey((uint32_t)eu[i], (uint32_t)ev[i]);
-  std::sort(k.begin(), k.end());
+  // sorted runs in parallel, then pairwise merges (the keys are distinct, so
+  // the result is the one order)
+  const int nt = m > (1 << 20) ? worker_count() : 1;
+  std::vector<int64_t> cut(nt + 1);
+  for (int t = 0; t <= nt; t++) cut[t] = m * t / nt;
+  {
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; t++)
+      th.emplace_back([&, t]() { std::sort(k.begin() + cut[t], k.begin() + cut[t + 1]); });
+    for (auto& x : th) x.join();
+  }
+  for (int w = 1; w < nt; w *= 2) {
+    std::vector<std::thread> th;
+    for (int t = 0; t + w < nt; t += 2 * w) {
+      const int64_t a = cut[t], mid = cut[t + w], b = cut[std::min(nt, t + 2 * w)];
+      th.emplace_back([&, a, mid, b]() { std::inplace_merge(k.begin() + a, k.begin() + mid, k.begin() + b); });
+    }
+    for (auto& x : th) x.join();
+  }
   for (int64_t i = 0; i < m; i++) {
     eu[i] = (int32_t)(k[i] >> 32);
     ev[i] = (int32_t)(k[i] & 0xFFFFFFFFu);
