@@ -18,6 +18,7 @@
 #include <string>
 #include <thread>
 #include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #include "gst.h"
@@ -352,13 +353,20 @@ class Engine {
         n++;
         continue;
       }
+      // the job takes the front query and the next pending queries of the
+      // same bound (skipping others, which keep their order), within a window
+      // of the queue
       std::vector<QueryReq*> grp;
+      std::vector<size_t> at;  // their positions in pend
       int L = 0;
-      for (auto it = pend.begin(); it != pend.end(); ++it) {
-        const int k = (int)(*it)->groups.size();
-        if (!uses_job(g_, k, (*it)->D) || (*it)->D != q0->D || L + k > gst::kJobLabels) break;
-        if (used_slots_ + (int)grp.size() + 2 > max_slots_) break;
-        grp.push_back(*it);
+      const size_t window = std::min(pend.size(), (size_t)(4 * gst::kJobLabels));
+      for (size_t i = 0; i < window; i++) {
+        QueryReq* q = pend[i];
+        const int k = (int)q->groups.size();
+        if (!uses_job(g_, k, q->D) || q->D != q0->D) continue;
+        if (L + k > gst::kJobLabels || used_slots_ + (int)grp.size() + 2 > max_slots_) break;
+        grp.push_back(q);
+        at.push_back(i);
         L += k;
       }
       if ((int64_t)grp.size() < g_->opt.job_min_queries) {
@@ -373,8 +381,9 @@ class Engine {
       while (!grp.empty() && !admit_job_group(grp, L)) {
         L -= (int)grp.back()->groups.size();
         grp.pop_back();
+        at.pop_back();
       }
-      for (size_t i = 0; i < grp.size(); i++) pend.pop_front();
+      for (size_t i = at.size(); i-- > 0;) pend.erase(pend.begin() + (std::ptrdiff_t)at[i]);
       n += (int)grp.size();
       if (grp.size() < want || grp.empty()) break;
     }
@@ -867,6 +876,27 @@ void parallel_for(int64_t n, F f) {
   for (auto& x : th) x.join();
 }
 
+// Parallel loop over the vertices [0, V), in ranges of about equal arc counts
+// (power-law graphs put most arcs on a few low-numbered hubs).
+template <class F>
+void parallel_for_rows(const int64_t* beg_pos, int64_t V, F f) {
+  int nt = (int)std::min<int64_t>(std::max(1u, std::thread::hardware_concurrency()), 16);
+  const int64_t arcs = beg_pos[V];
+  if (V < 100000 && arcs < 1000000) nt = 1;
+  std::vector<std::thread> th;
+  int64_t b = 0;
+  for (int t = 0; t < nt && b < V; t++) {
+    const int64_t target = arcs / nt * (t + 1);
+    int64_t e = t + 1 == nt ? V : (int64_t)(std::upper_bound(beg_pos, beg_pos + V + 1, target) - beg_pos) - 1;
+    e = std::max(e, b + 1);
+    e = std::min(e, V);
+    th.emplace_back([=]() { f(b, e); });
+    b = e;
+  }
+  if (b < V) f(b, V);
+  for (auto& x : th) x.join();
+}
+
 gst_status engine_for(gst_graph* g, bool batch, size_t need, Engine** out) {
   std::unique_ptr<Engine>& e = batch ? g->batch : g->plain;
   if (!e) {
@@ -965,7 +995,7 @@ gst_status gst_graph_create(const int64_t* beg_pos, int64_t V, const int32_t* ad
   // rows sorted by (neighbour, weight), so the reverse of every arc is found
   // by binary search
   std::vector<int32_t> col(adj, adj + n_arcs), wt(w, w + n_arcs);
-  parallel_for(V, [&](int64_t b, int64_t e) {
+  parallel_for_rows(beg_pos, V, [&](int64_t b, int64_t e) {
     std::vector<std::pair<int32_t, int32_t>> tmp;
     for (int64_t v = b; v < e; v++) {
       const int64_t r0 = beg_pos[v], r1 = beg_pos[v + 1];
@@ -984,7 +1014,7 @@ gst_status gst_graph_create(const int64_t* beg_pos, int64_t V, const int32_t* ad
   });
   std::vector<int32_t> rev(n_arcs);
   std::atomic<int64_t> asym{-1};
-  parallel_for(V, [&](int64_t b, int64_t e) {
+  parallel_for_rows(beg_pos, V, [&](int64_t b, int64_t e) {
     for (int64_t u = b; u < e; u++) {
       for (int64_t i = beg_pos[u]; i < beg_pos[u + 1]; i++) {
         const int32_t v = col[i], ww = wt[i];
@@ -1262,10 +1292,22 @@ void server_loop(gst_server* s) {
           pr.second.tag = pr.first;
           pend.push_back(&pr.second);
         }
+        const std::vector<QueryReq*> offered(pend.begin(), pend.end());
         const int n = s->eng->admit_front(pend);
-        for (int i = 0; i < n; i++) {
-          s->known[s->pending.front().first] = 1;
-          s->pending.pop_front();
+        if (n > 0) {
+          // the admitted requests are the offered ones no longer pending (not
+          // always a prefix: a job gathers requests of one bound from further
+          // back in the queue)
+          std::unordered_set<QueryReq*> left(pend.begin(), pend.end());
+          std::unordered_set<uint64_t> taken;
+          for (QueryReq* q : offered)
+            if (!left.count(q)) taken.insert(q->tag);
+          std::deque<std::pair<uint64_t, QueryReq>> rest;
+          for (auto& pr : s->pending) {
+            if (taken.count(pr.first)) s->known[pr.first] = 1;
+            else rest.push_back(std::move(pr));
+          }
+          s->pending.swap(rest);
         }
         if (n > 0 || s->eng->active() > 0) break;
         // the first pending query can never fit: answer it with an error result
