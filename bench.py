@@ -98,6 +98,76 @@ def profiled_traffic(kernel):
     return None
 
 
+def poisson_serve(g, qs, Ds, rate, seed):
+    """Open-loop Poisson arrivals of the requests (qs, Ds) into a fresh server
+    at `rate` per second; returns (achieved q/s, p50 ms, p99 ms, results)."""
+    import threading
+    from gpugst_amd import Server
+    n = len(qs)
+    gaps = np.random.default_rng(seed).exponential(1.0 / rate, n)
+    srv = Server(g)
+    tickets = [None] * n
+
+    def submitter():
+        t_next = time.perf_counter()
+        for i in range(n):
+            t_next += gaps[i]
+            dt = t_next - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            tickets[i] = srv.submit(qs[i], D=Ds[i])
+
+    ts = time.perf_counter()
+    th = threading.Thread(target=submitter)
+    th.start()
+    th.join()
+    res = [srv.wait(t) for t in tickets]
+    span = time.perf_counter() - ts
+    srv.close()
+    lat = [r.latency_ms for r in res]
+    return n / span, pct(lat, 50), pct(lat, 99), res
+
+
+def serve_config5(log):
+    """The serving configuration: Chung-Lu graph, V = 3.0M, 117M undirected
+    edges (exponent 2.0, weights in [1, 1000]), 2M groups (geometric sizes,
+    mean 10, max 5,000); requests with k uniform in {3..6}, 20% with hop bound
+    D = 3.  Capacity = one batch of 96 requests; then 96 Poisson requests at
+    each of 0.5x, 0.8x and 1.1x of it, every answer checked against a one-query
+    call for the first 16."""
+    from gpugst_amd import Graph, data
+    t0 = time.time()
+    inst = data.make_config(5, nq=96 * 5)
+    log(f"serving config: V={inst.V} arcs={inst.col.shape[0]} groups={inst.gptr.shape[0]-1} "
+        f"gen {time.time()-t0:.1f}s")
+    g = Graph(inst.rowptr, inst.col, inst.wt)
+    g.set_groups(inst.gptr, inst.gmem)
+    qs, Ds = list(inst.queries), list(inst.D)
+    g.solve_batch(qs[:16], Ds[:16])  # warm-up
+    tb = time.perf_counter()
+    rb = g.solve_batch(qs[16:112], Ds[16:112])
+    cap = 96 / (time.perf_counter() - tb)
+    log(f"serving config: batched capacity {cap:.1f} q/s")
+    g.set_option("budget_mb", 0)  # the batch engine hands its arena back to the server
+    out = {"workload": "chunglu-3M-117M-edges, k in {3..6}, 20% D=3", "batched_capacity_qps": round(cap, 1),
+           "rates": []}
+    mism = 0
+    for i, f in enumerate((0.5, 0.8, 1.1)):
+        a = 112 + 96 * i
+        q, d = qs[a:a + 96], Ds[a:a + 96]
+        ach, p50, p99, res = poisson_serve(g, q, d, f * cap, 21 + i)
+        if i == 0:
+            for j in range(16):
+                mism += int(g.solve(q[j], D=d[j]).cost != res[j].cost)
+        out["rates"].append({"offered_qps": round(f * cap, 1), "achieved_qps": round(ach, 1),
+                             "p50_ms": round(p50, 1), "p99_ms": round(p99, 1), "requests": len(q)})
+        log(f"serving config: offered {f*cap:.1f} q/s -> {ach:.1f} q/s, p50 {p50:.1f} ms p99 {p99:.1f} ms")
+    out["mix_vs_alone_mismatches"] = mism
+    out["batch_infeasible"] = sum(1 for r in rb if not r.feasible)
+    g.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,6 +187,9 @@ def main():
     ap.add_argument("--serve", type=int, default=512, help="queries of the serving leg (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=0.0,
                     help="offered queries/s of the serving leg (default: half the batched throughput)")
+    ap.add_argument("--serve-c5", action="store_true",
+                    help="also serve the 3M-vertex serving configuration (k in {3..6}, 20%% D=3) "
+                         "at 0.5x, 0.8x and 1.1x of its batched capacity")
     ap.add_argument("--tune", action="store_true",
                     help="run the schedule autotuner on a sample batch before the batched leg")
     ap.add_argument("--tune-db", default=None,
@@ -318,6 +391,13 @@ def main():
                          f"single thread, {cs:.1f} s"}
         log(f"cpu oracle: {n} queries in {cs:.1f}s, mismatches {oracle_mism}")
 
+    # ---------------- serving on the 3M-vertex serving configuration (optional;
+    # the first graph's engines give their device memory back first)
+    serving_c5 = None
+    if args.serve_c5:
+        g.close()
+        serving_c5 = serve_config5(log)
+
     lat = [r.latency_ms for r in timed]
     dev = [r.device_ms for r in timed]
     out = {
@@ -347,6 +427,7 @@ def main():
                    "oracle_checked": oracle_checked, "oracle_mismatches": oracle_mism,
                    "infeasible": infeas},
         "serving": serving,
+        "serving_c5": serving_c5,
         "roofline": roof,
         "cpu_baseline": cpu,
     }
