@@ -50,7 +50,7 @@ def lean_bytes(g, res_list, V, ks):
                       per 4-label chunk relaxed 16 B (target chunk read); per
                       improved chunk 16 B (write back)
       job SCATTER     4 B per (vertex, label) read + 4 B per (vertex, label) written
-      query FILL      24 B per vertex (3 masks) + 1/8 B live flags"""
+      query FILL      24 B per vertex (3 masks) + 1/4 B live and hold bits"""
     labels = int(sum(ks))
     jobs = max(1, g.get_option("batch_jobs"))
     lp = labels + 3 * jobs  # rows padded to whole 4-label chunks
@@ -61,23 +61,27 @@ def lean_bytes(g, res_list, V, ks):
     b = 4 * V * lp + 24 * V * jobs
     b += 36 * ent + 8 * arcs + 16 * (relax // 4) + 16 * impr
     b += 4 * V * lp + 4 * V * labels
-    b += (24 * V + V // 8) * len(res_list)
+    b += (24 * V + V // 4) * len(res_list)
     return b
 
 
 def heavy_bytes(res_list, V, ks, ND=1):
-    """Algorithmic bytes of the heavy kernel's items: per query the live test
-    (C row, 4 B per vertex and group), T segments of live vertices (8 B per
-    state: initialised once, read by every merge level and FINAL), and the
-    level >= 2 grow rounds (24 B per frontier entry, 8 B per arc, 16 B per
-    relaxation candidate, 8 B per improvement) -- the counters of a query fed
-    by a job hold only these rounds."""
+    """Algorithmic bytes of the heavy kernel's items, per query: the live test
+    (C row, 4 B per vertex and group), the T segments of live vertices written
+    once (8 B per state), the level-2 merge reading the singletons and C rows
+    of every live vertex (12 B per vertex, group and height), the later merges
+    and FINAL reading the T segments of the active vertices (8 B per state and
+    pass), and the level >= 2 grow rounds (24 B per frontier entry, 8 B per arc,
+    16 B per relaxation candidate, 8 B per improvement) -- the counters of a
+    query fed by a job hold only these rounds."""
     total = 0
     for r, k in zip(res_list, ks):
         W = 1 << k
         c = r.counters
         nl = c.get("live_vertices", 0)
-        total += 4 * V * k + nl * W * ND * 8 * (1 + max(k - 1, 1))
+        na = c.get("active_vertices", 0)
+        passes = max(k - 3, 0) + 1  # merges of levels 3..k-1, then FINAL
+        total += 4 * V * k + nl * W * ND * 8 + nl * k * ND * 12 + na * W * ND * 8 * passes
         total += 24 * c["frontier_entries"] + 8 * c["arcs"] + 16 * c["relaxations"] + 8 * c["improvements"]
     return total
 

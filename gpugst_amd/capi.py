@@ -53,7 +53,8 @@ class c_counters(ctypes.Structure):
                 ("arcs", ctypes.c_int64), ("relaxations", ctypes.c_int64),
                 ("improvements", ctypes.c_int64), ("merge_evals", ctypes.c_int64),
                 ("live_vertices", ctypes.c_int64), ("shared_entries", ctypes.c_int64),
-                ("shared_arcs", ctypes.c_int64), ("shared_relaxations", ctypes.c_int64)]
+                ("shared_arcs", ctypes.c_int64), ("shared_relaxations", ctypes.c_int64),
+                ("active_vertices", ctypes.c_int64)]
 
 
 class c_result(ctypes.Structure):

@@ -747,6 +747,7 @@ class Engine {
     out->counters.shared_entries = h.extra[1];
     out->counters.shared_arcs = h.extra[2];
     out->counters.shared_relaxations = h.extra[3];
+    out->counters.active_vertices = (int64_t)d.nact;
     gst_status st = GST_OK;
     if (d.status == 1) {
       out->status = GST_DEVICE_ERROR;

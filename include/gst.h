@@ -84,6 +84,7 @@ typedef struct {
   int64_t shared_entries;   /* share of its job's frontier entries */
   int64_t shared_arcs;      /* share of its job's arcs */
   int64_t shared_relaxations; /* share of its job's label relaxations */
+  int64_t active_vertices;  /* live vertices that received a state of two or more groups */
 } gst_counters;
 
 typedef struct {
