@@ -1022,10 +1022,16 @@ __device__ void item_merge(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* 
       stage_rows(sp, vl + vb, nv, rowlen, W, L->stage);
     }
     uint32_t* cst = (uint32_t*)(L->stage + nv * rowlen);  // C rows of the staged vertices
-    for (int t = lane; t < nv * ND * k; t += 64) {
-      const int jj = t / (ND * k), rem = t - jj * ND * k;
-      const int d = rem / k;
-      cst[t] = crow(sp, vl[vb + jj], d)[rem - d * k];
+    if (l2 && !sp->S1) {
+      // a job-fed query's singletons are its C rows: take them from the stage
+      wave_sync();
+      for (int t = lane; t < nv * rowlen; t += 64) cst[t] = (uint32_t)(L->stage[t] >> 32);
+    } else {
+      for (int t = lane; t < nv * ND * k; t += 64) {
+        const int jj = t / (ND * k), rem = t - jj * ND * k;
+        const int d = rem / k;
+        cst[t] = crow(sp, vl[vb + jj], d)[rem - d * k];
+      }
     }
     if (lane < nv) {
       L->f.wv[lane] = vl[vb + lane];
