@@ -99,7 +99,8 @@ struct alignas(64) Slot {
   int32_t job;          // query fed by a job: its slot index, else -1
   int32_t lab;          // query fed by a job: its first label in the job
   uint32_t jdone;       // set by the job when this query's C is complete
-  int32_t k, W, ND, D;  // D < 0: unbounded (ND == 1)
+  int32_t k, W, ND, D;  // D < 0: unbounded (ND == 1); ND = D + 1 heights of S1 / C / job rows
+  int32_t NS, pad5;     // heights of the states T: floor(D/2) + 1 rounded up (1 unbounded)
   int32_t qg[kMaxK];
   int32_t edge_cap;
   int32_t chunk;        // arcs per ROUND item

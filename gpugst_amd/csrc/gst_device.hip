@@ -119,19 +119,20 @@ typedef GST_AS_GLOBAL uint32_t g_u32;
 typedef GST_AS_GLOBAL const uint32_t g_cu32;
 typedef GST_AS_GLOBAL unsigned long long g_u64;
 
-// State (v, p, d): singletons in the compact S1 table or, when the slot's
-// distances come from a group-distance job, synthesised from C with the
-// kPredPath predecessor; every other subset in the vertex-major T segments.
+// State (v, p, d): singletons in the compact S1 table (heights 0..D) or,
+// when the slot's distances come from a group-distance job, synthesised from
+// the job's rows with the kPredPath predecessor; every other subset in the
+// vertex-major T segments (heights 0..NS-1).
 struct States {
   g_u64* T;
   g_u64* S1;
   g_cu32* J;
   uint32_t jvs, jds;
-  int k, ND, W;
+  int k, ND, NS, W;
   // address of a state that is written (never a singleton of a C-fed slot)
   __device__ __forceinline__ g_u64* at(uint32_t v, uint32_t p, uint32_t d) const {
     if ((p & (p - 1u)) == 0u) return S1 + ((uint64_t)v * ND + d) * (uint64_t)k + (uint32_t)(__ffs(p) - 1);
-    return T + ((uint64_t)v * ND + d) * (uint64_t)W + p;
+    return T + ((uint64_t)v * NS + d) * (uint64_t)W + p;
   }
   __device__ __forceinline__ uint64_t get(uint32_t v, uint32_t p, uint32_t d) const {
     if ((p & (p - 1u)) == 0u) {
@@ -139,7 +140,7 @@ struct States {
       if (S1) return S1[((uint64_t)v * ND + d) * (uint64_t)k + i];
       return ((uint64_t)jcost(v, i, d) << 32) | kPredPath;
     }
-    return T[((uint64_t)v * ND + d) * (uint64_t)W + p];
+    return T[((uint64_t)v * NS + d) * (uint64_t)W + p];
   }
   // cost of the singleton T(v, {i}, d) of a job-fed query
   __device__ __forceinline__ uint32_t jcost(uint64_t v, uint32_t i, uint32_t d) const {
@@ -156,6 +157,7 @@ __device__ __forceinline__ States states_of(const Slot* sp) {
   st.jds = sp->jds;
   st.k = sp->k;
   st.ND = sp->ND;
+  st.NS = sp->NS;
   st.W = sp->W;
   return st;
 }
@@ -464,8 +466,10 @@ __device__ void item_round(const GraphDev& g, Slot* sp, uint32_t chunk, FrontLDS
   const uint64_t V = (uint64_t)g.V;
   const int k = sp->k, W = sp->W, D = sp->D;
   const bool hop = D >= 0;
-  const int dmax = hop ? D : 1;  // source depths that may grow
   const int level = sp->level;
+  // source heights that may grow: singletons up to D (the C rows need every
+  // height), states of two or more groups up to NS - 1
+  const int dmax = hop ? (level >= 2 ? sp->NS - 1 : D) : 1;
   const uint32_t full = (uint32_t)W - 1;
   const uint32_t r = sp->step;
   const int cur = r % 3, nxt = (r + 1) % 3;
@@ -544,7 +548,7 @@ __device__ void item_round(const GraphDev& g, Slot* sp, uint32_t chunk, FrontLDS
               if (__popc(x) <= level) {
                 // a complete walk-tree: this state plus the complement state at u
                 uint32_t cbest = kInf;
-                const int d2max = hop ? (D - td) : 0;
+                const int d2max = hop ? ((D - td) < sp->NS - 1 ? (D - td) : sp->NS - 1) : 0;
                 for (int d2 = 0; d2 <= d2max; d2++) {
                   const uint32_t c2 = (uint32_t)(S.get(u, x, d2) >> 32);
                   cbest = c2 < cbest ? c2 : cbest;
@@ -913,8 +917,9 @@ __device__ void item_vvr(const GraphDev& g, Slot* sp, WaveLDS* L) {
   if (lane < kk) sp->vv[lane] = vvl[lane];
 }
 
-// Stages the W * ND segments of nv live vertices into LDS (row-major like T);
-// singleton entries are taken from S1 or C, the unused p = 0 column is INF.
+// Stages the W * NS segments of nv live vertices into LDS (row-major like T);
+// singleton entries are taken from S1 or the job rows, the unused p = 0
+// column is INF.
 __device__ __forceinline__ void stage_rows(const Slot* sp, const uint32_t* ids, int nv, int rowlen,
                                            int W, uint64_t* stage) {
   const States S = states_of(sp);
@@ -934,8 +939,8 @@ __device__ __forceinline__ void stage_rows(const Slot* sp, const uint32_t* ids, 
 // and a live flag; only they are ever read or written later.
 __device__ void item_live(Slot* sp, uint32_t tile, uint64_t V, WaveLDS* L) {
   const int lane = lane_id();
-  const int k = sp->k, W = sp->W, ND = sp->ND;
-  const int rowlen = W * ND;
+  const int k = sp->k, W = sp->W;
+  const int rowlen = W * sp->NS;
   const uint32_t bst = __hip_atomic_load(&sp->best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   const uint64_t n = sp->vsrc ? (uint64_t)sp->nvsrc : V;  // candidate vertices
   const uint64_t v0 = (uint64_t)tile * kDistTV;
@@ -972,14 +977,14 @@ __device__ void item_live(Slot* sp, uint32_t tile, uint64_t V, WaveLDS* L) {
 }
 
 // Level-L merge: T(v,p,nd) for every |p| = L subset p and height nd of every
-// live vertex, from the final lower levels.  The W * ND segments of up to VPW
+// live vertex, from the final lower levels.  The W * NS segments of up to VPW
 // live vertices are staged in LDS; the wave's lanes then take the (height,
 // vertex, subset) tuples of the staged vertices in turn, height-major so that
 // neighbouring lanes run the same height-pair loops.
 __device__ void item_merge(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* L) {
   const int lane = lane_id();
   const uint64_t V = (uint64_t)g.V;
-  const int k = sp->k, W = sp->W, ND = sp->ND, D = sp->D;
+  const int k = sp->k, W = sp->W, ND = sp->ND, NS = sp->NS, D = sp->D;
   const bool hop = D >= 0;
   const int Dlim = hop ? D : 0;
   const int level = sp->level;
@@ -987,10 +992,10 @@ __device__ void item_merge(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* 
   const int nxt = (sp->step + 1) % 3;
   uint64_t* mnxt = sp->mask + (uint64_t)nxt * V;
   // Level 2 reads only singletons (every state of two or more groups is still
-  // INF), so it stages the ND * k singletons of a vertex instead of its whole
+  // INF), so it stages the NS * k singletons of a vertex instead of its whole
   // segment: row[d * k + i] instead of row[d * W + p].
   const bool l2 = level == 2;
-  const int rowlen = l2 ? ND * k : W * ND;
+  const int rowlen = l2 ? NS * k : W * NS;
   // each staged vertex also brings its C rows (ND * k words, in pairs)
   const int crl = (ND * k + 1) / 2;
   const int VPW = (kStageWords / (rowlen + crl)) < 64 ? (kStageWords / (rowlen + crl)) : 64;
@@ -1022,7 +1027,7 @@ __device__ void item_merge(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* 
       stage_rows(sp, vl + vb, nv, rowlen, W, L->stage);
     }
     uint32_t* cst = (uint32_t*)(L->stage + nv * rowlen);  // C rows of the staged vertices
-    if (l2 && !sp->S1) {
+    if (l2 && !sp->S1 && NS == ND) {
       // a job-fed query's singletons are its C rows: take them from the stage
       wave_sync();
       for (int t = lane; t < nv * rowlen; t += 64) cst[t] = (uint32_t)(L->stage[t] >> 32);
@@ -1039,7 +1044,7 @@ __device__ void item_merge(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* 
     }
     wave_sync();
     const int per_h = nv * nL;  // tuples of one height
-    for (int t = lane; t < per_h * ND; t += 64) {
+    for (int t = lane; t < per_h * NS; t += 64) {
       const int nd = t / per_h;
       const int r = t - nd * per_h;
       const int jj = r / nL;
@@ -1096,7 +1101,7 @@ __device__ void item_merge(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* 
       if (c >= kInf) continue;
       // a state of height nd reaches the rest of its tree within D - nd hops
       if (c + lower_bound(crow(sp, v, hop ? D - nd : 0), k, x, (g_cu32*)sp->R) > bst0) continue;
-      sp->T[((uint64_t)v * ND + nd) * W + p] = bestpk;
+      sp->T[((uint64_t)v * NS + nd) * W + p] = bestpk;
       atomicOr((unsigned long long*)&L->f.wm[jj], 1ull << p);
       // upper bounds from complete walk-trees at v: this state plus one
       // shortest path per missing group (hop: paths of at most h hops, with
@@ -1109,7 +1114,7 @@ __device__ void item_merge(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* 
       uint32_t ub = sat(c + psum);
       // ... or the complement state at v, when its level is final
       if (__popc(x) < level) {
-        const int d2m = Dlim - nd;
+        const int d2m = (Dlim - nd) < NS - 1 ? (Dlim - nd) : NS - 1;
         for (int d2 = 0; d2 <= d2m; d2++) {
           const uint32_t u2 = sat(c + (uint32_t)(st(d2, x) >> 32));
           ub = u2 < ub ? u2 : ub;
@@ -1143,11 +1148,11 @@ __device__ void item_merge(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* 
 // Full-set merge: W/2 lanes per vertex, one lane per split (sub contains bit 0).
 __device__ void item_final(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* L) {
   const int lane = lane_id();
-  const int k = sp->k, W = sp->W, ND = sp->ND, D = sp->D;
+  const int k = sp->k, W = sp->W, ND = sp->ND, NS = sp->NS, D = sp->D;
   const bool hop = D >= 0;
   const int Dlim = hop ? D : 0;
   const uint32_t full = (uint32_t)W - 1;
-  const int rowlen = W * ND;
+  const int rowlen = W * NS;
   const int half = W / 2;  // W >= 2
   const int crl = (ND * k + 1) / 2;  // C rows of a staged vertex, in u64 words
   int VPF = 64 / half;
@@ -1178,7 +1183,7 @@ __device__ void item_final(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* 
     const bool vact = j < nv;
     const uint32_t v = vact ? vl[vb + j] : 0u;
     const uint64_t* row = L->stage + (vact ? j : 0) * rowlen;
-    for (int nd = 0; nd < ND; nd++) {
+    for (int nd = 0; nd < NS; nd++) {
       // a tree of height nd at v reaches every group within nd hops: when the
       // farthest one costs more than the upper bound it cannot be the answer
       bool vok = vact;
@@ -1219,7 +1224,7 @@ __device__ void item_final(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* 
         bestpk = y < bestpk ? y : bestpk;
       }
       if (vact && s == 0 && (uint32_t)(bestpk >> 32) < kInf) {
-        sp->T[((uint64_t)v * ND + nd) * W + full] = bestpk;
+        sp->T[((uint64_t)v * NS + nd) * W + full] = bestpk;
         const uint64_t a = (bestpk & 0xFFFFFFFF00000000ull) | v;
         ansmin = a < ansmin ? a : ansmin;
       }
@@ -1242,7 +1247,7 @@ __device__ void item_final(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* 
 // found by a wave-wide scan of v's row (it exists because the rows are exact).
 __device__ void item_recon(const GraphDev& g, Slot* sp) {
   const int lane = lane_id();
-  const int W = sp->W, ND = sp->ND, D = sp->D;
+  const int W = sp->W, NS = sp->NS, D = sp->D;
   const States S = states_of(sp);
   const bool hop = D >= 0;
   const uint32_t full = (uint32_t)W - 1;
@@ -1260,8 +1265,8 @@ __device__ void item_recon(const GraphDev& g, Slot* sp) {
   const uint32_t root = (uint32_t)ans;
   // smallest depth holding the optimum at the root
   uint32_t cd = kInf;
-  if (lane < ND) cd = (uint32_t)(S.get(root, full, (uint32_t)lane) >> 32);
-  const uint64_t hit = __ballot(lane < ND && cd == cost);
+  if (lane < NS) cd = (uint32_t)(S.get(root, full, (uint32_t)lane) >> 32);
+  const uint64_t hit = __ballot(lane < NS && cd == cost);
   const int dstar = hit ? (__ffsll((long long)hit) - 1) : 0;
   const uint32_t qcap = (uint32_t)sp->edge_cap + 2 * kMaxK;
   uint32_t* qv = sp->rq;

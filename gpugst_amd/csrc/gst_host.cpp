@@ -194,7 +194,10 @@ Layout layout_for(const gst_graph* g, int k, int D, bool own) {
   L.edge_cap = (int)std::max<int64_t>(cap, 64);
   L.istride = istride_for(g, g->opt.chunk);
   size_t off = 0;
-  L.T = off;    off += align256(V * ND * W * 8);
+  // states need heights 0..ceil(D/2): rooted at its center, a tree of diameter
+  // <= D has height <= ceil(D/2)
+  const size_t NS = D < 0 ? 1 : (size_t)(D + 1) / 2 + 1;
+  L.T = off;    off += align256(V * NS * W * 8);
   L.S1 = off;   off += own ? align256(V * ND * k * 8) : 0;
   L.live = off; off += align256(V * 4);
   L.lvb = off;  off += align256(2 * ((V + 63) / 64) * 8);  // live bits, then hold bits
@@ -430,7 +433,8 @@ class Engine {
     d.edge_cap = L.edge_cap;
     d.chunk = (int32_t)g_->opt.chunk;
     d.istride = L.istride;
-    d.nT = (uint64_t)g_->V * d.ND * d.W;
+    d.NS = q.D < 0 ? 1 : (q.D + 1) / 2 + 1;
+    d.nT = (uint64_t)g_->V * d.NS * d.W;
     char* base = arena_mem_ + off;
     d.T = (uint64_t*)(base + L.T);
     d.S1 = own ? (uint64_t*)(base + L.S1) : nullptr;
