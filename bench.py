@@ -41,15 +41,17 @@ def pct(x, q):
     return float(np.percentile(np.asarray(x, dtype=np.float64), q)) if len(x) else None
 
 
-def lean_bytes(g, res_list, V, ks):
+def lean_bytes(g, res_list, V, ks, ND=1):
     """Algorithmic bytes of the lean kernel's items for these queries (one
-    group-distance job per up to 256 labels):
-      job FILL        4 B per (vertex, label) + 24 B per vertex (3 masks)
+    group-distance job per up to 256 labels; ND = D + 1 rows per vertex for a
+    hop job, 1 otherwise):
+      job FILL        4 B per (vertex, label, height) + 24 B per vertex (3 masks)
       job ROUND       per frontier entry 24 B (list id, arc offset, mask, row
                       pointer) + 12 B clear; per arc 8 B (column + weight);
                       per 4-label chunk relaxed 16 B (target chunk read); per
                       improved chunk 16 B (write back)
-      job SCATTER     4 B per (vertex, label) read + 4 B per (vertex, label) written
+      job STAR        4 B per (vertex, label, height) read; a hop job writes
+                      its rows cumulative, 4 B per (vertex, label, height > 0)
       query FILL      24 B per vertex (3 masks) + 1/4 B live and hold bits"""
     labels = int(sum(ks))
     jobs = max(1, g.get_option("batch_jobs"))
@@ -58,9 +60,9 @@ def lean_bytes(g, res_list, V, ks):
     arcs = g.get_option("batch_job_arcs")
     relax = g.get_option("batch_job_relaxations")
     impr = g.get_option("batch_job_improvements")
-    b = 4 * V * lp + 24 * V * jobs
+    b = 4 * V * lp * ND + 24 * V * jobs
     b += 36 * ent + 8 * arcs + 16 * (relax // 4) + 16 * impr
-    b += 4 * V * lp + 4 * V * labels
+    b += 4 * V * lp * ND + 4 * V * lp * (ND - 1)
     b += (24 * V + V // 4) * len(res_list)
     return b
 
@@ -300,7 +302,7 @@ def main():
         lean_ms = g.get_option("batch_lean_ns") / 1e6 / n_steps
         heavy_ms = g.get_option("batch_heavy_ns") / 1e6 / n_steps
         rk = [len(q) for q in qs]
-        lb = lean_bytes(g, rr, V, rk) if args.D < 0 else None
+        lb = lean_bytes(g, rr, V, rk, 1 if args.D < 0 else args.D + 1)
         hb = heavy_bytes(rr, V, rk, 1 if args.D < 0 else args.D + 1)
         dom = "gst_lean_kernel" if lean_ms >= heavy_ms else "gst_heavy_kernel"
         ms, byts = (lean_ms, lb) if dom == "gst_lean_kernel" else (heavy_ms, hb)
