@@ -70,7 +70,8 @@ enum Phase : int32_t {
   PH_DONE = 8,
   PH_LIVE = 9,   // live-vertex list + initialisation of their T segments
   PH_WAITJ = 10, // query waiting for its group-distance job (no work items)
-  PH_STAR = 11     // job: star bound of every member query
+  PH_STAR = 11,    // job: star bound of every member query
+  PH_COPY = 12     // hop job: height level starts as a copy of height level - 1
 };
 
 enum SlotKind : int32_t {

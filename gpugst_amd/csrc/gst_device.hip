@@ -115,6 +115,7 @@ __device__ __forceinline__ uint32_t sat(uint32_t a) { return a < kInf ? a : kInf
 #endif
 typedef uint32_t jchunk __attribute__((vector_size(4 * kJobChunkLabels)));
 typedef GST_AS_GLOBAL const jchunk g_cchunk;
+typedef GST_AS_GLOBAL jchunk g_chunk;
 typedef GST_AS_GLOBAL uint32_t g_u32;
 typedef GST_AS_GLOBAL const uint32_t g_cu32;
 typedef GST_AS_GLOBAL unsigned long long g_u64;
@@ -363,7 +364,7 @@ __device__ __forceinline__ void mark_hold(bool flag, uint32_t v, Slot* sp) {
 __device__ __forceinline__ void fill_regions(const Slot* sp, uint64_t V, uint64_t* nA, uint64_t* nM,
                                              uint64_t* nF) {
   if (sp->kind == KIND_JOB) {
-    *nA = V * (uint64_t)sp->ND * (uint64_t)sp->Lp / 2;
+    *nA = V * (uint64_t)sp->Lp / 2;  // height 0 of every row (COPY writes the others)
     *nM = 3 * V;
     *nF = 0;
     return;
@@ -386,9 +387,12 @@ __device__ void item_fill(Slot* sp, uint32_t idx, uint64_t V) {
   uint64_t* F64 = (uint64_t*)sp->lvb;
   const uint64_t cinf = ((uint64_t)kInf << 32) | kInf;
   const uint64_t aval = job ? cinf : kInfState;
+  // a hop job's height-0 rows: Lp / 2 words per vertex, ND * Lp / 2 words apart
+  const bool rows = job && sp->ND > 1;
+  const uint64_t hw = (uint64_t)sp->Lp / 2, rw = (uint64_t)sp->ND * hw;
   for (uint64_t t = base + lane_id(); t < end; t += 64) {
     if (t < nA)
-      A[t] = aval;
+      A[rows ? (t / hw) * rw + t % hw : t] = aval;
     else if (t < nA + nM)
       M[t - nA] = 0;
     else
@@ -626,8 +630,10 @@ __device__ __forceinline__ bool jrelax(uint32_t s, uint32_t t, uint32_t w, g_u32
 // kJobChunkLabels-label chunks r, r + G, ... of the source row into the target
 // row (one vector load per chunk, coalesced over the group).  Unbounded jobs
 // relax within the one row per vertex until no chunk is dirty; hop jobs relax
-// the rows of height level - 1 into those of height level (walks of exactly
-// level arcs), one round per height.
+// the rows of height level - 1 into those of height level, one round per
+// height, after COPY has set height level to height level - 1: the rows hold
+// the distances within level hops, and a round's frontier is only the chunks
+// that the previous height improved.
 __device__ void item_jround(const GraphDev& g, Slot* sp, uint32_t item, FrontLDS* L) {
   const int lane = lane_id();
   const uint64_t V = (uint64_t)g.V;
@@ -708,12 +714,32 @@ __device__ void item_jround(const GraphDev& g, Slot* sp, uint32_t item, FrontLDS
   }
 }
 
+// Hop job COPY: height `level` of the rows of kStarTV vertices starts as a copy
+// of height level - 1 (one lane per chunk, coalesced).
+__device__ void item_jcopy(Slot* sp, uint32_t tile, uint64_t V) {
+  const uint64_t v0 = (uint64_t)tile * kStarTV;
+  const uint64_t v1 = v0 + kStarTV < V ? v0 + kStarTV : V;
+  const uint64_t Lp = (uint64_t)sp->Lp;
+  const uint64_t rs = (uint64_t)sp->ND * Lp;
+  const int nch = (int)(Lp / kJobChunkLabels);
+  const uint64_t from = (uint64_t)(sp->level - 1) * Lp, to = (uint64_t)sp->level * Lp;
+  for (uint64_t v = v0; v < v1; v++) {
+    g_u32* row = (g_u32*)(sp->dist + v * rs);
+    for (int c = lane_id(); c < nch; c += 64) {
+      *(g_chunk*)(row + to + (uint64_t)c * kJobChunkLabels) =
+          *(g_cchunk*)(row + from + (uint64_t)c * kJobChunkLabels);
+    }
+  }
+}
+
 // Job STAR: the star bound min_v sum_i c_i(v) of every member query, from
 // its k labels of each distance row (one lane per member, at most two members
 // per lane since a job has at most 128; the wave reads each row once,
 // coalesced), and the list of vertices where some member has every group at
 // a finite distance.  A vertex off that list cannot be live for any member,
-// so the members' LIVE passes examine only the list.
+// so the members' LIVE passes examine only the list.  A hop job's height d
+// holds the distance within d hops, so height floor(D/2) gives the star arms
+// and height D the reach.
 __device__ void item_jstar(Slot* slots, Slot* sp, uint32_t tile, uint64_t V) {
   const int lane = lane_id();
   const uint64_t v0 = (uint64_t)tile * kStarTV;
@@ -730,37 +756,6 @@ __device__ void item_jstar(Slot* slots, Slot* sp, uint32_t tile, uint64_t V) {
   const uint32_t lab0 = q0 ? (uint32_t)q0->lab : 0u, lab1 = q1 ? (uint32_t)q1->lab : 0u;
   uint32_t star0 = kInf, star1 = kInf;
   uint64_t reached = 0;
-  if (ND > 1) {
-    // a hop job's rows become cumulative: height d holds the distance within d
-    // hops (walks of at most d arcs), which is what the members read as C and
-    // as their singleton states
-    const int nch = (int)(Lp / kJobChunkLabels);
-    for (uint64_t v = v0; v < v1; v++) {
-      g_u32* row = (g_u32*)(sp->dist + v * rs);
-      for (int c = lane; c < nch; c += 64) {
-        jchunk run = *(g_cchunk*)(row + (uint64_t)c * kJobChunkLabels);
-        for (int d = 1; d < ND; d++) {
-          g_u32* w = row + (uint64_t)d * Lp + (uint64_t)c * kJobChunkLabels;
-          jchunk x = *(g_cchunk*)w;
-          bool lower = false;
-#pragma unroll
-          for (int q = 0; q < kJobChunkLabels; q++) {
-            if (run[q] < x[q]) {
-              x[q] = run[q];
-              lower = true;
-            }
-          }
-          if (lower) {
-#pragma unroll
-            for (int q = 0; q < kJobChunkLabels; q++) w[q] = x[q];
-          }
-          run = x;
-        }
-      }
-    }
-    __threadfence();  // the rows just written are read below by other lanes
-    wave_sync();
-  }
   for (uint64_t v = v0; v < v1; v++) {
     const uint32_t* row = sp->dist + v * rs;
     bool fin = false;
@@ -1416,21 +1411,34 @@ __device__ void advance_level(Slot* s) {
 
 __device__ void complete_step(Slot* slots, Slot* s, Ctrl* ctrl) {
   const uint32_t r = s->step;
+  bool keep_frontier = false;
   if (s->kind == KIND_JOB) {
     switch (s->phase) {
       case PH_FILL:
         s->phase = PH_SEED;
         break;
       case PH_SEED:
-        // a hop job computes heights 1..D, one round each
+        // a hop job computes heights 1..D, each a COPY and one round
         s->level = 1;
-        s->phase = (s->D == 0) ? PH_STAR : PH_ROUND;
+        s->phase = (s->D < 0) ? PH_ROUND : (s->D == 0 ? PH_STAR : PH_COPY);
+        break;
+      case PH_COPY:
+        // the round's frontier is the one the previous round appended: the
+        // step index (which selects the frontier buffers) stays
+        s->phase = PH_ROUND;
+        keep_frontier = true;
         break;
       case PH_ROUND:
         s->ctr[1] += (unsigned long long)(s->app[r % 3] >> 32);
         s->ctr[2] += (unsigned long long)(uint32_t)s->app[r % 3];
-        if ((s->app[(r + 1) % 3] >> 32) == 0 || (s->D >= 0 && s->level >= s->D)) s->phase = PH_STAR;
-        else if (s->D >= 0) s->level++;
+        if (s->D < 0) {
+          if ((s->app[(r + 1) % 3] >> 32) == 0) s->phase = PH_STAR;
+        } else if (s->level >= s->D) {
+          s->phase = PH_STAR;
+        } else {
+          s->level++;
+          s->phase = PH_COPY;  // every height is written, also after an empty round
+        }
         break;
       case PH_STAR:
         for (int m = 0; m < s->nmem; m++) {
@@ -1488,8 +1496,8 @@ __device__ void complete_step(Slot* slots, Slot* s, Ctrl* ctrl) {
         break;
     }
   }
-  s->app[(r + 2) % 3] = 0;
-  s->step = r + 1;
+  s->app[(r + 2) % 3] = 0;  // cleared by this step's CLEAR items
+  if (!keep_frontier) s->step = r + 1;
   s->ctr[0]++;
 }
 
@@ -1512,7 +1520,8 @@ __device__ uint32_t plan_step(Slot* s, uint64_t V) {
       work = ceil_div((uint32_t)s->app[r % 3], (uint32_t)s->chunk);
       if (!job) cls = kClsHeavy;  // the per-query grow carries the lower-bound code
       break;
-    case PH_STAR: work = ceil_div(V, kStarTV); break;
+    case PH_STAR:
+    case PH_COPY: work = ceil_div(V, kStarTV); break;
     case PH_DIST: work = (s->S1 ? ceil_div(V, kDistTV) : 0u) + 1u; cls = kClsHeavy; break;
     case PH_LIVE: work = ceil_div(s->vsrc ? (uint64_t)s->nvsrc : V, kDistTV); cls = kClsHeavy; break;
     case PH_MERGE: work = ceil_div(s->level >= 3 ? s->nact : s->nlive, kMergeTV); cls = kClsHeavy; break;
@@ -1673,6 +1682,7 @@ __device__ void step_body(const GraphDev& g, Slot* slots, Ctrl* ctrl, LDS* L) {
             case PH_SEED: item_jseed(g, sp, local); break;
             case PH_ROUND: item_jround(g, sp, local, (FrontLDS*)L); break;
             case PH_STAR: item_jstar(slots, sp, local, V); break;
+            case PH_COPY: item_jcopy(sp, local, V); break;
             default: break;
           }
         } else {

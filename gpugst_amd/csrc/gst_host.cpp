@@ -553,7 +553,7 @@ class Engine {
     finished->clear();
     if (nslots_used_ == 0) return GST_OK;
     step_before_.resize(max_slots_);
-    for (int i = 0; i < nslots_used_; i++) step_before_[i] = h_slots_[i].step;
+    for (int i = 0; i < nslots_used_; i++) step_before_[i] = (uint32_t)h_slots_[i].ctr[0];
     const bool use_graph = g_->opt.graph != 0;
     const bool timed = g_->opt.time_steps != 0 && !use_graph;
     for (auto& u : pending_uploads_)
@@ -645,7 +645,7 @@ class Engine {
     // progress guard: every epoch runs at least one step of some admitted slot
     bool any_step = false;
     for (int i = 0; i < nslots_used_; i++)
-      if (sh_[i].used && (h_slots_[i].step != step_before_[i] || h_slots_[i].phase == gst::PH_DONE))
+      if (sh_[i].used && ((uint32_t)h_slots_[i].ctr[0] != step_before_[i] || h_slots_[i].phase == gst::PH_DONE))
         any_step = true;
     if (!any_step)
       return fail(GST_DEVICE_ERROR, "engine epoch made no progress on %d admitted queries", active_);
