@@ -45,13 +45,15 @@ def lean_bytes(g, res_list, V, ks, ND=1):
     """Algorithmic bytes of the lean kernel's items for these queries (one
     group-distance job per up to 256 labels; ND = D + 1 rows per vertex for a
     hop job, 1 otherwise):
-      job FILL        4 B per (vertex, label, height) + 24 B per vertex (3 masks)
+      job FILL        4 B per (vertex, label) of height 0 + 24 B per vertex (3 masks)
       job ROUND       per frontier entry 24 B (list id, arc offset, mask, row
                       pointer) + 12 B clear; per arc 8 B (column + weight);
                       per 4-label chunk relaxed 16 B (target chunk read); per
                       improved chunk 16 B (write back)
-      job STAR        4 B per (vertex, label, height) read; a hop job writes
-                      its rows cumulative, 4 B per (vertex, label, height > 0)
+      job COPY        a hop job's heights 1..D: 4 B read + 4 B written per
+                      (vertex, label, height)
+      job STAR        4 B per (vertex, label) read, from one row (unbounded)
+                      or two (hop: heights floor(D/2) and D)
       query FILL      24 B per vertex (3 masks) + 1/4 B live and hold bits"""
     labels = int(sum(ks))
     jobs = max(1, g.get_option("batch_jobs"))
@@ -60,9 +62,10 @@ def lean_bytes(g, res_list, V, ks, ND=1):
     arcs = g.get_option("batch_job_arcs")
     relax = g.get_option("batch_job_relaxations")
     impr = g.get_option("batch_job_improvements")
-    b = 4 * V * lp * ND + 24 * V * jobs
+    b = 4 * V * lp + 24 * V * jobs
     b += 36 * ent + 8 * arcs + 16 * (relax // 4) + 16 * impr
-    b += 4 * V * lp * ND + 4 * V * lp * (ND - 1)
+    b += 8 * V * lp * (ND - 1)
+    b += 4 * V * lp * (1 if ND == 1 else 2)
     b += (24 * V + V // 4) * len(res_list)
     return b
 
@@ -70,10 +73,12 @@ def lean_bytes(g, res_list, V, ks, ND=1):
 def heavy_bytes(res_list, V, ks, ND=1):
     """Algorithmic bytes of the heavy kernel's items, per query: the live test
     (C row, 4 B per vertex and group), the T segments of live vertices written
-    once (8 B per state), the level-2 merge reading the singletons and C rows
-    of every live vertex (12 B per vertex, group and height), the later merges
-    and FINAL reading the T segments of the active vertices (8 B per state and
-    pass), and the level >= 2 grow rounds (24 B per frontier entry, 8 B per arc,
+    once (8 B per state; NS = ceil(D/2) + 1 = ND // 2 + 1 state heights for a
+    hop query, 1 otherwise), the level-2 merge reading the
+    singletons (8 B per vertex, group and state height) and C rows (4 B per
+    vertex, group and row) of every live vertex, the later merges and FINAL
+    reading the T segments of the active vertices (8 B per state and pass),
+    and the level >= 2 grow rounds (24 B per frontier entry, 8 B per arc,
     16 B per relaxation candidate, 8 B per improvement) -- the counters of a
     query fed by a job hold only these rounds."""
     total = 0
@@ -83,7 +88,8 @@ def heavy_bytes(res_list, V, ks, ND=1):
         nl = c.get("live_vertices", 0)
         na = c.get("active_vertices", 0)
         passes = max(k - 3, 0) + 1  # merges of levels 3..k-1, then FINAL
-        total += 4 * V * k + nl * W * ND * 8 + nl * k * ND * 12 + na * W * ND * 8 * passes
+        NS = ND // 2 + 1
+        total += 4 * V * k + nl * W * NS * 8 + nl * k * (8 * NS + 4 * ND) + na * W * NS * 8 * passes
         total += 24 * c["frontier_entries"] + 8 * c["arcs"] + 16 * c["relaxations"] + 8 * c["improvements"]
     return total
 

@@ -165,7 +165,7 @@ __device__ __forceinline__ States states_of(const Slot* sp) {
 
 // Row of vertex v in the distance table C.  A hop query's table holds D + 1
 // cumulative rows per vertex, row h the group distances within h hops (its own
-// table, or its job's rows made cumulative at STAR); an unbounded query has
+// table, or its job's cumulative rows); an unbounded query has
 // one row (the job's row for a job-fed query).
 __device__ __forceinline__ g_cu32* crow(const Slot* sp, uint64_t v, int h) {
   return (g_cu32*)sp->C + v * sp->cstride + (sp->D >= 0 ? (uint64_t)h * (uint64_t)sp->chs : 0);
