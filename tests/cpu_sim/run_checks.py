@@ -3,8 +3,9 @@
 Started by tests/test_engine_cpu.py in its own process with GST_LIB pointing
 at tests/cpu_sim/_build/libgst_cpu.so.  Prints one JSON line of results.
   tiny   : golden tiny cases (every `stride`-th), every D, one call each and
-           all D variants in one batch; costs against the fixture, trees
-           validated
+           all D variants in one batch; for the cases just before those, each
+           D twice in a row (the pairs share a group-distance job); costs
+           against the fixture, trees validated
   jobs   : one batch of random k=5 queries (duplicates allowed) over several
            group-distance jobs, against the oracle and against group_pass=0
   graph  : engine steps replayed from a captured graph against eager launches
@@ -41,26 +42,40 @@ def groups_csr(groups):
 def check_tiny(stride):
     cases = json.load(open(os.path.join(ROOT, "tests", "golden", "tiny.json")))
     bad, n = [], 0
-    for ci in range(0, len(cases), stride):
+    for ci in range(len(cases)):
+        full, shared = ci % stride == 0, ci % stride == stride - 1
+        if not (full or shared):
+            continue
         c = cases[ci]
         rp, col, wt = data.csr_from_edges(c["V"], c["edges"])
         g = Graph(rp, col, wt)
         g.set_groups(*groups_csr(c["groups"]))
         ids = list(range(len(c["groups"])))
         todo = [(-1, c["cost"])] + [(int(D), hc) for D, hc in c["hop"].items()]
-        for D, exp in todo:
-            r = g.solve(ids, D=D)
-            n += 1
-            ok = r.cost == exp
-            if ok and r.feasible:
-                ok = validate_tree(rp, col, wt, c["groups"], r.cost, r.root, r.edges, D=D)["ok"]
-            if not ok:
-                bad.append([ci, D, r.cost, exp])
-        rs = g.solve_batch([ids] * len(todo), [d for d, _ in todo])
-        for (D, exp), r in zip(todo, rs):
-            n += 1
-            if r.cost != exp:
-                bad.append([ci, D, r.cost, exp, "batch"])
+        if full:
+            for D, exp in todo:
+                r = g.solve(ids, D=D)
+                n += 1
+                ok = r.cost == exp
+                if ok and r.feasible:
+                    ok = validate_tree(rp, col, wt, c["groups"], r.cost, r.root, r.edges, D=D)["ok"]
+                if not ok:
+                    bad.append([ci, D, r.cost, exp])
+            rs = g.solve_batch([ids] * len(todo), [d for d, _ in todo])
+            for (D, exp), r in zip(todo, rs):
+                n += 1
+                if r.cost != exp:
+                    bad.append([ci, D, r.cost, exp, "batch"])
+        if shared:
+            # each bound twice in a row: the pairs share a group-distance job,
+            # and on these small graphs a hop job's frontier empties before its
+            # last height
+            rs = g.solve_batch([ids] * (2 * len(todo)), [d for d, _ in todo for _ in (0, 1)])
+            for i, r in enumerate(rs):
+                D, exp = todo[i // 2]
+                n += 1
+                if r.cost != exp:
+                    bad.append([ci, D, r.cost, exp, "job"])
         g.close()
     return {"checked": n, "bad": bad}
 

@@ -52,6 +52,13 @@ def test_tiny_golden(golden_dir):
         _check(rp, col, wt, c["groups"], g.solve(ids), c["cost"])
         for D, hc in c["hop"].items():
             _check(rp, col, wt, c["groups"], g.solve(ids, D=int(D)), hc, D=int(D))
+        # each bound twice in a row: the pairs share a group-distance job, and on
+        # these small graphs a hop job's frontier empties before its last height
+        todo = [(-1, c["cost"])] + [(int(D), hc) for D, hc in c["hop"].items()]
+        rs = g.solve_batch([ids] * (2 * len(todo)), [d for d, _ in todo for _ in (0, 1)])
+        for i, r in enumerate(rs):
+            D, hc = todo[i // 2]
+            _check(rp, col, wt, c["groups"], r, hc, D=D)
         g.close()
 
 
