@@ -55,6 +55,7 @@ def test_tiny_golden(golden_dir):
         # each bound twice in a row: the pairs share a group-distance job, and on
         # these small graphs a hop job's frontier empties before its last height
         todo = [(-1, c["cost"])] + [(int(D), hc) for D, hc in c["hop"].items()]
+        g.set_option("budget_mb", 16)  # the batch engine's state arena (default: most of the HBM)
         rs = g.solve_batch([ids] * (2 * len(todo)), [d for d, _ in todo for _ in (0, 1)])
         for i, r in enumerate(rs):
             D, hc = todo[i // 2]
