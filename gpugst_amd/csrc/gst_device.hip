@@ -499,17 +499,30 @@ __device__ void item_round(const GraphDev& g, Slot* sp, uint32_t chunk, FrontLDS
     const uint32_t wend = (i + 64 < n) ? lof[i + 64] : arcs_total;
     wave_sync();
     const uint32_t hia = a1 < wend ? a1 : wend;
+    // the arc of the lane's next iteration is read while this one is grown
+    int no = 0;
+    uint32_t ne = 0, nu = 0, nw = 0;
+    if (a + lane < hia) {
+      no = find_entry(L, a + lane);
+      ne = L->wrs[no] + (a + lane - L->woff[no]);
+      nu = (uint32_t)g.col[ne];
+      nw = (uint32_t)g.wt[ne];
+    }
     for (uint32_t base = a; base < hia; base += 64) {
       const uint32_t pos = base + lane;
       bool app = false, held = false;
-      uint32_t u = 0;
+      const int o = no;
+      const uint32_t e = ne, w = nw;
+      uint32_t u = pos < hia ? nu : 0u;
+      if (pos + 64 < hia) {
+        no = find_entry(L, pos + 64);
+        ne = L->wrs[no] + (pos + 64 - L->woff[no]);
+        nu = (uint32_t)g.col[ne];
+        nw = (uint32_t)g.wt[ne];
+      }
       if (pos < hia) {
-        const int o = find_entry(L, pos);
         const uint32_t vs = L->wv[o];
         const uint64_t m = L->wm[o];
-        const uint32_t e = L->wrs[o] + (pos - L->woff[o]);
-        u = (uint32_t)g.col[e];
-        const uint32_t w = (uint32_t)g.wt[e];
         uint32_t revarc = 0xFFFFFFFFu;
         uint64_t improved = 0;
         // above level 1 only live targets can hold useful states
