@@ -679,17 +679,30 @@ __device__ void item_jround(const GraphDev& g, Slot* sp, uint32_t item, FrontLDS
     const uint32_t wend = (i + 64 < n) ? lof[i + 64] : arcs_total;
     wave_sync();
     const uint32_t hia = a1 < wend ? a1 : wend;
+    // the arc of the group's next iteration is read while this one is relaxed
+    int no = 0;
+    uint32_t nu = 0, nw = 0;
+    if (a + (uint32_t)sub < hia) {
+      no = find_entry(L, a + (uint32_t)sub);
+      const uint32_t e = L->wrs[no] + (a + (uint32_t)sub - L->woff[no]);
+      nu = (uint32_t)g.col[e];
+      nw = (uint32_t)g.wt[e];
+    }
     for (uint32_t base = a; base < hia; base += (uint32_t)APW) {
       const uint32_t pos = base + (uint32_t)sub;
       uint64_t imp = 0;
-      uint32_t u = 0;
+      const int o = no;
+      const uint32_t w = nw;
+      uint32_t u = pos < hia ? nu : 0u;
+      if (pos + (uint32_t)APW < hia) {
+        no = find_entry(L, pos + (uint32_t)APW);
+        const uint32_t e = L->wrs[no] + (pos + (uint32_t)APW - L->woff[no]);
+        nu = (uint32_t)g.col[e];
+        nw = (uint32_t)g.wt[e];
+      }
       if (pos < hia) {
-        const int o = find_entry(L, pos);
         const uint32_t vs = L->wv[o];
         const uint64_t m = L->wm[o];
-        const uint32_t e = L->wrs[o] + (pos - L->woff[o]);
-        u = (uint32_t)g.col[e];
-        const uint32_t w = (uint32_t)g.wt[e];
         g_cchunk* srow = (g_cchunk*)(sdist + (uint64_t)vs * rs);
         g_cchunk* trow = (g_cchunk*)(tdist + (uint64_t)u * rs);
         g_u32* tw = (g_u32*)(tdist + (uint64_t)u * rs);
