@@ -120,6 +120,15 @@ typedef GST_AS_GLOBAL uint32_t g_u32;
 typedef GST_AS_GLOBAL const uint32_t g_cu32;
 typedef GST_AS_GLOBAL unsigned long long g_u64;
 
+// A store that is not read again soon (non-temporal: kept out of the caches).
+__device__ __forceinline__ void store_stream(uint64_t* p, uint64_t v) {
+#ifdef GST_CPU_SIM
+  *p = v;
+#else
+  __builtin_nontemporal_store(v, (GST_AS_GLOBAL uint64_t*)p);
+#endif
+}
+
 // State (v, p, d): singletons in the compact S1 table (heights 0..D) or,
 // when the slot's distances come from a group-distance job, synthesised from
 // the job's rows with the kPredPath predecessor; every other subset in the
@@ -991,7 +1000,8 @@ __device__ void item_live(Slot* sp, uint32_t tile, uint64_t V, WaveLDS* L) {
     wave_sync();
     for (int t = lane; t < nl * rowlen; t += 64) {
       const int jj = t / rowlen;
-      sp->T[(uint64_t)L->f.wv[jj] * rowlen + (t - jj * rowlen)] = kInfState;
+      // not read again before the merges above level 2
+      store_stream(&sp->T[(uint64_t)L->f.wv[jj] * rowlen + (t - jj * rowlen)], kInfState);
     }
     wave_sync();
   }
