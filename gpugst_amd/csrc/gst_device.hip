@@ -128,6 +128,13 @@ __device__ __forceinline__ void store_stream(uint64_t* p, uint64_t v) {
   __builtin_nontemporal_store(v, (GST_AS_GLOBAL uint64_t*)p);
 #endif
 }
+__device__ __forceinline__ void store_stream(g_chunk* p, jchunk v) {
+#ifdef GST_CPU_SIM
+  *p = v;
+#else
+  __builtin_nontemporal_store(v, p);
+#endif
+}
 
 // State (v, p, d): singletons in the compact S1 table (heights 0..D) or,
 // when the slot's distances come from a group-distance job, synthesised from
@@ -401,7 +408,7 @@ __device__ void item_fill(Slot* sp, uint32_t idx, uint64_t V) {
   const uint64_t hw = (uint64_t)sp->Lp / 2, rw = (uint64_t)sp->ND * hw;
   for (uint64_t t = base + lane_id(); t < end; t += 64) {
     if (t < nA)
-      A[rows ? (t / hw) * rw + t % hw : t] = aval;
+      store_stream(&A[rows ? (t / hw) * rw + t % hw : t], aval);  // read back row by row, much later
     else if (t < nA + nM)
       M[t - nA] = 0;
     else
@@ -761,8 +768,8 @@ __device__ void item_jcopy(Slot* sp, uint32_t tile, uint64_t V) {
   for (uint64_t v = v0; v < v1; v++) {
     g_u32* row = (g_u32*)(sp->dist + v * rs);
     for (int c = lane_id(); c < nch; c += 64) {
-      *(g_chunk*)(row + to + (uint64_t)c * kJobChunkLabels) =
-          *(g_cchunk*)(row + from + (uint64_t)c * kJobChunkLabels);
+      store_stream((g_chunk*)(row + to + (uint64_t)c * kJobChunkLabels),
+                   *(g_cchunk*)(row + from + (uint64_t)c * kJobChunkLabels));
     }
   }
 }
