@@ -10,7 +10,9 @@ Run from the repository root:  python tests/golden/make_golden.py
   reduction.  Needs /root/reference (read here only, never at test time).
 * tiny.json -- random graphs with V <= 11 whose unbounded optimum is pinned
   by brute force over vertex subsets AND the Dreyfus-Wagner reduction AND the
-  oracle, plus hop-bounded optima from the oracle.
+  oracle, plus hop-bounded optima (D = 0..4) from the oracle, each equal to the
+  center formulation (oracle.hop_center_dp) and no more than the cheapest
+  proper subtree of that diameter (oracle.brute_force_hop_trees).
 * c1_costs.json -- optimal costs of the 50 queries of the V=20k Chung-Lu
   configuration, from the oracle, with a checksum of the generated instance so
   generator drift is caught.
@@ -73,6 +75,12 @@ def make_tiny(n=60, seed=7):
         c_dw = O.gst_via_steiner(rp, col, wt, groups)
         assert c_or == c_bf == c_dw, (el, groups, c_or, c_bf, c_dw)
         hop = {str(D): O.dpbf(rp, col, wt, groups, D=D)[0] for D in range(0, 5)}
+        # the hop values are pinned by the center formulation, which does not
+        # share the oracle's rooted recurrence, and bounded by proper trees
+        bf = O.brute_force_hop_trees(V, el, groups, 4)
+        for D, hc in hop.items():
+            assert hc == O.hop_center_dp(V, el, groups, int(D)), (el, groups, D)
+            assert hc <= bf[int(D)], (el, groups, D)
         cases.append(dict(V=V, edges=el, groups=groups, cost=c_or, hop=hop))
     return cases
 
