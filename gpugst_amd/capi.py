@@ -32,7 +32,7 @@ STATUS_NAMES = {0: "OK", 1: "INVALID_ARG", 2: "OUT_OF_MEMORY", 3: "DEVICE_ERROR"
 
 # every function include/gst.h declares
 EXPORTED = ("gst_graph_create", "gst_graph_destroy", "gst_groups_set", "gst_solve",
-            "gst_solve_batch", "gst_result_free", "gst_set_option", "gst_get_option",
+            "gst_solve_static", "gst_solve_batch", "gst_result_free", "gst_set_option", "gst_get_option",
             "gst_query_bytes", "gst_server_create", "gst_submit", "gst_poll", "gst_wait",
             "gst_server_destroy", "gst_last_error", "gst_version")
 
@@ -94,6 +94,7 @@ def load():
         "gst_groups_set": [P, P, P, I64],
         "gst_solve": [P, P, I32, I32, ctypes.POINTER(c_result)],
         "gst_solve_batch": [P, I64, P, P],
+        "gst_solve_static": [P, P, I32, I32, P, I32, ctypes.POINTER(c_result)],
         "gst_set_option": [P, ctypes.c_char_p, I64],
         "gst_get_option": [P, ctypes.c_char_p, ctypes.POINTER(I64)],
         "gst_query_bytes": [P, I32, I32, ctypes.POINTER(I64)],
@@ -198,6 +199,21 @@ class Graph:
         r = c_result()
         _check(load().gst_solve(self._h, _ptr(ids), ids.shape[0], int(D), ctypes.byref(r)))
         return _take(r)
+
+    def solve_static(self, group_ids: Sequence[int], D: int = UNBOUNDED,
+                     edge_cap: int = 4096) -> Result:
+        """gst_solve_static: the same answer as solve() with no device
+        allocation (state in the library's static scratch); the tree comes
+        back through a caller-side buffer of edge_cap records."""
+        ids = np.ascontiguousarray(group_ids, dtype=np.int32)
+        buf = np.zeros((max(edge_cap, 1), 4), dtype=np.int32)
+        r = c_result()
+        _check(load().gst_solve_static(self._h, _ptr(ids), ids.shape[0], int(D), _ptr(buf),
+                                       int(edge_cap), ctypes.byref(r)))
+        c = r.counters
+        return Result(cost=r.cost, root=r.root, edges=buf[:r.n_edges].copy(), status=r.status,
+                      counters={f: getattr(c, f) for f, _ in c_counters._fields_},
+                      latency_ms=r.latency_ms, device_ms=r.device_ms)
 
     def solve_batch(self, queries: Sequence[Sequence[int]],
                     D: Optional[Sequence[int]] = None) -> List[Result]:

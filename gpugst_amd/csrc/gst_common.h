@@ -10,6 +10,7 @@
 // code/D-GPUGST/include/DPQ.cuh:51-58) and writes cost and predecessor with two
 // separate, racing stores; here both live in one word updated by one atomicMin.
 #pragma once
+#include <stddef.h>
 #include <stdint.h>
 
 namespace gst {
@@ -20,6 +21,11 @@ constexpr int kMaxK = 6;                // W = 2^k <= 64: one wave lane per subs
 constexpr int kMaxW = 1 << kMaxK;
 constexpr int kMaxND = 16;              // D <= 15
 constexpr int kMaxSlots = 1024;         // concurrently admitted queries per engine
+// Static device scratch of the allocation-free entry point gst_solve_static:
+// its slot table, control block and one query's state arena.  Every GPU the
+// library runs on holds one copy of it until the process ends.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+constexpr int kMaxDevices = 64;
 
 // pred word: bits 31..30 type; grow: bits 29..0 = index of the arc v->u in the
 // row of the state's own vertex v (so u and w(u,v) are one load away during

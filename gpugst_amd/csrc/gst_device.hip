@@ -39,6 +39,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
+
 #include "gst_common.h"
 
 namespace gst {
@@ -1808,3 +1810,35 @@ extern "C" hipError_t gst_step_occupancy(int cls, int* blocks_per_cu) {
   return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, f, gst::kBlock, 0);
 }
 #endif  // GST_CPU_SIM
+
+// ------------------------------------------------------- static scratch ----
+// Device memory of the allocation-free entry point (gst_solve_static): the
+// slot table, the control block and one query's state arena are carved out of
+// this file-scope array, so that path calls no allocator.  Each GPU holds its
+// own copy at its own address.  In host code the array's name is not a device
+// address; gst_scratch_setup resolves the address once per device (with that
+// device current) and every host use goes through the pointer it returns.
+namespace gst {
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+}  // namespace gst
+
+extern "C" hipError_t gst_scratch_setup(int device, void** ptr) {
+  static std::mutex mu;
+  static void* addr[gst::kMaxDevices] = {};
+  if (device < 0 || device >= gst::kMaxDevices || !ptr) return hipErrorInvalidValue;
+  std::lock_guard<std::mutex> lk(mu);
+  if (!addr[device]) {
+    int cur = 0;
+    hipError_t e = hipGetDevice(&cur);
+    if (e != hipSuccess) return e;
+    if (cur != device && (e = hipSetDevice(device)) != hipSuccess) return e;
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(gst::scratch));
+    if (cur != device) (void)hipSetDevice(cur);
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidValue;
+    addr[device] = p;
+  }
+  *ptr = addr[device];
+  return hipSuccess;
+}

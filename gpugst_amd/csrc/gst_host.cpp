@@ -28,6 +28,7 @@ extern "C" hipError_t gst_launch_step(int cls, const gst::GraphDev* g, gst::Slot
                                       int grid, hipStream_t stream);
 extern "C" hipError_t gst_launch_plan(gst::Slot* slots, gst::Ctrl* ctrl, uint64_t V, hipStream_t stream);
 extern "C" hipError_t gst_step_occupancy(int cls, int* blocks_per_cu);
+extern "C" hipError_t gst_scratch_setup(int device, void** ptr);
 
 namespace {
 
@@ -288,11 +289,16 @@ class Engine {
   Engine(gst_graph* g, int max_slots) : g_(g), max_slots_(std::min(max_slots, gst::kMaxSlots)) {}
   ~Engine() {
     if (stream_) (void)hipStreamSynchronize(stream_);
-    if (arena_mem_) (void)hipFree(arena_mem_);
-    if (d_slots_) (void)hipFree(d_slots_);
-    if (d_ctrl_) (void)hipFree(d_ctrl_);
-    if (h_slots_) (void)hipHostFree(h_slots_);
-    if (h_ctrl_) (void)hipHostFree(h_ctrl_);
+    if (borrowed_) {
+      delete[] h_slots_;
+      delete h_ctrl_;
+    } else {
+      if (arena_mem_) (void)hipFree(arena_mem_);
+      if (d_slots_) (void)hipFree(d_slots_);
+      if (d_ctrl_) (void)hipFree(d_ctrl_);
+      if (h_slots_) (void)hipHostFree(h_slots_);
+      if (h_ctrl_) (void)hipHostFree(h_ctrl_);
+    }
     drop_graph();
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);
@@ -316,7 +322,45 @@ class Engine {
     return resize_arena(arena_bytes);
   }
 
+  // An engine over a device region it does not own (the static scratch of
+  // gst_solve_static): slot table, control block and arena are carved out of
+  // [base, base + bytes); the host mirrors are plain host memory.  Nothing
+  // here or later allocates device memory.  Every query it admits runs its own
+  // level 1 (no group-distance job).
+  gst_status init_static(char* base, size_t bytes) {
+    borrowed_ = true;
+    force_own_ = true;
+    HIP_TRY(hipSetDevice(g_->device));
+    HIP_TRY(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreate(&ev0_));
+    HIP_TRY(hipEventCreate(&ev1_));
+    size_t off = 0;
+    d_slots_ = (gst::Slot*)(base + off);
+    off += align256(sizeof(gst::Slot) * max_slots_);
+    d_ctrl_ = (gst::Ctrl*)(base + off);
+    off += align256(sizeof(gst::Ctrl));
+    if (off >= bytes) return fail(GST_OUT_OF_MEMORY, "static scratch too small for the engine tables");
+    h_slots_ = new gst::Slot[max_slots_];
+    h_ctrl_ = new gst::Ctrl;
+    memset(h_slots_, 0, sizeof(gst::Slot) * max_slots_);
+    memset(h_ctrl_, 0, sizeof(gst::Ctrl));
+    HIP_TRY(hipMemcpyAsync(d_ctrl_, h_ctrl_, sizeof(gst::Ctrl), hipMemcpyHostToDevice, stream_));
+    HIP_TRY(hipStreamSynchronize(stream_));
+    sh_.assign(max_slots_, SlotHost());
+    arena_mem_ = base + off;
+    arena_.reset((bytes - off) & ~size_t(255));
+    return GST_OK;
+  }
+
+  // Points an idle engine at another graph of the same device (the static
+  // engine serves every graph handle of its device, one call at a time).
+  void rebind(gst_graph* g) {
+    if (g_ != g) drop_graph();  // a captured chunk holds the old graph's arguments
+    g_ = g;
+  }
+
   gst_status resize_arena(size_t bytes) {
+    if (borrowed_) return fail(GST_INVALID_ARG, "the static engine's arena is fixed");
     if (active_ > 0) return fail(GST_INVALID_ARG, "arena resize with active queries");
     if (arena_mem_) {
       HIP_TRY(hipFree(arena_mem_));
@@ -350,7 +394,7 @@ class Engine {
     while (!pend.empty()) {
       QueryReq* q0 = pend.front();
       const int k0 = (int)q0->groups.size();
-      if (!uses_job(g_, k0, q0->D)) {
+      if (force_own_ || !uses_job(g_, k0, q0->D)) {
         if (used_slots_ >= max_slots_ || admit_query(*q0, -1, 0) < 0) break;
         pend.pop_front();
         n++;
@@ -730,8 +774,10 @@ class Engine {
     gexec_ = nullptr;
   }
 
-  // Copies the result of a finished query out and releases its memory.
-  gst_status harvest(int s, gst_result* out, uint64_t* tag) {
+  // Copies the result of a finished query out and releases its memory.  The
+  // tree goes to a malloc'd buffer owned by `out`, or, with ext, into the
+  // caller's buffer of ext_cap edges (out->edges stays null then).
+  gst_status harvest(int s, gst_result* out, uint64_t* tag, gst_edge* ext = nullptr, int32_t ext_cap = 0) {
     const gst::Slot& d = h_slots_[s];
     SlotHost& h = sh_[s];
     memset(out, 0, sizeof(*out));
@@ -767,7 +813,20 @@ class Engine {
       out->n_edges = 0;
     } else {
       out->n_edges = d.out_nedges;
-      if (d.out_nedges > 0) {
+      if (ext) {
+        if (d.out_nedges > ext_cap) {
+          out->status = GST_INVALID_ARG;
+          st = fail(GST_INVALID_ARG, "the tree has %d edges; the edge buffer holds %d", d.out_nedges, ext_cap);
+        } else if (d.out_nedges > 0) {
+          hipError_t e = hipMemcpyAsync(ext, d.edges, sizeof(gst_edge) * (size_t)d.out_nedges,
+                                        hipMemcpyDeviceToHost, stream_);
+          if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+          if (e != hipSuccess) {
+            out->status = GST_DEVICE_ERROR;
+            st = fail(GST_DEVICE_ERROR, "edge copy failed: %s", hipGetErrorString(e));
+          }
+        }
+      } else if (d.out_nedges > 0) {
         out->edges = (gst_edge*)malloc(sizeof(gst_edge) * (size_t)d.out_nedges);
         if (!out->edges) return fail(GST_OUT_OF_MEMORY, "host edge buffer");
         hipError_t e = hipMemcpyAsync(out->edges, d.edges, sizeof(gst_edge) * (size_t)d.out_nedges,
@@ -835,6 +894,8 @@ class Engine {
  private:
   gst_graph* g_;
   int max_slots_;
+  bool borrowed_ = false;   // device tables live in memory the engine does not own
+  bool force_own_ = false;  // every query runs its own level 1
   hipStream_t stream_ = nullptr;
   gst::Slot* d_slots_ = nullptr;
   gst::Slot* h_slots_ = nullptr;
@@ -934,7 +995,8 @@ size_t single_query_bytes(const gst_graph* g, int k, int D) {
 
 // Drives an engine over a list of queries: admits while memory and slots
 // allow, runs epochs, harvests finished queries, refills.
-gst_status run_queries(gst_graph* g, Engine* eng, std::vector<QueryReq>& qs, gst_result* out) {
+gst_status run_queries(gst_graph* g, Engine* eng, std::vector<QueryReq>& qs, gst_result* out,
+                       gst_edge* ext = nullptr, int32_t ext_cap = 0) {
   std::deque<QueryReq*> pend;
   for (size_t i = 0; i < qs.size(); i++) {
     qs[i].tag = i;
@@ -959,12 +1021,26 @@ gst_status run_queries(gst_graph* g, Engine* eng, std::vector<QueryReq>& qs, gst
     for (int s : fin) {
       uint64_t tag = 0;
       gst_result r;
-      gst_status hs = eng->harvest(s, &r, &tag);
+      gst_status hs = eng->harvest(s, &r, &tag, ext, ext_cap);
       out[tag] = r;
       if (hs != GST_OK && first_err == GST_OK) first_err = hs;
     }
   }
   return first_err;
+}
+
+// The engine of gst_solve_static on each device: built over the static
+// scratch on the first call there and kept until the process ends (like the
+// scratch itself); calls on one device take its lock, so they never overlap.
+struct StaticDev {
+  std::mutex mu;
+  Engine* eng = nullptr;
+};
+StaticDev g_static[gst::kMaxDevices];
+
+// arena bytes of the static engine (2 slots: the scratch minus its tables)
+size_t static_arena_bytes() {
+  return gst::kScratchBytes - align256(2 * sizeof(gst::Slot)) - align256(sizeof(gst::Ctrl));
 }
 
 }  // namespace
@@ -1051,6 +1127,12 @@ gst_status gst_graph_create(const int64_t* beg_pos, int64_t V, const int32_t* ad
   }
   g->V = V;
   g->n_arcs = n_arcs;
+  {
+    // resolve this device's static scratch now (gst_solve_static reports a
+    // failure on its own call)
+    void* sp = nullptr;
+    (void)gst_scratch_setup(g->device, &sp);
+  }
   HIP_TRY(hipMalloc(&g->d_rowptr, sizeof(int64_t) * (V + 1)));
   HIP_TRY(hipMalloc(&g->d_col, sizeof(int32_t) * std::max<int64_t>(n_arcs, 1)));
   HIP_TRY(hipMalloc(&g->d_wt, sizeof(int32_t) * std::max<int64_t>(n_arcs, 1)));
@@ -1220,6 +1302,44 @@ gst_status gst_solve(gst_graph* g, const int32_t* group_ids, int32_t k, int32_t 
   qs[0].D = D;
   qs[0].t_submit = now_ms();
   return run_queries(g, eng, qs, out);
+}
+
+gst_status gst_solve_static(gst_graph* g, const int32_t* group_ids, int32_t k, int32_t D, gst_edge* edges,
+                            int32_t edge_cap, gst_result* out) {
+  if (!out) return fail(GST_INVALID_ARG, "null result");
+  memset(out, 0, sizeof(*out));
+  gst_status st = validate_query(g, group_ids, k, D);
+  if (st != GST_OK) return st;
+  if (edge_cap < 0 || (edge_cap > 0 && !edges)) return fail(GST_INVALID_ARG, "bad edge buffer");
+  if (g->device < 0 || g->device >= gst::kMaxDevices) return fail(GST_INVALID_ARG, "device %d", g->device);
+  if (has_empty_group(g, group_ids, k)) {
+    fill_infeasible(out);
+    return GST_OK;
+  }
+  const size_t need = layout_for(g, k, D, true).total;
+  if (need > static_arena_bytes())
+    return fail(GST_OUT_OF_MEMORY,
+                "a query of k=%d D=%d on %lld vertices needs %zu bytes of state; the static scratch holds "
+                "%zu (gst_solve allocates what it needs instead)",
+                k, D, (long long)g->V, need, static_arena_bytes());
+  std::lock_guard<std::mutex> lk(g->dev_mu);
+  StaticDev& sd = g_static[g->device];
+  std::lock_guard<std::mutex> sl(sd.mu);
+  if (!sd.eng) {
+    void* base = nullptr;
+    const hipError_t e = gst_scratch_setup(g->device, &base);
+    if (e != hipSuccess) return fail(GST_DEVICE_ERROR, "static scratch address: %s", hipGetErrorString(e));
+    std::unique_ptr<Engine> ne(new Engine(g, 2));
+    st = ne->init_static((char*)base, gst::kScratchBytes);
+    if (st != GST_OK) return st;
+    sd.eng = ne.release();
+  }
+  sd.eng->rebind(g);
+  std::vector<QueryReq> qs(1);
+  qs[0].groups.assign(group_ids, group_ids + k);
+  qs[0].D = D;
+  qs[0].t_submit = now_ms();
+  return run_queries(g, sd.eng, qs, out, edges, edge_cap);
 }
 
 gst_status gst_solve_batch(gst_graph* g, int64_t n, const gst_query* q, gst_result* out) {

@@ -126,6 +126,22 @@ gst_status gst_groups_set(gst_graph *g, const int64_t *grp_ptr,
 gst_status gst_solve(gst_graph *g, const int32_t *group_ids, int32_t k,
                      int32_t D, gst_result *out);
 
+/* gst_solve without any device allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * query's state lives in a static device array of the library (64 MiB per
+ * GPU, held until the process ends; a query whose state needs more -- about
+ * V * 2^k * 8 B plus ~80 B per vertex, times the state heights of a hop
+ * bound -- is rejected with GST_OUT_OF_MEMORY).  The tree is written to the
+ * caller's `edges` (edge_cap records; GST_INVALID_ARG if the tree has more,
+ * with out->n_edges set to its size); out->edges stays NULL.  Calls on one
+ * GPU share that array and are serialised by the library; it keeps the last
+ * call's contents, which the next call re-initialises.  Every query runs its
+ * own group-distance pass (no shared job). */
+gst_status gst_solve_static(gst_graph *g, const int32_t *group_ids, int32_t k,
+                            int32_t D, gst_edge *edges, int32_t edge_cap,
+                            gst_result *out);
+
 /* Many queries advanced together in shared engine steps, with memory-based
  * admission (continuous batching).  out[i] answers q[i]. */
 gst_status gst_solve_batch(gst_graph *g, int64_t n, const gst_query *q,

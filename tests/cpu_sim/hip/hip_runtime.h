@@ -15,6 +15,7 @@
 #define __host__
 #define __global__
 #define __forceinline__ inline
+#define __align__(x) __attribute__((aligned(x)))
 
 #define __ATOMIC_RELAXED_HIP __ATOMIC_SEQ_CST
 #define __HIP_MEMORY_SCOPE_WAVEFRONT 0
@@ -187,4 +188,11 @@ hipError_t hipGraphDestroy(hipGraph_t);
 hipError_t hipGraphExecDestroy(hipGraphExec_t);
 hipError_t hipGetLastError();
 const char* hipGetErrorString(hipError_t);
+// a file-scope "device" array is host memory here: its address is the symbol's
+#define HIP_SYMBOL(X) X
+template <class T>
+inline hipError_t hipGetSymbolAddress(void** p, const T& sym) {
+  *p = (void*)&sym;
+  return hipSuccess;
+}
 #endif

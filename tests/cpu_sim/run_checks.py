@@ -186,12 +186,55 @@ def check_graph():
     return {"expect": exp, "eager": eager, "graph": rep}
 
 
+def check_static():
+    """gst_solve_static (state in the library's static scratch, no device
+    allocation) against gst_solve and the oracle: queries on a 150-vertex
+    graph, then on a smaller graph (the same scratch, rebound), then the first
+    graph again; an edge buffer that is too small is an error."""
+    from gpugst_amd import GstError
+    rng, rp, col, wt, groups = random_instance(150, 17, deg=3, gmax=4)
+    g = Graph(rp, col, wt)
+    g.set_groups(*groups_csr(groups))
+    c = json.load(open(os.path.join(ROOT, "tests", "golden", "tiny.json")))[5]
+    trp, tcol, twt = data.csr_from_edges(c["V"], c["edges"])
+    t = Graph(trp, tcol, twt)
+    t.set_groups(*groups_csr(c["groups"]))
+    bad, n = [], 0
+    plan = [(g, rp, col, wt, groups, [rng.randrange(40) for _ in range(4)], -1),
+            (g, rp, col, wt, groups, [rng.randrange(40) for _ in range(3)], 3),
+            (t, trp, tcol, twt, c["groups"], list(range(len(c["groups"]))), -1),
+            (t, trp, tcol, twt, c["groups"], list(range(len(c["groups"]))), 2),
+            (g, rp, col, wt, groups, [rng.randrange(40) for _ in range(5)], -1)]
+    for gr, a, b, w, gs, q, D in plan:
+        r = gr.solve_static(q, D=D)
+        p = gr.solve(q, D=D)
+        gl = [gs[x] for x in q]
+        exp = O.dpbf(a, b, w, gl, D=D)[0]
+        ok = r.cost == p.cost == exp
+        if ok and r.feasible:
+            ok = validate_tree(a, b, w, gl, r.cost, r.root, r.edges, D=D)["ok"]
+        n += 1
+        if not ok:
+            bad.append([n, D, r.cost, p.cost, exp])
+    small_err = False
+    q = plan[0][5]
+    need = g.solve(q).edges.shape[0]
+    if need > 1:
+        try:
+            g.solve_static(q, edge_cap=need - 1)
+        except GstError as e:
+            small_err = e.status == 1
+    g.close()
+    t.close()
+    return {"checked": n, "bad": bad, "small_buffer_error": small_err or need <= 1}
+
+
 def main():
     stride = int(sys.argv[1]) if len(sys.argv) > 1 else 3
     nq = int(sys.argv[2]) if len(sys.argv) > 2 else 60
     nh = int(sys.argv[3]) if len(sys.argv) > 3 else 24
     out = {"tiny": check_tiny(stride), "jobs": check_jobs(150, nq, 7), "hop": check_hop(150, nh, 11),
-           "branching": check_branching(), "graph": check_graph()}
+           "branching": check_branching(), "graph": check_graph(), "static": check_static()}
     print(json.dumps(out), flush=True)
 
 

@@ -62,3 +62,9 @@ def test_graph_replay_on_cpu_build(cpu_results):
     r = cpu_results["graph"]
     assert r["eager"] == r["expect"]
     assert all(x == r["expect"] for x in r["graph"]), r
+
+
+def test_static_entry_on_cpu_build(cpu_results):
+    r = cpu_results["static"]
+    assert r["checked"] == 5 and r["bad"] == [], r
+    assert r["small_buffer_error"], r
