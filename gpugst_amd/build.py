@@ -4,8 +4,6 @@
   the host engine and the C ABI of ``include/gst.h`` (hipcc).
 * ``gpugst_amd/_lib/libgstdata.so`` -- host-only synthetic-data generators and
   CSR construction (g++).
-* ``gpugst_amd/_lib/copy_probe``    -- a plain copy kernel of known size, the
-  calibration point for rocprofv3's HBM byte counters.
 
 Run ``python -m gpugst_amd.build``.  Nothing here needs a GPU.
 """
@@ -22,7 +20,6 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "_lib")
 LIBGST = os.path.join(OUT, "libgst.so")
 LIBDATA = os.path.join(OUT, "libgstdata.so")
-COPY_PROBE = os.path.join(OUT, "copy_probe")
 
 ARCH = os.environ.get("GST_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
@@ -66,9 +63,6 @@ def build(verbose: bool = True) -> None:
     os.replace(tmp, LIBDATA)
     for o in objs:
         os.remove(o)
-    tmp = COPY_PROBE + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, os.path.join(CSRC, "copy_probe.hip"), "-o", tmp])
-    os.replace(tmp, COPY_PROBE)
 
 
 if __name__ == "__main__":
