@@ -34,7 +34,7 @@ STATUS_NAMES = {0: "OK", 1: "INVALID_ARG", 2: "OUT_OF_MEMORY", 3: "DEVICE_ERROR"
 EXPORTED = ("gst_graph_create", "gst_graph_destroy", "gst_groups_set", "gst_solve",
             "gst_solve_static", "gst_solve_batch", "gst_result_free", "gst_set_option", "gst_get_option",
             "gst_query_bytes", "gst_server_create", "gst_submit", "gst_poll", "gst_wait",
-            "gst_server_destroy", "gst_last_error", "gst_version")
+            "gst_cancel", "gst_server_destroy", "gst_last_error", "gst_version")
 
 
 class GstError(RuntimeError):
@@ -54,7 +54,8 @@ class c_counters(ctypes.Structure):
                 ("improvements", ctypes.c_int64), ("merge_evals", ctypes.c_int64),
                 ("live_vertices", ctypes.c_int64), ("shared_entries", ctypes.c_int64),
                 ("shared_arcs", ctypes.c_int64), ("shared_relaxations", ctypes.c_int64),
-                ("active_vertices", ctypes.c_int64)]
+                ("active_vertices", ctypes.c_int64), ("max_frontier", ctypes.c_int64),
+                ("states_finite", ctypes.c_int64)]
 
 
 class c_result(ctypes.Structure):
@@ -102,6 +103,7 @@ def load():
         "gst_submit": [P, ctypes.POINTER(c_query), ctypes.POINTER(ctypes.c_uint64)],
         "gst_poll": [P, ctypes.c_uint64, ctypes.POINTER(c_result)],
         "gst_wait": [P, ctypes.c_uint64, ctypes.POINTER(c_result)],
+        "gst_cancel": [P, ctypes.c_uint64],
         "gst_server_destroy": [P],
     }
     for name, args in sig.items():
@@ -227,7 +229,8 @@ class Graph:
         rs = (c_result * max(n, 1))()
         st = load().gst_solve_batch(self._h, n, qs, rs)
         out = [_take(rs[i]) for i in range(n)]
-        _check(st)
+        if st != GST_CANCELLED:  # cancelled queries carry their status in the results
+            _check(st)
         return out
 
     def close(self):
@@ -269,9 +272,14 @@ class Server:
         return _take(r)
 
     def wait(self, ticket: int) -> Result:
+        """The request's result; a cancelled request comes back with status
+        GST_CANCELLED (no exception)."""
         r = c_result()
         _check(load().gst_wait(self._h, ticket, ctypes.byref(r)))
         return _take(r)
+
+    def cancel(self, ticket: int):
+        _check(load().gst_cancel(self._h, ticket))
 
     def close(self):
         if getattr(self, "_h", None):

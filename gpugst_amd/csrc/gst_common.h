@@ -77,8 +77,14 @@ enum Phase : int32_t {
   PH_LIVE = 9,   // live-vertex list + initialisation of their T segments
   PH_WAITJ = 10, // query waiting for its group-distance job (no work items)
   PH_STAR = 11,    // job: star bound of every member query
-  PH_COPY = 12     // hop job: height level starts as a copy of height level - 1
+  PH_COPY = 12,    // hop job: height level starts as a copy of height level - 1
+  PH_COUNT = 13    // optional: count the finite states (statistics, after FINAL)
 };
+
+// Slot::status values set on the device
+constexpr uint32_t kStatusEdgeCap = 1;    // tree larger than the edge buffer
+constexpr uint32_t kStatusNoPath = 2;     // reconstruction found no shortest-path arc
+constexpr uint32_t kStatusCancelled = 3;  // the host cancelled the query (request or deadline)
 
 enum SlotKind : int32_t {
   KIND_QUERY = 0,
@@ -156,6 +162,9 @@ struct alignas(64) Slot {
   uint32_t vv[kMaxK * kMaxK];
   int32_t out_cost, out_root, out_nedges, out_d;
   unsigned long long ctr[6];
+  uint32_t maxq;        // largest frontier (vertices) of one of its rounds
+  uint32_t count;       // 1: run PH_COUNT before RECON
+  unsigned long long nfinite;  // PH_COUNT: finite states (singletons over all heights + T)
 };
 
 struct alignas(64) Ctrl {
@@ -169,6 +178,11 @@ struct alignas(64) Ctrl {
   uint32_t max_steps;
   uint32_t nslots;
   uint32_t trace_slot;  // slot whose planned phase is logged per step (~0u: none)
+  uint32_t pad0;
+  // Host-set cancellation, read by the plan kernel every step: host memory
+  // the device can read (null: none).  Word 0 is nonzero once any flag is
+  // set; word 1 + i asks slot i to stop.
+  const uint32_t* cancel;
   uint32_t active[kNumCls][kMaxSlots];
   uint32_t prefix[kNumCls][kMaxSlots + 1];
   uint32_t trace[kTraceSteps];  // phase | level << 8 | planned << 16, by step of the epoch

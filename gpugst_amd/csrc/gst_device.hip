@@ -1434,9 +1434,33 @@ __device__ void item_recon(const GraphDev& g, Slot* sp) {
     sp->out_root = (int32_t)root;
     sp->out_nedges = (int32_t)ne;
     sp->out_d = dstar;
-    if (overflow || ne > (uint32_t)sp->edge_cap) sp->status = 1;  // edge capacity exceeded
-    if (bad) sp->status = 2;                                       // no shortest-path arc found
+    if (overflow || ne > (uint32_t)sp->edge_cap) sp->status = kStatusEdgeCap;
+    if (bad) sp->status = kStatusNoPath;
   }
+}
+
+// Finite-state count (option count_states; the reference's count_set,
+// code/GPUGST/src/DPQ.cu:225-240, D-GPUGST :128-148): per vertex of a tile,
+// the finite singleton states over every height (from S1 or the job's rows)
+// and, at live vertices (the only ones whose T segment is initialised), the
+// finite states of two or more groups.
+__device__ void item_count(Slot* sp, uint32_t tile, uint64_t V) {
+  const int lane = lane_id();
+  const States S = states_of(sp);
+  const int k = sp->k, W = sp->W, ND = sp->ND, NS = sp->NS;
+  const uint64_t v0 = (uint64_t)tile * kDistTV;
+  const uint64_t v1 = v0 + kDistTV < V ? v0 + kDistTV : V;
+  uint64_t n = 0;
+  for (uint64_t v = v0 + (uint64_t)lane; v < v1; v += 64) {
+    for (int d = 0; d < ND; d++)
+      for (int i = 0; i < k; i++) n += (S.get((uint32_t)v, 1u << i, (uint32_t)d) >> 32) < kInf ? 1u : 0u;
+    if (!((sp->lvb[v >> 5] >> (v & 31u)) & 1u)) continue;
+    for (int d = 0; d < NS; d++)
+      for (int p = 3; p < W; p++)
+        if (p & (p - 1)) n += (S.get((uint32_t)v, (uint32_t)p, (uint32_t)d) >> 32) < kInf ? 1u : 0u;
+  }
+  n = wave_sum_u64(n);
+  if (lane == 0 && n) atomicAdd(&sp->nfinite, (unsigned long long)n);
 }
 
 // -------------------------------------------------------- state machine ----
@@ -1476,6 +1500,7 @@ __device__ void complete_step(Slot* slots, Slot* s, Ctrl* ctrl) {
       case PH_ROUND:
         s->ctr[1] += (unsigned long long)(s->app[r % 3] >> 32);
         s->ctr[2] += (unsigned long long)(uint32_t)s->app[r % 3];
+        if ((uint32_t)(s->app[r % 3] >> 32) > s->maxq) s->maxq = (uint32_t)(s->app[r % 3] >> 32);
         if (s->D < 0) {
           if ((s->app[(r + 1) % 3] >> 32) == 0) s->phase = PH_STAR;
         } else if (s->level >= s->D) {
@@ -1513,6 +1538,7 @@ __device__ void complete_step(Slot* slots, Slot* s, Ctrl* ctrl) {
       case PH_ROUND:
         s->ctr[1] += (unsigned long long)(s->app[r % 3] >> 32);
         s->ctr[2] += (unsigned long long)(uint32_t)s->app[r % 3];
+        if ((uint32_t)(s->app[r % 3] >> 32) > s->maxq) s->maxq = (uint32_t)(s->app[r % 3] >> 32);
         if ((s->app[(r + 1) % 3] >> 32) == 0) advance_level(s);
         break;
       case PH_DIST:
@@ -1531,6 +1557,9 @@ __device__ void complete_step(Slot* slots, Slot* s, Ctrl* ctrl) {
         else s->phase = PH_ROUND;
         break;
       case PH_FINAL:
+        s->phase = s->count ? PH_COUNT : PH_RECON;
+        break;
+      case PH_COUNT:
         s->phase = PH_RECON;
         break;
       case PH_RECON:
@@ -1572,6 +1601,7 @@ __device__ uint32_t plan_step(Slot* s, uint64_t V) {
     case PH_MERGE: work = ceil_div(s->level >= 3 ? s->nact : s->nlive, kMergeTV); cls = kClsHeavy; break;
     case PH_FINAL: work = ceil_div(s->k >= 3 ? s->nact : s->nlive, kMergeTV); cls = kClsHeavy; break;
     case PH_RECON: work = 1; cls = kClsHeavy; break;
+    case PH_COUNT: work = ceil_div(V, kDistTV); cls = kClsHeavy; break;
     default: return (s->n_work = s->n_clr = 0);
   }
   // every active step runs at least one (possibly empty) item so that it completes
@@ -1592,6 +1622,24 @@ __device__ void phase_b(Slot* slots, Ctrl* ctrl, uint64_t V, uint32_t (*sh_items
     if (s->in_step) {
       complete_step(slots, s, ctrl);
       s->in_step = 0;
+    }
+  }
+  // Host-set cancellation: a query asked to stop ends here, without an
+  // answer.  A query fed by a group-distance job stops only once its job has
+  // finished, because the job writes into its members' slots until then.
+  const uint32_t* cf = ctrl->cancel;
+  if (cf && __hip_atomic_load(&cf[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0u) {
+    for (int i = tid; i < nslots; i += kBlock) {
+      Slot* s = &slots[i];
+      if (s->kind != KIND_QUERY || s->phase == PH_FREE || s->phase == PH_DONE) continue;
+      if (s->job >= 0 && !s->jdone) continue;
+      if (__hip_atomic_load(&cf[1 + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) == 0u) continue;
+      s->phase = PH_DONE;
+      s->status = kStatusCancelled;
+      s->out_cost = (int32_t)kInf;
+      s->out_root = -1;
+      s->out_nedges = 0;
+      atomicAdd(&ctrl->finished, 1u);
     }
   }
   __threadfence();
@@ -1749,6 +1797,7 @@ __device__ void step_body(const GraphDev& g, Slot* slots, Ctrl* ctrl, LDS* L) {
           case PH_MERGE: item_merge(g, sp, local, (WaveLDS*)L); break;
           case PH_FINAL: item_final(g, sp, local, (WaveLDS*)L); break;
           case PH_RECON: item_recon(g, sp); break;
+          case PH_COUNT: item_count(sp, local, V); break;
           default: break;
         }
       }

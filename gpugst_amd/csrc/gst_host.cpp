@@ -125,6 +125,8 @@ struct Options {
   int64_t time_steps = 0;  // time each step's lean and heavy kernels with events
   int64_t trace_slot = 0;  // with time_steps: slot whose phases the step times are binned by
   int64_t job_min_queries = 2;  // fewer unbounded queries admitted together run their own level 1
+  int64_t deadline_ms = 0;  // a query still running this long after its submission is cancelled (0: none)
+  int64_t count_states = 0;  // count the finite DP states of every query (an extra pass)
 };
 
 }  // namespace
@@ -167,6 +169,7 @@ struct QueryReq {
   int32_t D = -1;
   uint64_t tag = 0;
   double t_submit = 0;
+  double t_deadline = 0;  // host time after which the query is cancelled (0: none)
 };
 
 struct Layout {
@@ -279,7 +282,10 @@ struct SlotHost {
   size_t off = 0, len = 0;
   uint64_t tag = 0;
   double t_submit = 0, t_admit = 0;
+  double t_deadline = 0;  // 0: none
+  bool flagged = false;   // its cancellation flag is set
   int64_t extra[5] = {0, 0, 0, 0, 0};  // level-1 work charged from the query's job
+  uint32_t jmaxq = 0;     // largest frontier of its job
 };
 
 }  // namespace
@@ -289,6 +295,7 @@ class Engine {
   Engine(gst_graph* g, int max_slots) : g_(g), max_slots_(std::min(max_slots, gst::kMaxSlots)) {}
   ~Engine() {
     if (stream_) (void)hipStreamSynchronize(stream_);
+    if (h_cancel_) (void)hipHostFree(h_cancel_);
     if (borrowed_) {
       delete[] h_slots_;
       delete h_ctrl_;
@@ -319,6 +326,11 @@ class Engine {
     memset(h_ctrl_, 0, sizeof(gst::Ctrl));
     HIP_TRY(hipMemset(d_ctrl_, 0, sizeof(gst::Ctrl)));
     sh_.assign(max_slots_, SlotHost());
+    // cancellation flags: host memory the plan kernel reads while it runs
+    HIP_TRY(hipHostMalloc((void**)&h_cancel_, sizeof(uint32_t) * (max_slots_ + 1),
+                          hipHostMallocMapped | hipHostMallocCoherent));
+    memset(h_cancel_, 0, sizeof(uint32_t) * (max_slots_ + 1));
+    HIP_TRY(hipHostGetDevicePointer((void**)&d_cancel_, h_cancel_, 0));
     return resize_arena(arena_bytes);
   }
 
@@ -450,6 +462,7 @@ class Engine {
   int admit_query(const QueryReq& q, int job, int lab, uint32_t* jdist = nullptr, int jLp = 0) {
     // jdist: the job's rows, ND rows of jLp labels per vertex
     const int k = (int)q.groups.size();
+    if (k < 1 || k > gst::kMaxK) return -1;  // validated at submission; never admitted
     const bool own = job < 0;
     const Layout L = layout_for(g_, k, q.D, own);
     const int s = free_slot();
@@ -464,6 +477,11 @@ class Engine {
     h.tag = q.tag;
     h.t_submit = q.t_submit;
     h.t_admit = now_ms();
+    h.t_deadline = q.t_deadline;
+    if (h.t_deadline > 0) n_deadline_++;
+    h.flagged = false;
+    h.jmaxq = 0;
+    if (h_cancel_) __atomic_store_n(&h_cancel_[1 + s], 0u, __ATOMIC_RELEASE);
     gst::Slot& d = h_slots_[s];
     memset(&d, 0, sizeof(d));
     d.kind = gst::KIND_QUERY;
@@ -502,6 +520,7 @@ class Engine {
     d.edges = (gst::EdgeDev*)(base + L.edges);
     d.rq = (uint32_t*)(base + L.rq);
     d.phase = gst::PH_FILL;
+    d.count = g_->opt.count_states ? 1u : 0u;
     d.best = gst::kInf;
     d.ans = ~0ull;
     d.out_cost = (int32_t)gst::kInf;
@@ -514,6 +533,14 @@ class Engine {
 
   void release_slot(int s) {
     SlotHost& h = sh_[s];
+    if (h.flagged) {
+      h.flagged = false;
+      if (--n_flagged_ == 0 && h_cancel_) __atomic_store_n(&h_cancel_[0], 0u, __ATOMIC_RELEASE);
+    }
+    if (h.t_deadline > 0) {
+      h.t_deadline = 0;
+      n_deadline_--;
+    }
     arena_.release(h.off, h.len);
     if (!h.job) active_--;
     h.used = false;
@@ -596,6 +623,7 @@ class Engine {
   gst_status run_epoch(uint32_t exit_after, uint32_t max_steps, std::vector<int>* finished) {
     finished->clear();
     if (nslots_used_ == 0) return GST_OK;
+    if (n_deadline_ > 0) flag_expired(now_ms());
     step_before_.resize(max_slots_);
     for (int i = 0; i < nslots_used_; i++) step_before_[i] = (uint32_t)h_slots_[i].ctr[0];
     const bool use_graph = g_->opt.graph != 0;
@@ -610,6 +638,7 @@ class Engine {
     c.max_steps = max_steps;
     c.nslots = (uint32_t)(use_graph ? max_slots_ : nslots_used_);
     c.trace_slot = timed ? (uint32_t)g_->opt.trace_slot : ~0u;
+    c.cancel = d_cancel_;
     const size_t hdr = offsetof(gst::Ctrl, active);
     pair_ns_.clear();
     HIP_TRY(hipMemcpyAsync(d_ctrl_, h_ctrl_, hdr, hipMemcpyHostToDevice, stream_));
@@ -645,6 +674,7 @@ class Engine {
       HIP_TRY(hipMemcpyAsync(h_ctrl_, d_ctrl_, hdr, hipMemcpyDeviceToHost, stream_));
       gst_status st = wait_stream();
       if (st != GST_OK) return st;
+      if (n_deadline_ > 0) flag_expired(now_ms());
       if (timed) {
         for (int i = 0; i < chunk; i++) {
           float a = 0, b = 0;
@@ -705,6 +735,7 @@ class Engine {
           SlotHost& qh = sh_[m];
           for (int t = 1; t <= 4; t++) qh.extra[t] += (int64_t)(j.ctr[t] * (uint64_t)q.k / (uint64_t)std::max(1, j.L));
           qh.extra[0] += (int64_t)j.ctr[0];
+          qh.jmaxq = std::max(qh.jmaxq, j.maxq);
         }
         jobs_done_++;
         for (int t = 0; t < 6; t++) job_ctr_[t] += (int64_t)j.ctr[t];
@@ -731,16 +762,71 @@ class Engine {
 
   // Waits for the stream with a deadline, so a launch that never finishes is
   // reported instead of blocking the caller forever.
+  // While it waits, it raises the cancellation flag of every query past its
+  // deadline; the plan kernel reads the flags at each step.  After a timeout
+  // or a device error the engine is poisoned: steps may still be running and
+  // copies into the host mirrors may still be queued, so nothing of the
+  // engine is touched again until its stream has drained (recover()).
   gst_status wait_stream() {
     const double t_end = now_ms() + (double)g_->opt.timeout_ms;
-    for (;;) {
+    for (int it = 0;; it++) {
       const hipError_t q = hipStreamQuery(stream_);
       if (q == hipSuccess) return GST_OK;
-      if (q != hipErrorNotReady) return fail(GST_DEVICE_ERROR, "engine step failed: %s", hipGetErrorString(q));
-      if (now_ms() > t_end) return fail(GST_DEVICE_ERROR, "engine step did not finish within %lld ms",
-                                        (long long)g_->opt.timeout_ms);
+      if (q != hipErrorNotReady) {
+        poisoned_ = true;
+        return fail(GST_DEVICE_ERROR, "engine step failed: %s", hipGetErrorString(q));
+      }
+      const double t = now_ms();
+      if (t > t_end) {
+        poisoned_ = true;
+        return fail(GST_DEVICE_ERROR, "engine step did not finish within %lld ms", (long long)g_->opt.timeout_ms);
+      }
+      if (n_deadline_ > 0 && (it & 15) == 0) flag_expired(t);
       std::this_thread::sleep_for(std::chrono::microseconds(20));
     }
+  }
+
+  // Raises the cancellation flag of slot s (a query); returns false when the
+  // engine has no flags (the static engine).
+  bool flag_slot(int s) {
+    if (!h_cancel_) return false;
+    SlotHost& h = sh_[s];
+    if (!h.used || h.job || h.flagged) return true;
+    h.flagged = true;
+    n_flagged_++;
+    __atomic_store_n(&h_cancel_[1 + s], 1u, __ATOMIC_RELEASE);
+    __atomic_store_n(&h_cancel_[0], 1u, __ATOMIC_RELEASE);
+    return true;
+  }
+  void flag_expired(double t) {
+    for (int i = 0; i < nslots_used_; i++)
+      if (sh_[i].used && !sh_[i].job && !sh_[i].flagged && sh_[i].t_deadline > 0 && t > sh_[i].t_deadline)
+        flag_slot(i);
+  }
+  // Cancels the admitted query with this tag; false if none.
+  bool cancel_tag(uint64_t tag) {
+    for (int i = 0; i < nslots_used_; i++)
+      if (sh_[i].used && !sh_[i].job && sh_[i].tag == tag) return flag_slot(i);
+    return false;
+  }
+  void cancel_all() {
+    for (int i = 0; i < nslots_used_; i++)
+      if (sh_[i].used && !sh_[i].job) flag_slot(i);
+  }
+
+  bool poisoned() const { return poisoned_; }
+  void poison() { poisoned_ = true; }
+  // Makes a poisoned engine usable again once its stream has drained: the
+  // admitted queries are dropped (their callers already got the error).
+  gst_status recover() {
+    if (!poisoned_) return GST_OK;
+    const hipError_t q = hipStreamQuery(stream_);
+    if (q == hipErrorNotReady)
+      return fail(GST_DEVICE_ERROR, "an earlier call timed out and its engine steps are still running on the device");
+    if (q != hipSuccess) return fail(GST_DEVICE_ERROR, "engine stream failed: %s", hipGetErrorString(q));
+    poisoned_ = false;
+    reset_slots();
+    return GST_OK;
   }
 
   // A chunk of kGraphChunk (lean, heavy, plan) launches captured once and
@@ -798,8 +884,16 @@ class Engine {
     out->counters.shared_arcs = h.extra[2];
     out->counters.shared_relaxations = h.extra[3];
     out->counters.active_vertices = (int64_t)d.nact;
+    out->counters.max_frontier = (int64_t)std::max(d.maxq, h.jmaxq);
+    out->counters.states_finite = d.count ? (int64_t)d.nfinite : -1;
     gst_status st = GST_OK;
-    if (d.status == 1) {
+    if (d.status == gst::kStatusCancelled) {
+      out->status = GST_CANCELLED;
+      out->cost = GST_INFEASIBLE;
+      out->root = -1;
+      out->n_edges = 0;
+      st = fail(GST_CANCELLED, "query cancelled before it finished (request or deadline)");
+    } else if (d.status == gst::kStatusEdgeCap) {
       out->status = GST_DEVICE_ERROR;
       st = fail(GST_DEVICE_ERROR, "tree of the query exceeds the edge capacity %d", d.edge_cap);
       out->n_edges = 0;
@@ -853,6 +947,9 @@ class Engine {
       sh_[i] = SlotHost();
       memset(&h_slots_[i], 0, sizeof(gst::Slot));
     }
+    if (h_cancel_) memset(h_cancel_, 0, sizeof(uint32_t) * (max_slots_ + 1));
+    n_flagged_ = 0;
+    n_deadline_ = 0;
     active_ = 0;
     used_slots_ = 0;
     nslots_used_ = 0;
@@ -896,6 +993,11 @@ class Engine {
   int max_slots_;
   bool borrowed_ = false;   // device tables live in memory the engine does not own
   bool force_own_ = false;  // every query runs its own level 1
+  bool poisoned_ = false;   // a wait timed out or the device failed: see recover()
+  uint32_t* h_cancel_ = nullptr;  // cancellation flags (pinned host memory)
+  const uint32_t* d_cancel_ = nullptr;  // the same words as the device reads them
+  int n_flagged_ = 0;       // slots whose flag is set
+  int n_deadline_ = 0;      // admitted queries with a deadline
   hipStream_t stream_ = nullptr;
   gst::Slot* d_slots_ = nullptr;
   gst::Slot* h_slots_ = nullptr;
@@ -965,6 +1067,10 @@ void parallel_for_rows(const int64_t* beg_pos, int64_t V, F f) {
 
 gst_status engine_for(gst_graph* g, bool batch, size_t need, Engine** out) {
   std::unique_ptr<Engine>& e = batch ? g->batch : g->plain;
+  if (e) {
+    gst_status st = e->recover();
+    if (st != GST_OK) return st;
+  }
   if (!e) {
     int slots = batch ? (int)g->opt.max_slots : 2;  // plain: the query and its job
     std::unique_ptr<Engine> ne(new Engine(g, slots));
@@ -1003,7 +1109,8 @@ gst_status run_queries(gst_graph* g, Engine* eng, std::vector<QueryReq>& qs, gst
     pend.push_back(&qs[i]);
   }
   std::vector<int> fin;
-  gst_status first_err = GST_OK;
+  gst_status first_err = eng->recover();
+  if (first_err != GST_OK) return first_err;
   while (!pend.empty() || eng->active() > 0) {
     if (!pend.empty()) {
       const int n = eng->admit_front(pend);
@@ -1015,7 +1122,7 @@ gst_status run_queries(gst_graph* g, Engine* eng, std::vector<QueryReq>& qs, gst
     const uint32_t exit_after = pend.empty() ? (uint32_t)act : std::max(1u, (uint32_t)act / 8);
     gst_status st = eng->run_epoch(exit_after, kEpochMaxSteps, &fin);
     if (st != GST_OK) {
-      eng->reset_slots();
+      eng->poison();  // reset once the stream has drained (next call)
       return st;
     }
     for (int s : fin) {
@@ -1200,6 +1307,8 @@ static int64_t* option_ref(gst_graph* g, const std::string& k) {
   if (k == "time_steps") return &g->opt.time_steps;
   if (k == "trace_slot") return &g->opt.trace_slot;
   if (k == "job_min_queries") return &g->opt.job_min_queries;
+  if (k == "deadline_ms") return &g->opt.deadline_ms;
+  if (k == "count_states") return &g->opt.count_states;
   return nullptr;
 }
 
@@ -1218,7 +1327,8 @@ gst_status gst_set_option(gst_graph* g, const char* key, int64_t value) {
     return fail(GST_INVALID_ARG, "max_slots outside [1,%d]", gst::kMaxSlots);
   if (k == "chunk" && (value < 64 || value > (1 << 20))) return fail(GST_INVALID_ARG, "chunk outside [64, 2^20]");
   if (k == "edge_cap" && (value < 64 || value > (1 << 26))) return fail(GST_INVALID_ARG, "edge_cap outside [64, 2^26]");
-  if ((k == "budget_mb" || k == "blocks_per_cu" || k == "timeout_ms" || k == "trace_slot") && value < 0)
+  if ((k == "budget_mb" || k == "blocks_per_cu" || k == "timeout_ms" || k == "trace_slot" || k == "deadline_ms" ||
+       k == "count_states") && value < 0)
     return fail(GST_INVALID_ARG, "negative value");
   if (k == "job_min_queries" && (value < 1 || value > gst::kJobLabels))
     return fail(GST_INVALID_ARG, "job_min_queries outside [1,%d]", gst::kJobLabels);
@@ -1301,6 +1411,7 @@ gst_status gst_solve(gst_graph* g, const int32_t* group_ids, int32_t k, int32_t 
   qs[0].groups.assign(group_ids, group_ids + k);
   qs[0].D = D;
   qs[0].t_submit = now_ms();
+  if (g->opt.deadline_ms > 0) qs[0].t_deadline = qs[0].t_submit + (double)g->opt.deadline_ms;
   return run_queries(g, eng, qs, out);
 }
 
@@ -1359,6 +1470,7 @@ gst_status gst_solve_batch(gst_graph* g, int64_t n, const gst_query* q, gst_resu
     r.groups.assign(q[i].group_ids, q[i].group_ids + q[i].k);
     r.D = q[i].D;
     r.t_submit = t0;
+    if (g->opt.deadline_ms > 0) r.t_deadline = t0 + (double)g->opt.deadline_ms;
     qs.push_back(std::move(r));
     map.push_back(i);
   }
@@ -1399,9 +1511,39 @@ struct gst_server {
   std::string error_msg;
   size_t max_pending = 1 << 20;
   uint32_t epoch_steps = 256;
+  int waiters = 0;  // threads inside gst_wait
 };
 
 namespace {
+
+gst_result cancelled_result() {
+  gst_result r;
+  memset(&r, 0, sizeof(r));
+  r.status = GST_CANCELLED;
+  r.cost = GST_INFEASIBLE;
+  r.root = -1;
+  return r;
+}
+
+// Answers the pending requests whose deadline has passed (caller holds s->mu).
+void drop_expired(gst_server* s) {
+  const double t = now_ms();
+  auto expired = [t](const std::pair<uint64_t, QueryReq>& pr) {
+    return pr.second.t_deadline > 0 && t > pr.second.t_deadline;
+  };
+  if (std::none_of(s->pending.begin(), s->pending.end(), expired)) return;
+  std::deque<std::pair<uint64_t, QueryReq>> rest;
+  for (auto& pr : s->pending) {
+    if (expired(pr)) {
+      s->done[pr.first] = cancelled_result();
+      s->known[pr.first] = 2;
+    } else {
+      rest.push_back(std::move(pr));
+    }
+  }
+  s->pending.swap(rest);
+  s->cv_done.notify_all();
+}
 
 void server_loop(gst_server* s) {
   std::vector<int> fin;
@@ -1410,6 +1552,7 @@ void server_loop(gst_server* s) {
       std::unique_lock<std::mutex> lk(s->mu);
       s->cv_new.wait(lk, [&] { return s->stop || !s->pending.empty() || s->eng->active() > 0; });
       if (s->stop && s->pending.empty() && s->eng->active() == 0) break;
+      drop_expired(s);
       while (!s->pending.empty()) {
         std::deque<QueryReq*> pend;
         for (auto& pr : s->pending) {
@@ -1457,7 +1600,7 @@ void server_loop(gst_server* s) {
     {
       std::lock_guard<std::mutex> dk(s->g->dev_mu);
       st = s->eng->run_epoch(exit_after, s->epoch_steps, &fin);
-      if (st != GST_OK) s->eng->reset_slots();
+      if (st != GST_OK) s->eng->poison();  // the server stops; nothing runs on this engine again
     }
     std::lock_guard<std::mutex> lk(s->mu);
     if (st != GST_OK) {
@@ -1528,6 +1671,7 @@ gst_status gst_submit(gst_server* s, const gst_query* q, uint64_t* ticket) {
   r.groups.assign(q->group_ids, q->group_ids + q->k);
   r.D = q->D;
   r.t_submit = now_ms();
+  if (s->g->opt.deadline_ms > 0) r.t_deadline = r.t_submit + (double)s->g->opt.deadline_ms;
   s->pending.emplace_back(t, std::move(r));
   s->known[t] = 0;
   s->cv_new.notify_one();
@@ -1555,7 +1699,10 @@ gst_status gst_wait(gst_server* s, uint64_t ticket, gst_result* out) {
   if (!s || !out) return fail(GST_INVALID_ARG, "null argument");
   std::unique_lock<std::mutex> lk(s->mu);
   if (!s->known.count(ticket)) return fail(GST_INVALID_ARG, "unknown ticket %llu", (unsigned long long)ticket);
+  s->waiters++;
   s->cv_done.wait(lk, [&] { return s->done.count(ticket) || s->error != GST_OK; });
+  s->waiters--;
+  if (s->waiters == 0) s->cv_done.notify_all();  // gst_server_destroy waits for the last waiter
   auto it = s->done.find(ticket);
   if (it == s->done.end()) return fail(s->error, "server stopped: %s", s->error_msg.c_str());
   *out = it->second;
@@ -1564,15 +1711,50 @@ gst_status gst_wait(gst_server* s, uint64_t ticket, gst_result* out) {
   return GST_OK;
 }
 
+gst_status gst_cancel(gst_server* s, uint64_t ticket) {
+  if (!s) return fail(GST_INVALID_ARG, "null server");
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto kn = s->known.find(ticket);
+  if (kn == s->known.end()) return fail(GST_INVALID_ARG, "unknown ticket %llu", (unsigned long long)ticket);
+  if (kn->second == 0) {
+    // still queued: answered at once
+    for (auto it = s->pending.begin(); it != s->pending.end(); ++it)
+      if (it->first == ticket) {
+        s->pending.erase(it);
+        break;
+      }
+    s->done[ticket] = cancelled_result();
+    kn->second = 2;
+    s->cv_done.notify_all();
+  } else if (kn->second == 1) {
+    // on the device: the plan kernel stops it at its next step
+    s->eng->cancel_tag(ticket);
+  }
+  return GST_OK;
+}
+
 gst_status gst_server_destroy(gst_server* s) {
   if (!s) return GST_OK;
   {
-    std::lock_guard<std::mutex> lk(s->mu);
+    // queued requests are answered GST_CANCELLED, running ones are cancelled
+    // on the device, and every waiter wakes before the server goes away
+    std::unique_lock<std::mutex> lk(s->mu);
     s->stop = true;
+    for (auto& pr : s->pending) {
+      s->done[pr.first] = cancelled_result();
+      s->known[pr.first] = 2;
+    }
     s->pending.clear();
+    s->eng->cancel_all();
+    s->cv_done.notify_all();
   }
   s->cv_new.notify_all();
   if (s->worker.joinable()) s->worker.join();
+  {
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->cv_done.notify_all();
+    s->cv_done.wait(lk, [&] { return s->waiters == 0; });
+  }
   for (auto& kv : s->done) gst_result_free(&kv.second);
   {
     std::lock_guard<std::mutex> dk(s->g->dev_mu);

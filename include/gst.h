@@ -70,9 +70,14 @@ typedef struct {
 
 /* Per-query device counters.  The reference's `records{process_queue_num,
  * counts}` (code/GPUGST/include/DPQ.cuh:59-62) map to frontier_entries and
- * improvements.  The first five count the query's own work; a query whose
- * level 1 came from a shared group-distance job (unbounded, k >= 2) also gets
- * its share of that job (per label) in the shared_ fields. */
+ * states_finite (the reference counts the finite states of its table with
+ * count_set after the main loop, code/GPUGST/src/DPQ.cu:545-551; here the
+ * same count is an optional pass, option "count_states").  Its RAM figure,
+ * finite states + largest queue + V * 2^k (DPQ.cu:549), is
+ * states_finite + max_frontier + V << k.  The first five count the query's
+ * own work; a query whose level 1 came from a shared group-distance job
+ * (unbounded, k >= 2) also gets its share of that job (per label) in the
+ * shared_ fields. */
 typedef struct {
   int64_t steps;            /* engine steps (rounds + merges + phases), incl. its job's */
   int64_t frontier_entries; /* frontier vertices processed */
@@ -85,13 +90,18 @@ typedef struct {
   int64_t shared_arcs;      /* share of its job's arcs */
   int64_t shared_relaxations; /* share of its job's label relaxations */
   int64_t active_vertices;  /* live vertices that received a state of two or more groups */
+  int64_t max_frontier;     /* largest frontier of one of its rounds (its job's included), vertices */
+  int64_t states_finite;    /* finite DP states (v, p[, d]) at the end: singletons over every height
+                               plus the stored states of two or more groups; -1 unless the option
+                               "count_states" was set when the query was admitted */
 } gst_counters;
 
 typedef struct {
-  int32_t cost;     /* optimal cost, or GST_INFEASIBLE */
+  int32_t cost;     /* optimal cost, or GST_INFEASIBLE (also when the query was cancelled) */
   int32_t root;     /* root vertex of the returned tree, -1 if infeasible */
   int32_t n_edges;  /* number of records in edges */
-  int32_t status;   /* gst_status of this query */
+  int32_t status;   /* gst_status of this query (GST_CANCELLED: stopped by gst_cancel, a
+                       deadline or gst_server_destroy, no answer) */
   gst_edge *edges;  /* owned by the library until gst_result_free */
   gst_counters counters;
   double latency_ms; /* submit -> result available (batch and server) */
@@ -150,7 +160,13 @@ gst_status gst_solve_batch(gst_graph *g, int64_t n, const gst_query *q,
 void gst_result_free(gst_result *r);
 
 /* Tunables: "max_slots", "budget_mb", "blocks_per_cu", "chunk",
- * "edge_cap", "timeout_ms", "graph" (replay engine steps as a hipGraph),
+ * "edge_cap", "timeout_ms" (host-side deadline for one chunk of engine steps;
+ * on expiry the call returns GST_DEVICE_ERROR and the engine refuses work
+ * until the device has finished those steps), "deadline_ms" (0: none; a
+ * query of gst_solve / gst_solve_batch / gst_submit still running this long
+ * after its submission is stopped on the device at its next step and
+ * answered with status GST_CANCELLED), "count_states" (1: fill
+ * counters.states_finite with an extra pass), "graph" (replay engine steps as a hipGraph),
  * "group_pass" (1: unbounded queries share group-distance jobs; 0: each
  * query runs its own level 1), "job_min_queries" (default 2: fewer
  * unbounded queries admitted together run their own level 1 instead of a
@@ -179,6 +195,15 @@ gst_status gst_submit(gst_server *s, const gst_query *q, uint64_t *ticket);
 /* GST_NOT_READY while pending; on GST_OK *out owns the result. */
 gst_status gst_poll(gst_server *s, uint64_t ticket, gst_result *out);
 gst_status gst_wait(gst_server *s, uint64_t ticket, gst_result *out);
+/* Cancels a request: a queued one is answered at once, a running one is
+ * stopped by the engine at its next step (a request fed by a shared
+ * group-distance job once that job is done); its result has status
+ * GST_CANCELLED unless it finished first.  No reference counterpart (the
+ * reference has no request loop). */
+gst_status gst_cancel(gst_server *s, uint64_t ticket);
+/* Answers every queued request with GST_CANCELLED, cancels the running ones,
+ * wakes every gst_wait and returns once they have returned; no call on s may
+ * start after it. */
 gst_status gst_server_destroy(gst_server *s);
 
 /* Thread-local text of the last error. */

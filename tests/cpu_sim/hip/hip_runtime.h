@@ -21,6 +21,7 @@
 #define __HIP_MEMORY_SCOPE_WAVEFRONT 0
 #define __HIP_MEMORY_SCOPE_WORKGROUP 1
 #define __HIP_MEMORY_SCOPE_AGENT 2
+#define __HIP_MEMORY_SCOPE_SYSTEM 3
 
 namespace gsim {
 struct Wave {
@@ -150,6 +151,7 @@ typedef struct ihipGraphExec* hipGraphExec_t;
 typedef int hipMemcpyKind;
 enum { hipMemcpyHostToHost = 0, hipMemcpyHostToDevice = 1, hipMemcpyDeviceToHost = 2, hipMemcpyDeviceToDevice = 3 };
 enum { hipStreamNonBlocking = 1, hipHostMallocDefault = 0, hipStreamCaptureModeThreadLocal = 1 };
+enum { hipHostMallocMapped = 2, hipHostMallocCoherent = 0x40000000 };
 struct hipDeviceProp_t {
   int multiProcessorCount;
 };
@@ -168,6 +170,7 @@ hipError_t hipHostMalloc(void**, size_t, unsigned);
 template <class T>
 inline hipError_t hipHostMalloc(T** p, size_t n, unsigned f) { return hipHostMalloc((void**)p, n, f); }
 hipError_t hipHostFree(void*);
+hipError_t hipHostGetDevicePointer(void**, void*, unsigned);
 hipError_t hipMemcpy(void*, const void*, size_t, hipMemcpyKind);
 hipError_t hipMemcpyAsync(void*, const void*, size_t, hipMemcpyKind, hipStream_t);
 hipError_t hipMemset(void*, int, size_t);

@@ -29,6 +29,7 @@ hipError_t hipMalloc(void** p, size_t n) {
 hipError_t hipFree(void* p) { free(p); return hipSuccess; }
 hipError_t hipHostMalloc(void** p, size_t n, unsigned) { return hipMalloc(p, n); }
 hipError_t hipHostFree(void* p) { free(p); return hipSuccess; }
+hipError_t hipHostGetDevicePointer(void** d, void* h, unsigned) { *d = h; return hipSuccess; }
 hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind) { memmove(d, s, n); return hipSuccess; }
 hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind k, hipStream_t) { return hipMemcpy(d, s, n, k); }
 hipError_t hipMemset(void* p, int v, size_t n) { memset(p, v, n); return hipSuccess; }
@@ -36,7 +37,13 @@ hipError_t hipMemGetInfo(size_t* f, size_t* t) { *f = (size_t)8 << 30; *t = (siz
 hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) { *s = (hipStream_t)malloc(8); return hipSuccess; }
 hipError_t hipStreamDestroy(hipStream_t s) { free(s); return hipSuccess; }
 hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
-hipError_t hipStreamQuery(hipStream_t) { return hipSuccess; }
+// Test hook: while set, every stream reports work still running (a device
+// that does not finish in time); launches still complete when issued.
+static int g_stream_busy = 0;
+extern "C" void gsim_stream_busy(int on) { __atomic_store_n(&g_stream_busy, on, __ATOMIC_SEQ_CST); }
+hipError_t hipStreamQuery(hipStream_t) {
+  return __atomic_load_n(&g_stream_busy, __ATOMIC_SEQ_CST) ? hipErrorNotReady : hipSuccess;
+}
 hipError_t hipEventCreate(hipEvent_t* e) { *e = new ihipEvent{0}; return hipSuccess; }
 hipError_t hipEventDestroy(hipEvent_t e) { delete e; return hipSuccess; }
 hipError_t hipEventRecord(hipEvent_t e, hipStream_t) { e->t = now_ms(); return hipSuccess; }

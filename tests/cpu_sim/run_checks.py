@@ -229,12 +229,95 @@ def check_static():
     return {"checked": n, "bad": bad, "small_buffer_error": small_err or need <= 1}
 
 
+def check_robust():
+    """Failure handling on the CPU build, forced from the host:
+      * count_states fills states_finite (every singleton of a connected graph
+        is finite) and max_frontier; -1 without the option;
+      * a deadline stops a query on the device (status CANCELLED) and the
+        engine answers the next query exactly;
+      * a wait that times out (the mock stream reports busy) poisons the
+        engine: the call and the next one fail with DEVICE_ERROR while the
+        stream is busy, and once it drains the engine answers exactly;
+      * the server: gst_cancel of a request, and gst_server_destroy with a
+        thread blocked in gst_wait (it returns; nothing hangs)."""
+    import ctypes
+    import threading
+    from gpugst_amd import GstError, Server
+    from gpugst_amd.capi import GST_CANCELLED, GST_DEVICE_ERROR, lib_path
+    rng, rp, col, wt, groups = random_instance(150, 23)
+    g = Graph(rp, col, wt)
+    g.set_groups(*groups_csr(groups))
+    q = rng.sample(range(len(groups)), 5)
+    exp = O.dpbf(rp, col, wt, [groups[x] for x in q])[0]
+    out = {}
+    r0 = g.solve(q)
+    g.set_option("count_states", 1)
+    r1 = g.solve(q)
+    g.set_option("count_states", 0)
+    out["count"] = [r0.counters["states_finite"], r1.counters["states_finite"], r1.counters["max_frontier"],
+                    r1.cost == exp]
+    out["count_ok"] = (r0.counters["states_finite"] == -1 and r1.counters["states_finite"] >= 150 * 5
+                       and r1.counters["max_frontier"] > 0 and r0.cost == r1.cost == exp)
+    # deadline
+    g.set_option("deadline_ms", 1)
+    st = None
+    try:
+        g.solve(q)
+    except GstError as e:
+        st = e.status
+    rb = g.solve_batch([q, q[:3], q[1:]])
+    g.set_option("deadline_ms", 0)
+    out["deadline_ok"] = (st == GST_CANCELLED and all(r.status == GST_CANCELLED for r in rb)
+                          and g.solve(q).cost == exp)
+    # timeout -> poisoned until the stream drains
+    mock = ctypes.CDLL(lib_path())
+    g.set_option("timeout_ms", 30)
+    mock.gsim_stream_busy(1)
+    errs = []
+    for _ in range(2):
+        try:
+            g.solve(q)
+            errs.append(0)
+        except GstError as e:
+            errs.append(e.status)
+    mock.gsim_stream_busy(0)
+    g.set_option("timeout_ms", 60000)
+    out["poison"] = errs
+    out["poison_ok"] = errs == [GST_DEVICE_ERROR, GST_DEVICE_ERROR] and g.solve(q).cost == exp
+    # server: a mix of bounds through the request loop equals one call each
+    mq = [rng.sample(range(len(groups)), rng.randint(3, 5)) for _ in range(6)]
+    mD = [-1, 2, -1, -1, 3, -1]
+    srv = Server(g)
+    ts = [srv.submit(x, D=d) for x, d in zip(mq, mD)]
+    mixed = [srv.wait(t).cost for t in ts]
+    srv.close()
+    out["mix"] = mixed
+    out["mix_ok"] = mixed == [O.dpbf(rp, col, wt, [groups[y] for y in x], D=d)[0] for x, d in zip(mq, mD)]
+    # server cancel and destroy
+    srv = Server(g)
+    t = srv.submit(q)
+    srv.cancel(t)
+    rc = srv.wait(t)
+    cancel_ok = rc.status == GST_CANCELLED or rc.cost == exp
+    tickets = [srv.submit(q) for _ in range(12)]
+    got = []
+    th = threading.Thread(target=lambda: got.append(srv.wait(tickets[-1])))
+    th.start()
+    srv.close()
+    th.join(timeout=120)
+    out["server_ok"] = bool(cancel_ok and not th.is_alive() and len(got) == 1
+                            and (got[0].status == GST_CANCELLED or got[0].cost == exp))
+    g.close()
+    return out
+
+
 def main():
     stride = int(sys.argv[1]) if len(sys.argv) > 1 else 3
     nq = int(sys.argv[2]) if len(sys.argv) > 2 else 60
     nh = int(sys.argv[3]) if len(sys.argv) > 3 else 24
     out = {"tiny": check_tiny(stride), "jobs": check_jobs(150, nq, 7), "hop": check_hop(150, nh, 11),
-           "branching": check_branching(), "graph": check_graph(), "static": check_static()}
+           "branching": check_branching(), "graph": check_graph(), "static": check_static(),
+           "robust": check_robust()}
     print(json.dumps(out), flush=True)
 
 

@@ -68,3 +68,12 @@ def test_static_entry_on_cpu_build(cpu_results):
     r = cpu_results["static"]
     assert r["checked"] == 5 and r["bad"] == [], r
     assert r["small_buffer_error"], r
+
+
+def test_failure_handling_on_cpu_build(cpu_results):
+    r = cpu_results["robust"]
+    assert r["mix_ok"], r
+    assert r["count_ok"], r
+    assert r["deadline_ok"], r
+    assert r["poison_ok"], r
+    assert r["server_ok"], r

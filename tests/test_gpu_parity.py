@@ -183,3 +183,27 @@ def test_batch_matches_plain():
         _check(inst.rowptr, inst.col, inst.wt, groups, a, expect, D=Ds[i])
         _check(inst.rowptr, inst.col, inst.wt, groups, b, expect, D=Ds[i])
     g.close()
+
+
+@pytest.mark.parametrize("D", [7, 15])
+def test_k6_large_hop_bounds(D):
+    # k = 6 with a large bound: 8 state heights (D = 15), the merge staging
+    # holds few vertices per pass, and the merge predecessor uses its wide
+    # height fields; plain and batched against the oracle
+    rng = random.Random(600 + D)
+    V = 120
+    u, v = data.chung_lu_edges(V, 3 * V, 2.3, 610 + D)
+    rp, col, wt, _ = data.weighted_csr(V, u, v, 1, 20, 611 + D)
+    G = 30
+    gptr, gmem = data.make_groups(V, G, 3, 10, 612 + D)
+    g = Graph(rp, col, wt)
+    g.set_groups(gptr, gmem)
+    qs = [rng.sample(range(G), 6) for _ in range(4)]
+    _say("k = 6, D", D)
+    batch = g.solve_batch(qs, [D] * len(qs))
+    for q, b in zip(qs, batch):
+        groups = [gmem[gptr[x]:gptr[x + 1]] for x in q]
+        expect = O.dpbf(rp, col, wt, groups, D=D)[0]
+        _check(rp, col, wt, groups, g.solve(q, D=D), expect, D=D)
+        _check(rp, col, wt, groups, b, expect, D=D)
+    g.close()

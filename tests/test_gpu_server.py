@@ -79,3 +79,58 @@ def test_poll_and_unknown_ticket():
         srv.poll(123456789)
     srv.close()
     g.close()
+
+
+def test_deadline_and_cancel_on_device():
+    """Cancellation polled by the plan kernel: with a 1 ms deadline the
+    queries of a batch stop on the device (status CANCELLED) or finish with
+    the exact cost; the engine then answers exactly again.  A cancelled server
+    request comes back CANCELLED or exact; destroying a server with requests
+    in flight wakes a waiting thread."""
+    from gpugst_amd.capi import GST_CANCELLED
+    inst, qs, Ds = _instance()
+    g = Graph(inst.rowptr, inst.col, inst.wt)
+    g.set_groups(inst.gptr, inst.gmem)
+    exact = [g.solve(q, D=d).cost for q, d in zip(qs, Ds)]
+    g.set_option("deadline_ms", 1)
+    rs = g.solve_batch(qs * 4, Ds * 4)
+    g.set_option("deadline_ms", 0)
+    n_c = sum(r.status == GST_CANCELLED for r in rs)
+    print("deadline: cancelled", n_c, "of", len(rs), flush=True)
+    assert n_c > 0
+    for i, r in enumerate(rs):
+        assert r.status == GST_CANCELLED or r.cost == exact[i % len(qs)], i
+    assert [r.cost for r in g.solve_batch(qs, Ds)] == exact
+    srv = Server(g)
+    t = srv.submit(qs[0], D=Ds[0])
+    srv.cancel(t)
+    r = srv.wait(t)
+    assert r.status == GST_CANCELLED or r.cost == exact[0]
+    tickets = [srv.submit(q, D=d) for q, d in zip(qs, Ds)]
+    got = []
+    th = threading.Thread(target=lambda: got.append(srv.wait(tickets[-1])))
+    th.start()
+    srv.close()
+    th.join(timeout=60)
+    assert not th.is_alive() and len(got) == 1
+    assert got[0].status == GST_CANCELLED or got[0].cost == exact[-1]
+    g.close()
+
+
+def test_finite_state_count():
+    inst, qs, Ds = _instance()
+    g = Graph(inst.rowptr, inst.col, inst.wt)
+    g.set_groups(inst.gptr, inst.gmem)
+    q = qs[1]
+    r0 = g.solve(q)
+    g.set_option("count_states", 1)
+    r1 = g.solve(q)
+    rb = g.solve_batch([q, q])
+    assert r0.counters["states_finite"] == -1
+    assert r1.cost == r0.cost == rb[0].cost
+    # the singleton states of every vertex the groups reach are finite
+    reach = int(np.count_nonzero(O.sssp(inst.rowptr, inst.col, inst.wt, inst.group(int(q[0]))) < O.INF))
+    for r in (r1, rb[0]):
+        assert r.counters["states_finite"] >= reach * len(q)
+        assert r.counters["max_frontier"] > 0
+    g.close()
