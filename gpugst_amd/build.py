@@ -2,8 +2,8 @@
 
 * ``gpugst_amd/_lib/libgst.so``     -- the engine: HIP kernels for gfx950 plus
   the host engine and the C ABI of ``include/gst.h`` (hipcc).
-* ``gpugst_amd/_lib/libgstdata.so`` -- host-only synthetic-data generators and
-  CSR construction (g++).
+* ``gpugst_amd/_lib/libgstdata.so`` -- host-only synthetic-data generators,
+  CSR construction and the readers of the reference's text formats (g++).
 
 Run ``python -m gpugst_amd.build``.  Nothing here needs a GPU.
 """
@@ -59,7 +59,8 @@ def build(verbose: bool = True) -> None:
     os.replace(tmp, LIBGST)
     tmp = LIBDATA + ".tmp"
     _run([CXX, "-O3", "-std=c++17", "-fPIC", "-shared", "-Wall",
-          os.path.join(CSRC, "gst_data.cpp"), "-o", tmp, "-lpthread"])
+          os.path.join(CSRC, "gst_data.cpp"), os.path.join(CSRC, "gst_formats.cpp"), "-o", tmp,
+          "-lpthread"])
     os.replace(tmp, LIBDATA)
     for o in objs:
         os.remove(o)

@@ -617,8 +617,12 @@ __device__ void item_round(const GraphDev& g, Slot* sp, uint32_t chunk, FrontLDS
 
 // ------------------------------------------------- group-distance jobs ----
 
-// Job SEED: distance 0 at every member of label l's group.
+// Job SEED: distance 0 at every member of label l's group.  A label whose
+// distances came from the cache is final already: it is not seeded, so its
+// chunk bit is only ever set by the other labels of its chunk (relaxing its
+// exact distances again never lowers them).
 __device__ void item_jseed(const GraphDev& g, Slot* sp, uint32_t l) {
+  if (sp->ccol && sp->ccol[l]) return;
   const int nxt = (sp->step + 1) % 3;
   const uint64_t V = (uint64_t)g.V;
   const int32_t grp = sp->lgrp[l];
@@ -755,6 +759,30 @@ __device__ void item_jround(const GraphDev& g, Slot* sp, uint32_t item, FrontLDS
     atomicAdd(&sp->ctr[3], (unsigned long long)n_relax);
     atomicAdd(&sp->ctr[4], (unsigned long long)n_impr);
     atomicAdd(&sp->ctr[5], (unsigned long long)n_atom);  // job: atomicMin issued
+  }
+}
+
+// Job CACHEIN / CACHEOUT over a tile of kStarTV vertices: one lane per label,
+// walking the tile's vertices, so each lane reads (or writes) one contiguous
+// run of its cache column and the lanes together touch a row's labels side by
+// side.  CACHEIN copies cached columns into the rows (after FILL set them to
+// INF); CACHEOUT copies a finished unbounded job's new labels out to their
+// columns.
+__device__ void item_jcache(Slot* sp, uint32_t tile, uint64_t V, bool out) {
+  const uint64_t v0 = (uint64_t)tile * kStarTV;
+  const uint64_t v1 = v0 + kStarTV < V ? v0 + kStarTV : V;
+  const uint64_t rs = (uint64_t)sp->ND * (uint64_t)sp->Lp;
+  const uint32_t L = (uint32_t)sp->L;
+  for (uint32_t l = (uint32_t)lane_id(); l < L; l += 64) {
+    uint32_t* col = sp->ccol[out ? L + l : l];
+    if (!col) continue;
+    g_u32* c = (g_u32*)col;
+    g_u32* d = (g_u32*)sp->dist + l;
+    if (out) {
+      for (uint64_t v = v0; v < v1; v++) c[v] = d[v * rs];
+    } else {
+      for (uint64_t v = v0; v < v1; v++) d[v * rs] = c[v];
+    }
   }
 }
 
@@ -1484,7 +1512,13 @@ __device__ void complete_step(Slot* slots, Slot* s, Ctrl* ctrl) {
   if (s->kind == KIND_JOB) {
     switch (s->phase) {
       case PH_FILL:
+        s->phase = s->ncached ? PH_CACHEIN : PH_SEED;
+        break;
+      case PH_CACHEIN:
         s->phase = PH_SEED;
+        break;
+      case PH_CACHEOUT:
+        s->phase = PH_STAR;
         break;
       case PH_SEED:
         // a hop job computes heights 1..D, each a COPY and one round
@@ -1502,7 +1536,7 @@ __device__ void complete_step(Slot* slots, Slot* s, Ctrl* ctrl) {
         s->ctr[2] += (unsigned long long)(uint32_t)s->app[r % 3];
         if ((uint32_t)(s->app[r % 3] >> 32) > s->maxq) s->maxq = (uint32_t)(s->app[r % 3] >> 32);
         if (s->D < 0) {
-          if ((s->app[(r + 1) % 3] >> 32) == 0) s->phase = PH_STAR;
+          if ((s->app[(r + 1) % 3] >> 32) == 0) s->phase = s->nout ? PH_CACHEOUT : PH_STAR;
         } else if (s->level >= s->D) {
           s->phase = PH_STAR;
         } else {
@@ -1595,7 +1629,9 @@ __device__ uint32_t plan_step(Slot* s, uint64_t V) {
       if (!job) cls = kClsHeavy;  // the per-query grow carries the lower-bound code
       break;
     case PH_STAR:
-    case PH_COPY: work = ceil_div(V, kStarTV); break;
+    case PH_COPY:
+    case PH_CACHEIN:
+    case PH_CACHEOUT: work = ceil_div(V, kStarTV); break;
     case PH_DIST: work = (s->S1 ? ceil_div(V, kDistTV) : 0u) + 1u; cls = kClsHeavy; break;
     case PH_LIVE: work = ceil_div(s->vsrc ? (uint64_t)s->nvsrc : V, kDistTV); cls = kClsHeavy; break;
     case PH_MERGE: work = ceil_div(s->level >= 3 ? s->nact : s->nlive, kMergeTV); cls = kClsHeavy; break;
@@ -1776,6 +1812,8 @@ __device__ void step_body(const GraphDev& g, Slot* slots, Ctrl* ctrl, LDS* L) {
             case PH_ROUND: item_jround(g, sp, local, (FrontLDS*)L); break;
             case PH_STAR: item_jstar(slots, sp, local, V); break;
             case PH_COPY: item_jcopy(sp, local, V); break;
+            case PH_CACHEIN: item_jcache(sp, local, V, false); break;
+            case PH_CACHEOUT: item_jcache(sp, local, V, true); break;
             default: break;
           }
         } else {

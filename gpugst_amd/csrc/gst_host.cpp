@@ -126,6 +126,7 @@ struct Options {
   int64_t trace_slot = 0;  // with time_steps: slot whose phases the step times are binned by
   int64_t job_min_queries = 2;  // fewer unbounded queries admitted together run their own level 1
   int64_t deadline_ms = 0;  // a query still running this long after its submission is cancelled (0: none)
+  int64_t cache_mb = 0;  // device memory for cached group-distance columns (0: no cache)
   int64_t count_states = 0;  // count the finite DP states of every query (an extra pass)
 };
 
@@ -137,6 +138,7 @@ struct gst_graph {
   int occ[gst::kNumCls] = {1, 1};  // blocks per CU of the lean and heavy step kernels
   int64_t V = 0, n_arcs = 0;
   int64_t n_groups = 0;
+  int64_t groups_version = 0;  // bumped by every gst_groups_set (group ids change meaning)
   std::vector<int64_t> h_gptr;
   int64_t* d_rowptr = nullptr;
   int32_t* d_col = nullptr;
@@ -222,7 +224,7 @@ Layout layout_for(const gst_graph* g, int k, int D) { return layout_for(g, k, D,
 
 // Group-distance job over L labels.
 struct JobLayout {
-  size_t dist, mask, list, loff, istart, lgrp, jmem, reach, total;
+  size_t dist, mask, list, loff, istart, lgrp, jmem, reach, ccol, total;
   int Lp, G, chunk;
   uint32_t istride;
 };
@@ -251,6 +253,7 @@ JobLayout job_layout(const gst_graph* g, int L, int nmem, int D) {
   J.lgrp = off;   off += align256((size_t)L * 4);
   J.jmem = off;   off += align256((size_t)nmem * 4);
   J.reach = off;  off += align256(V * 4);
+  J.ccol = off;   off += align256((size_t)L * 2 * sizeof(void*));
   J.total = off;
   return J;
 }
@@ -286,6 +289,17 @@ struct SlotHost {
   bool flagged = false;   // its cancellation flag is set
   int64_t extra[5] = {0, 0, 0, 0, 0};  // level-1 work charged from the query's job
   uint32_t jmaxq = 0;     // largest frontier of its job
+  int32_t cached = 0;     // its labels served from the group-distance cache
+};
+
+// One cached group-distance column (unbounded distances of one group to every
+// vertex), in the engine's arena.
+struct CacheEntry {
+  size_t off = 0;
+  uint64_t last = 0;   // LRU clock of the last admission that used it
+  int refs = 0;        // jobs reading or writing it
+  bool valid = false;  // written by a finished job
+  bool stale = false;  // the groups were replaced while it was referenced
 };
 
 }  // namespace
@@ -468,7 +482,7 @@ class Engine {
     const int s = free_slot();
     if (s < 0) return -1;
     size_t off = 0;
-    if (!arena_.alloc(L.total, &off)) return -1;
+    if (!alloc_or_evict(L.total, &off)) return -1;
     SlotHost& h = sh_[s];
     h.used = true;
     h.job = false;
@@ -481,6 +495,7 @@ class Engine {
     if (h.t_deadline > 0) n_deadline_++;
     h.flagged = false;
     h.jmaxq = 0;
+    h.cached = 0;
     if (h_cancel_) __atomic_store_n(&h_cancel_[1 + s], 0u, __ATOMIC_RELEASE);
     gst::Slot& d = h_slots_[s];
     memset(&d, 0, sizeof(d));
@@ -558,7 +573,7 @@ class Engine {
     const int js = free_slot();
     if (js < 0) return false;
     size_t off = 0;
-    if (!arena_.alloc(J.total, &off)) return false;
+    if (!alloc_or_evict(J.total, &off)) return false;
     SlotHost& h = sh_[js];
     h.used = true;
     h.job = true;
@@ -610,7 +625,130 @@ class Engine {
     std::vector<uint32_t> jm(mem.begin(), mem.end());
     pending_uploads_.push_back({d.lgrp, std::vector<char>((char*)lg.data(), (char*)(lg.data() + lg.size()))});
     pending_uploads_.push_back({d.jmem, std::vector<char>((char*)jm.data(), (char*)(jm.data() + jm.size()))});
+    // group-distance cache (unbounded jobs): a label whose group has a valid
+    // column is copied in instead of computed; a new group gets a column
+    // its job fills at the end, when the budget allows
+    if (D < 0 && g_->opt.cache_mb > 0) {
+      sync_cache_version();
+      std::vector<uint32_t*> cc(2 * (size_t)L, nullptr);
+      std::vector<std::pair<int32_t, bool>>& refs = job_cache_refs_[js];
+      std::unordered_set<int32_t> outs;
+      int lab2 = 0;
+      for (size_t m = 0; m < grp.size(); m++) {
+        for (size_t i = 0; i < grp[m]->groups.size(); i++, lab2++) {
+          const int32_t gid = grp[m]->groups[i];
+          auto it = cache_.find(gid);
+          if (it != cache_.end() && it->second.valid && !it->second.stale) {
+            cc[lab2] = (uint32_t*)(arena_mem_ + it->second.off);
+            it->second.refs++;
+            it->second.last = ++cache_clock_;
+            refs.push_back({gid, false});
+            sh_[mem[m]].cached++;
+            d.ncached++;
+            cache_hits_++;
+            continue;
+          }
+          cache_misses_++;
+          if (it != cache_.end() || outs.count(gid)) continue;  // being written already
+          size_t coff = 0;
+          if (!reserve_column(&coff)) continue;
+          CacheEntry e;
+          e.off = coff;
+          e.refs = 1;
+          e.last = ++cache_clock_;
+          cache_[gid] = e;
+          cache_bytes_ += column_bytes();
+          cc[(size_t)L + lab2] = (uint32_t*)(arena_mem_ + coff);
+          refs.push_back({gid, true});
+          outs.insert(gid);
+          d.nout++;
+          cache_inserts_++;
+        }
+      }
+      if (d.ncached || d.nout) {
+        d.ccol = (uint32_t* const*)(base + J.ccol);
+        pending_uploads_.push_back({base + J.ccol, std::vector<char>((char*)cc.data(), (char*)(cc.data() + cc.size()))});
+      }
+    }
     return true;
+  }
+
+  size_t column_bytes() const { return align256((size_t)g_->V * 4); }
+
+  // Releases the cache entries no job references, least recently used first,
+  // until `need` bytes are free in one run of the arena or `budget` bytes of
+  // cache remain; false when not reached.
+  bool evict_until(size_t need, size_t budget) {
+    for (;;) {
+      if (arena_.largest() >= need && cache_bytes_ <= budget) return true;
+      auto victim = cache_.end();
+      for (auto it = cache_.begin(); it != cache_.end(); ++it)
+        if (it->second.refs == 0 && (victim == cache_.end() || it->second.last < victim->second.last)) victim = it;
+      if (victim == cache_.end()) return false;
+      arena_.release(victim->second.off, column_bytes());
+      cache_bytes_ -= column_bytes();
+      cache_.erase(victim);
+      cache_evictions_++;
+    }
+  }
+
+  // Arena allocation for admission: the cache gives way to queries.
+  bool alloc_or_evict(size_t len, size_t* off) {
+    if (arena_.alloc(len, off)) return true;
+    if (cache_.empty() || !evict_until(align256(len), cache_bytes_)) return false;
+    return arena_.alloc(len, off);
+  }
+
+  // A column for a new cache entry within the cache budget.
+  bool reserve_column(size_t* off) {
+    const size_t budget = (size_t)g_->opt.cache_mb << 20;
+    const size_t cb = column_bytes();
+    if (cb > budget) return false;
+    if (cache_bytes_ + cb > budget && !evict_until(0, budget - cb)) return false;
+    return arena_.alloc(cb, off);
+  }
+
+  // New group lists make every column meaningless: unreferenced ones go now,
+  // referenced ones when their job ends.
+  void sync_cache_version() {
+    if (cache_version_ == g_->groups_version) return;
+    cache_version_ = g_->groups_version;
+    for (auto it = cache_.begin(); it != cache_.end();) {
+      if (it->second.refs == 0) {
+        arena_.release(it->second.off, column_bytes());
+        cache_bytes_ -= column_bytes();
+        it = cache_.erase(it);
+      } else {
+        it->second.stale = true;
+        ++it;
+      }
+    }
+  }
+
+  // A finished job hands back its cache references; the columns it wrote
+  // become valid.
+  void release_job_cache(int js) {
+    auto jr = job_cache_refs_.find(js);
+    if (jr == job_cache_refs_.end()) return;
+    for (auto& pr : jr->second) {
+      auto it = cache_.find(pr.first);
+      if (it == cache_.end()) continue;
+      it->second.refs--;
+      if (pr.second) it->second.valid = true;
+      if (it->second.stale && it->second.refs == 0) {
+        arena_.release(it->second.off, column_bytes());
+        cache_bytes_ -= column_bytes();
+        cache_.erase(it);
+      }
+    }
+    job_cache_refs_.erase(jr);
+  }
+
+  void clear_cache() {
+    for (auto& kv : cache_) arena_.release(kv.second.off, column_bytes());
+    cache_.clear();
+    cache_bytes_ = 0;
+    job_cache_refs_.clear();
   }
 
  public:
@@ -740,6 +878,7 @@ class Engine {
         jobs_done_++;
         for (int t = 0; t < 6; t++) job_ctr_[t] += (int64_t)j.ctr[t];
         sh_[i].charged = true;
+        release_job_cache(i);
       } else {
         finished->push_back(i);
       }
@@ -885,6 +1024,7 @@ class Engine {
     out->counters.shared_relaxations = h.extra[3];
     out->counters.active_vertices = (int64_t)d.nact;
     out->counters.max_frontier = (int64_t)std::max(d.maxq, h.jmaxq);
+    out->counters.cached_labels = h.cached;
     out->counters.states_finite = d.count ? (int64_t)d.nfinite : -1;
     gst_status st = GST_OK;
     if (d.status == gst::kStatusCancelled) {
@@ -942,6 +1082,7 @@ class Engine {
   // Drops every admitted query and job (after a device error) so the engine
   // is reusable.
   void reset_slots() {
+    clear_cache();
     for (int i = 0; i < max_slots_; i++) {
       if (sh_[i].used) arena_.release(sh_[i].off, sh_[i].len);
       sh_[i] = SlotHost();
@@ -998,6 +1139,16 @@ class Engine {
   const uint32_t* d_cancel_ = nullptr;  // the same words as the device reads them
   int n_flagged_ = 0;       // slots whose flag is set
   int n_deadline_ = 0;      // admitted queries with a deadline
+  // group-distance cache: group id -> column in the arena
+  std::unordered_map<int32_t, CacheEntry> cache_;
+  std::map<int, std::vector<std::pair<int32_t, bool>>> job_cache_refs_;  // job slot -> (group, writes it)
+  size_t cache_bytes_ = 0;
+  uint64_t cache_clock_ = 0;
+  int64_t cache_version_ = -1;
+ public:
+  int64_t cache_hits_ = 0, cache_misses_ = 0, cache_inserts_ = 0, cache_evictions_ = 0;
+  size_t cache_entries() const { return cache_.size(); }
+ private:
   hipStream_t stream_ = nullptr;
   gst::Slot* d_slots_ = nullptr;
   gst::Slot* h_slots_ = nullptr;
@@ -1292,6 +1443,7 @@ gst_status gst_groups_set(gst_graph* g, const int64_t* grp_ptr, const int32_t* m
   if (nm) HIP_TRY(hipMemcpy(g->d_gmem, members, sizeof(int32_t) * nm, hipMemcpyHostToDevice));
   g->h_gptr.assign(grp_ptr, grp_ptr + n_groups + 1);
   g->n_groups = n_groups;
+  g->groups_version++;
   return GST_OK;
 }
 
@@ -1309,6 +1461,7 @@ static int64_t* option_ref(gst_graph* g, const std::string& k) {
   if (k == "job_min_queries") return &g->opt.job_min_queries;
   if (k == "deadline_ms") return &g->opt.deadline_ms;
   if (k == "count_states") return &g->opt.count_states;
+  if (k == "cache_mb") return &g->opt.cache_mb;
   return nullptr;
 }
 
@@ -1328,7 +1481,7 @@ gst_status gst_set_option(gst_graph* g, const char* key, int64_t value) {
   if (k == "chunk" && (value < 64 || value > (1 << 20))) return fail(GST_INVALID_ARG, "chunk outside [64, 2^20]");
   if (k == "edge_cap" && (value < 64 || value > (1 << 26))) return fail(GST_INVALID_ARG, "edge_cap outside [64, 2^26]");
   if ((k == "budget_mb" || k == "blocks_per_cu" || k == "timeout_ms" || k == "trace_slot" || k == "deadline_ms" ||
-       k == "count_states") && value < 0)
+       k == "count_states" || k == "cache_mb") && value < 0)
     return fail(GST_INVALID_ARG, "negative value");
   if (k == "job_min_queries" && (value < 1 || value > gst::kJobLabels))
     return fail(GST_INVALID_ARG, "job_min_queries outside [1,%d]", gst::kJobLabels);
@@ -1357,6 +1510,11 @@ gst_status gst_get_option(gst_graph* g, const char* key, int64_t* value) {
     if (f == "job_relaxations") { *value = e ? e->job_ctr(3) : 0; return GST_OK; }
     if (f == "job_improvements") { *value = e ? e->job_ctr(4) : 0; return GST_OK; }
     if (f == "job_atomics") { *value = e ? e->job_ctr(5) : 0; return GST_OK; }
+    if (f == "cache_hits") { *value = e ? e->cache_hits_ : 0; return GST_OK; }
+    if (f == "cache_misses") { *value = e ? e->cache_misses_ : 0; return GST_OK; }
+    if (f == "cache_inserts") { *value = e ? e->cache_inserts_ : 0; return GST_OK; }
+    if (f == "cache_evictions") { *value = e ? e->cache_evictions_ : 0; return GST_OK; }
+    if (f == "cache_entries") { *value = e ? (int64_t)e->cache_entries() : 0; return GST_OK; }
     // phase_ns:<phase>:<level>, phase_steps:<phase>:<level>
     int ph = 0, lv = 0;
     if (sscanf(f.c_str(), "phase_ns:%d:%d", &ph, &lv) == 2) {

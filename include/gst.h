@@ -94,6 +94,7 @@ typedef struct {
   int64_t states_finite;    /* finite DP states (v, p[, d]) at the end: singletons over every height
                                plus the stored states of two or more groups; -1 unless the option
                                "count_states" was set when the query was admitted */
+  int64_t cached_labels;    /* its groups whose distances came from the group-distance cache */
 } gst_counters;
 
 typedef struct {
@@ -166,7 +167,11 @@ void gst_result_free(gst_result *r);
  * query of gst_solve / gst_solve_batch / gst_submit still running this long
  * after its submission is stopped on the device at its next step and
  * answered with status GST_CANCELLED), "count_states" (1: fill
- * counters.states_finite with an extra pass), "graph" (replay engine steps as a hipGraph),
+ * counters.states_finite with an extra pass), "cache_mb" (device memory of
+ * an engine's group-distance cache: columns of exact distances of one group
+ * to every vertex, kept across batches and server epochs, least recently used
+ * first out, evicted before admission fails; 0: no cache; used by queries
+ * without a hop bound that share a group-distance job), "graph" (replay engine steps as a hipGraph),
  * "group_pass" (1: unbounded queries share group-distance jobs; 0: each
  * query runs its own level 1), "job_min_queries" (default 2: fewer
  * unbounded queries admitted together run their own level 1 instead of a
@@ -176,7 +181,8 @@ void gst_result_free(gst_result *r);
  * Read-only: "num_cu", "occupancy", "occupancy_heavy", "grid",
  * "grid_heavy", "{plain,batch}_{kernel_ns,launches,steps,lean_ns,heavy_ns,
  * timed_steps,jobs,job_entries,job_arcs,job_relaxations,job_improvements,
- * job_atomics}", "{plain,batch}_phase_ns:<phase>:<level>" and
+ * job_atomics,cache_hits,cache_misses,cache_inserts,cache_evictions,
+ * cache_entries}", "{plain,batch}_phase_ns:<phase>:<level>" and
  * "{plain,batch}_phase_steps:<phase>:<level>" (step time and step count of
  * the traced slot per engine phase).
  * "reset_stats" (set) clears the statistics.  Unknown keys return

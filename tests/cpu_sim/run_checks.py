@@ -311,13 +311,53 @@ def check_robust():
     return out
 
 
+def check_cache():
+    """Group-distance cache across batches (unbounded jobs): a second batch
+    over the same groups reads columns written by the first (hits, labels
+    counted per query); single-query jobs reuse the columns of earlier ones;
+    new group lists retire the cache; every cost equals the oracle."""
+    rng, rp, col, wt, groups = random_instance(150, 29)
+    G = len(groups)
+    b1 = [rng.sample(range(G), 4) for _ in range(12)]
+    b2 = [rng.sample(range(G), 4) for _ in range(12)]
+    exp1 = [O.dpbf(rp, col, wt, [groups[x] for x in q])[0] for q in b1]
+    exp2 = [O.dpbf(rp, col, wt, [groups[x] for x in q])[0] for q in b2]
+    g = Graph(rp, col, wt)
+    g.set_groups(*groups_csr(groups))
+    g.set_option("cache_mb", 1)
+    r1 = g.solve_batch(b1)
+    h1 = g.get_option("batch_cache_hits")
+    r2 = g.solve_batch(b2)
+    h2 = g.get_option("batch_cache_hits") - h1
+    out = {"hits_first": h1, "hits_second": h2, "inserts": g.get_option("batch_cache_inserts"),
+           "per_query": sum(r.counters["cached_labels"] for r in r2)}
+    out["exact"] = [r.cost for r in r1] == exp1 and [r.cost for r in r2] == exp2
+    # one query per batch: a job of one query fills the cache for the next
+    g2 = Graph(rp, col, wt)
+    g2.set_groups(*groups_csr(groups))
+    g2.set_option("cache_mb", 1)
+    g2.set_option("job_min_queries", 1)
+    r3 = [g2.solve_batch([q])[0].cost for q in b1[:4] + b1[:2]]
+    out["single_exact"] = r3 == exp1[:4] + exp1[:2] and g2.get_option("batch_cache_hits") >= 8
+    # new group lists: the cache must not serve the old distances
+    g.set_option("cache_mb", 1)
+    g.solve_batch(b1)
+    perm = groups[1:] + groups[:1]
+    g.set_groups(*groups_csr(perm))
+    r5 = g.solve_batch(b1)
+    out["regroup_exact"] = [r.cost for r in r5] == [O.dpbf(rp, col, wt, [perm[x] for x in q])[0] for q in b1]
+    g.close()
+    g2.close()
+    return out
+
+
 def main():
     stride = int(sys.argv[1]) if len(sys.argv) > 1 else 3
     nq = int(sys.argv[2]) if len(sys.argv) > 2 else 60
     nh = int(sys.argv[3]) if len(sys.argv) > 3 else 24
     out = {"tiny": check_tiny(stride), "jobs": check_jobs(150, nq, 7), "hop": check_hop(150, nh, 11),
            "branching": check_branching(), "graph": check_graph(), "static": check_static(),
-           "robust": check_robust()}
+           "robust": check_robust(), "cache": check_cache()}
     print(json.dumps(out), flush=True)
 
 

@@ -77,3 +77,51 @@ def test_twitch_excerpt_matches_reference(golden_dir):
     gp, gm = data.read_groups(os.path.join(REF_DATA, "Twitch.g"))
     assert np.array_equal(gm[gp[0]:gp[1]], z["g0"])
     assert np.array_equal(np.diff(gp)[:256], z["sizes_head"])
+
+
+def test_native_readers_edge_cases(tmp_path):
+    # .in: first field of the first line is V; only 3-field lines are edges;
+    # blanks of any kind separate fields; CRLF endings
+    p = tmp_path / "e.in"
+    p.write_bytes(b"5 3\n0 1 7\r\n1  2\t9 \n\n3 4\n2 4 1\n")
+    V, u, v, w = data.read_edges(str(p))
+    assert V == 5 and u.tolist() == [0, 1, 2] and v.tolist() == [1, 2, 4] and w.tolist() == [7, 9, 1]
+    # a vertex id outside [0, V) or a non-integer field is an error with its line
+    p.write_bytes(b"3 1\n0 1 2\n0 5 1\n")
+    with pytest.raises(ValueError, match="line 3"):
+        data.read_edges(str(p))
+    p.write_bytes(b"3 1\n0 1 x\n")
+    with pytest.raises(ValueError, match="line 2"):
+        data.read_edges(str(p))
+    # .g: "gN:" is group N-1 whatever the line order; members sorted; a
+    # trailing blank and missing groups are fine
+    g = tmp_path / "x.g"
+    g.write_text("g3:9 4 \ng1:2 1\n\ng5:7\n")
+    gp, gm = data.read_groups(str(g))
+    assert gp.tolist() == [0, 2, 2, 4, 4, 5] and gm.tolist() == [1, 2, 4, 9, 7]
+    g.write_text("g1:1 2\nh2:3\n")
+    with pytest.raises(ValueError, match="line 2"):
+        data.read_groups(str(g))
+    q = tmp_path / "x3.csv"
+    q.write_text("1 2 3 \n\n4 5 6\n")
+    assert [a.tolist() for a in data.read_queries(str(q))] == [[1, 2, 3], [4, 5, 6]]
+    with pytest.raises(OSError):
+        data.read_queries(str(tmp_path / "missing.csv"))
+
+
+def test_native_reader_large_file(tmp_path):
+    # 3M edges in a few seconds, multithreaded, identical to the arrays written
+    rng = np.random.default_rng(3)
+    V, m = 1_000_000, 3_000_000
+    u = rng.integers(0, V, m).astype(np.int32)
+    v = rng.integers(0, V, m).astype(np.int32)
+    w = rng.integers(1, 1000, m).astype(np.int32)
+    p = str(tmp_path / "big.in")
+    with open(p, "w") as f:
+        f.write(f"{V} {m}\n")
+        np.savetxt(f, np.stack([u, v, w], 1), fmt="%d")
+    import time
+    t = time.time()
+    V2, u2, v2, w2 = data.read_edges(p)
+    assert time.time() - t < 10
+    assert V2 == V and np.array_equal(u2, u) and np.array_equal(v2, v) and np.array_equal(w2, w)

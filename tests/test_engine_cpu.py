@@ -77,3 +77,9 @@ def test_failure_handling_on_cpu_build(cpu_results):
     assert r["deadline_ok"], r
     assert r["poison_ok"], r
     assert r["server_ok"], r
+
+
+def test_group_distance_cache_on_cpu_build(cpu_results):
+    c = cpu_results["cache"]
+    assert c["exact"] and c["single_exact"] and c["regroup_exact"], c
+    assert c["inserts"] > 0 and c["hits_second"] > 0 and c["per_query"] == c["hits_second"], c
