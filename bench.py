@@ -18,9 +18,17 @@ replay, admitted slots) come from the autotuner's results file when it holds
 this graph and query shape (``--tune`` re-tunes on a sample batch first).
 
 Roofline: one extra batch (not in the timed steps) runs with every step's
-lean and heavy kernels bracketed by HIP events on the engine stream; the
-dominant kernel's average launch time and the algorithmic bytes its items
-moved (from the device counters, model below) give `achieved`.
+lean and heavy kernels bracketed by HIP events on the engine stream; each
+kernel's average launch time and the algorithmic bytes its items moved (from
+the device counters, models below) give its `achieved`; the dominant one is
+`roofline`, both are under `roofline.kernels`, with the measured HBM bytes and
+L2 atomics per launch from the committed rocprofv3 summary.
+
+After the headline legs (plain, batched, roofline, serving on the same graph)
+come the other configurations on one GPU: k = 5 with hop bound D = 4
+(`configs.d4`), 4,096 mixed-k queries in one batch (`configs.kmix_4096`), the
+CPU baseline on the host cores, and the 3M-vertex serving configuration at
+three offered rates (`serving_c5`).  `--no-extra` skips the extra legs.
 """
 import argparse
 import json
@@ -94,16 +102,23 @@ def heavy_bytes(res_list, V, ks, ND=1):
     return total
 
 
-def profiled_traffic(kernel):
-    """HBM bytes per launch of `kernel` on this workload, from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the copy calibration),
-    or None."""
+PROFILE = "step_kernels_rmat1.1M_k5_batch256.md"
+# MI355X L2 atomic rate used as the atomics roofline: 8 XCDs x 16 L2 channels
+# x 1 atomic per channel per clock x 2.4 GHz (an estimate from the cache
+# organisation, not a published figure)
+L2_ATOMIC_PEAK = 8 * 16 * 2.4e9
+
+
+def profiled(kernel, what):
+    """Per-launch figure of `kernel` on the headline workload from the committed
+    rocprofv3 summary: what = "traffic_bytes_per_launch" (FETCH_SIZE x2.0 +
+    WRITE_SIZE x1.0, the copy calibration) or "l2_atomics_per_launch"
+    (TCC_ATOMIC_sum), or None."""
     import os
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                        "step_kernels_rmat1.1M_k5_batch256.md")
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", PROFILE)
     try:
         for line in open(path):
-            if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
+            if line.startswith(f"{what} {kernel}:"):
                 return int(line.split(":")[1])
     except OSError:
         pass
@@ -140,44 +155,196 @@ def poisson_serve(g, qs, Ds, rate, seed):
     return n / span, pct(lat, 50), pct(lat, 99), res
 
 
-def serve_config5(log):
+def hit_rate(res, ks):
+    """Share of the requests' groups served from the group-distance cache."""
+    tot = sum(ks)
+    return round(sum(r.counters.get("cached_labels", 0) for r in res) / tot, 4) if tot else None
+
+
+def serve_config5(log, n_req=96, cache_mb=4096):
     """The serving configuration: Chung-Lu graph, V = 3.0M, 117M undirected
     edges (exponent 2.0, weights in [1, 1000]), 2M groups (geometric sizes,
     mean 10, max 5,000); requests with k uniform in {3..6}, 20% with hop bound
-    D = 3.  Capacity = one batch of 96 requests; then 96 Poisson requests at
-    each of 0.5x, 0.8x and 1.1x of it, every answer checked against a one-query
-    call for the first 16."""
+    D = 3.  Capacity = one batch of n_req requests; then n_req Poisson
+    requests at each of 0.5x, 0.8x and 1.1x of it, the first 16 answers of
+    the first rate checked against one-query calls.  The group-distance cache
+    is on (cache_mb); its hit rate is reported per rate."""
     from gpugst_amd import Graph, data
     t0 = time.time()
-    inst = data.make_config(5, nq=96 * 5)
+    inst = data.make_config(5, nq=n_req * 5)
     log(f"serving config: V={inst.V} arcs={inst.col.shape[0]} groups={inst.gptr.shape[0]-1} "
         f"gen {time.time()-t0:.1f}s")
     g = Graph(inst.rowptr, inst.col, inst.wt)
     g.set_groups(inst.gptr, inst.gmem)
+    g.set_option("cache_mb", cache_mb)
     qs, Ds = list(inst.queries), list(inst.D)
     g.solve_batch(qs[:16], Ds[:16])  # warm-up
     tb = time.perf_counter()
-    rb = g.solve_batch(qs[16:112], Ds[16:112])
-    cap = 96 / (time.perf_counter() - tb)
+    rb = g.solve_batch(qs[16:16 + n_req], Ds[16:16 + n_req])
+    cap = n_req / (time.perf_counter() - tb)
     log(f"serving config: batched capacity {cap:.1f} q/s")
     g.set_option("budget_mb", 0)  # the batch engine hands its arena back to the server
     out = {"workload": "chunglu-3M-117M-edges, k in {3..6}, 20% D=3", "batched_capacity_qps": round(cap, 1),
-           "rates": []}
+           "requests_per_rate": n_req, "cache_mb": cache_mb, "rates": []}
     mism = 0
     for i, f in enumerate((0.5, 0.8, 1.1)):
-        a = 112 + 96 * i
-        q, d = qs[a:a + 96], Ds[a:a + 96]
+        a = 16 + n_req * (1 + i)
+        q, d = qs[a:a + n_req], Ds[a:a + n_req]
         ach, p50, p99, res = poisson_serve(g, q, d, f * cap, 21 + i)
         if i == 0:
             for j in range(16):
                 mism += int(g.solve(q[j], D=d[j]).cost != res[j].cost)
         out["rates"].append({"offered_qps": round(f * cap, 1), "achieved_qps": round(ach, 1),
-                             "p50_ms": round(p50, 1), "p99_ms": round(p99, 1), "requests": len(q)})
+                             "p50_ms": round(p50, 1), "p99_ms": round(p99, 1),
+                             "cache_hit_rate": hit_rate(res, [len(x) for x in q])})
         log(f"serving config: offered {f*cap:.1f} q/s -> {ach:.1f} q/s, p50 {p50:.1f} ms p99 {p99:.1f} ms")
     out["mix_vs_alone_mismatches"] = mism
     out["batch_infeasible"] = sum(1 for r in rb if not r.feasible)
     g.close()
     return out
+
+
+def hop_leg(g, inst, log, batch=128, steps=2, plain=16):
+    """The diameter-bounded configuration on the same graph: k = 5, D = 4,
+    batches of `batch` queries (one untimed warm-up batch), plain one call at
+    a time; batched costs equal the plain ones on the shared queries."""
+    from gpugst_amd import data
+    n = batch * (steps + 1)
+    qs = data.make_queries(n, 5, inst.meta["eligible"], 3000 + 4 + 10 * 5 + 3)
+    Ds = [4] * n
+    tp = time.perf_counter()
+    pr = [g.solve(q, D=4) for q in qs[:plain]]
+    plain_qps = plain / (time.perf_counter() - tp)
+    g.solve_batch(list(qs[:batch]), Ds[:batch])
+    tb = time.perf_counter()
+    res = []
+    for s in range(steps):
+        a = batch * (1 + s)
+        res += g.solve_batch(list(qs[a:a + batch]), Ds[a:a + batch])
+    bq = batch * steps / (time.perf_counter() - tb)
+    chk = g.solve_batch(list(qs[:plain]), Ds[:plain])
+    mism = sum(int(a.cost != b.cost) for a, b in zip(pr, chk))
+    log(f"D=4 leg: batched {bq:.1f} q/s, plain {plain_qps:.1f} q/s, mismatches {mism}")
+    return {"workload": "rmat-1.1M k=5 D=4", "batch": batch, "batched_qps": round(bq, 1),
+            "plain_qps": round(plain_qps, 1), "batched_vs_plain": round(bq / plain_qps, 2),
+            "plain_vs_batched_mismatches": mism, "infeasible": sum(1 for r in res if not r.feasible)}
+
+
+def kmix_leg(g, inst, log, n=4096, check=32):
+    """The batched persistent configuration: 4,096 queries with k uniform in
+    {3..6}, unbounded, in one gst_solve_batch (continuous admission inside the
+    engine); the first `check` answers against one call each."""
+    from gpugst_amd import data
+    ks = data.uniform_ints(n, 3, 6, 4004)
+    qs = [data.make_queries(1, int(ks[i]), inst.meta["eligible"], 4004 * 1_000_003 + i)[0] for i in range(n)]
+    t = time.perf_counter()
+    res = g.solve_batch(qs)
+    dt = time.perf_counter() - t
+    dev = [r.device_ms for r in res]
+    mism = sum(int(g.solve(qs[i]).cost != res[i].cost) for i in range(check))
+    log(f"k-mix leg: {n} queries in {dt:.2f}s -> {n/dt:.1f} q/s, mismatches {mism}")
+    return {"workload": "rmat-1.1M k in {3..6} unbounded, one batch of 4096", "queries": n,
+            "qps": round(n / dt, 1), "device_p50_ms": round(pct(dev, 50), 1),
+            "device_p99_ms": round(pct(dev, 99), 1), "plain_vs_batched_mismatches": mism, "checked": check}
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_leg(inst, queries, gpu_res, D, seconds, log):
+    """The CPU baseline on the host cores: the oracle's exact solver (pruned
+    best-first DPBF for unbounded queries, the hop DP otherwise) over a bounded
+    sample of the same queries -- one thread for `seconds` (per-query
+    latency, and the reference's convention of timing the main DP loop only,
+    CPUNONHOP.h:634-665), then one query per thread on up to 16 threads (the
+    box's CPU share) for about as long.  Every answer is compared with the
+    GPU's."""
+    import os
+    from concurrent.futures import ThreadPoolExecutor
+    import oracle
+    oracle.lib()
+
+    def one(i):
+        groups = [inst.group(int(x)) for x in queries[i]]
+        t = time.perf_counter()
+        if D < 0:
+            c, st = oracle.dpbf_pruned(inst.rowptr, inst.col, inst.wt, groups)
+            loop = st[3] / 1e9
+        else:
+            c = oracle.dpbf(inst.rowptr, inst.col, inst.wt, groups, D=D)[0]
+            loop = None
+        return c, time.perf_counter() - t, loop
+
+    lat, loops, mism, n = [], [], 0, 0
+    tc = time.perf_counter()
+    while n < len(gpu_res) and time.perf_counter() - tc < seconds:
+        c, dt, lp = one(n)
+        mism += int(c != gpu_res[n].cost)
+        lat.append(dt)
+        if lp is not None:
+            loops.append(lp)
+        n += 1
+    single = n / (time.perf_counter() - tc)
+    threads = max(1, min(len(os.sched_getaffinity(0)), 16))
+    m = min(len(gpu_res), max(threads, int(single * seconds * threads)))
+    tm = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        outs = list(ex.map(one, range(m)))
+    multi = m / (time.perf_counter() - tm)
+    mism += sum(int(o[0] != gpu_res[i].cost) for i, o in enumerate(outs))
+    algo = ("exact pruned best-first DPBF (orc_dpbf_pruned, the exact-mode PrunedDP++ restatement)"
+            if D < 0 else "exact hop-bounded DP (orc_dpbf, the HOP_cpu restatement)")
+    log(f"cpu baseline: 1 thread {single:.2f} q/s ({n} queries), {threads} threads {multi:.2f} q/s "
+        f"({m} queries), mismatches {mism}")
+    return {"value": round(multi, 3), "unit": "queries/s", "cores": threads, "kind": "port",
+            "sample": f"{algo} in oracle/gst_oracle.c on the same queries: first {n} on one thread "
+                      f"({sum(lat):.1f} s), then the first {m} one per thread on {threads} threads",
+            "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
+            "single_thread_qps": round(single, 3),
+            "single_thread_p50_ms": round(pct([x * 1e3 for x in lat], 50), 1),
+            "single_thread_p99_ms": round(pct([x * 1e3 for x in lat], 99), 1),
+            "main_loop_only_qps": round(len(loops) / sum(loops), 3) if loops and sum(loops) > 0 else None,
+            "oracle_checked": n + m, "oracle_mismatches": mism}
+
+
+def roofline_pass(g, qs, Ds, V, D, log):
+    """One further batch with HIP events around every step kernel on the
+    engine stream: average launch time of the lean and heavy kernels, their
+    algorithmic bytes per launch (device counters, byte models above), and
+    the job rounds' atomics per launch; the rocprofv3 summary in profiles/
+    gives the measured HBM bytes and L2 atomics of the same workload."""
+    g.set_option("reset_stats", 0)
+    use_graph = g.get_option("graph")
+    g.set_option("graph", 0)  # events bracket eager launches
+    g.set_option("time_steps", 1)
+    rr = g.solve_batch(list(qs), Ds)
+    g.set_option("time_steps", 0)
+    g.set_option("graph", use_graph)
+    n_steps = max(1, g.get_option("batch_timed_steps"))
+    ND = 1 if D < 0 else D + 1
+    rk = [len(q) for q in qs]
+    kern = {}
+    for name, ms, byts in (("gst_lean_kernel", g.get_option("batch_lean_ns") / 1e6 / n_steps, lean_bytes(g, rr, V, rk, ND)),
+                           ("gst_heavy_kernel", g.get_option("batch_heavy_ns") / 1e6 / n_steps, heavy_bytes(rr, V, rk, ND))):
+        ach = byts / n_steps / (ms * 1e-3) / 1e9 if (byts and ms > 0) else 0.0
+        kern[name] = {"avg_launch_ms": round(ms, 4), "algorithmic_bytes_per_launch": int(byts / n_steps),
+                      "achieved": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4)}
+    job_atomics = g.get_option("batch_job_atomics") / n_steps
+    lean = kern["gst_lean_kernel"]
+    lean["job_atomics_per_launch"] = int(job_atomics)
+    lean["job_atomics_per_s"] = round(job_atomics / (lean["avg_launch_ms"] * 1e-3), 1) if lean["avg_launch_ms"] else None
+    log(f"roofline pass: {n_steps} steps; " + "; ".join(
+        f"{k} {v['avg_launch_ms']:.3f} ms/launch {v['achieved']:.0f} GB/s" for k, v in kern.items()))
+    log("job counters: " + " ".join(f"{f}={g.get_option('batch_job_' + f)}" for f in
+                                    ("entries", "arcs", "relaxations", "improvements", "atomics")))
+    return kern, int(n_steps), len(rr)
 
 
 def main():
@@ -192,16 +359,16 @@ def main():
                     help="k drawn uniformly from {3,4,5,6} per query (the mixed-k batched workload)")
     ap.add_argument("--D", type=int, default=-1)
     ap.add_argument("--scale", type=float, default=1.0, help="shrink the graph (tests only)")
-    ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU oracle sample budget")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU baseline budget per leg (s)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the D=4, 4096 mixed-k and 3M-vertex serving legs")
     ap.add_argument("--max-slots", type=int, default=0)
     ap.add_argument("--serve", type=int, default=512, help="queries of the serving leg (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=0.0,
                     help="offered queries/s of the serving leg (default: half the batched throughput)")
-    ap.add_argument("--serve-c5", action="store_true",
-                    help="also serve the 3M-vertex serving configuration (k in {3..6}, 20%% D=3) "
-                         "at 0.5x, 0.8x and 1.1x of its batched capacity")
+    ap.add_argument("--cache-mb", type=int, default=4096, help="group-distance cache of the serving legs")
     ap.add_argument("--tune", action="store_true",
                     help="run the schedule autotuner on a sample batch before the batched leg")
     ap.add_argument("--tune-db", default=None,
@@ -258,11 +425,12 @@ def main():
     # ---------------- schedule of the batched engine: the autotuner's choice
     from gpugst_amd import autotune
     db = args.tune_db or autotune.DEFAULT_DB
+    kb = "mix" if args.kmix else args.k
     if args.tune:
         sample = list(queries[:args.batch])
-        schedule = autotune.tune(g, sample, D=args.D, path=db, log=log)["best"]
+        schedule = autotune.tune(g, sample, D=args.D, path=db, log=log, kkey=kb)["best"]
     else:
-        schedule = autotune.apply(g, args.k, args.D, path=db)
+        schedule = autotune.apply(g, kb, args.D, path=db)
     if args.max_slots:
         g.set_option("max_slots", args.max_slots)
     log(f"schedule: {schedule if schedule else 'engine defaults (no tuned entry for this graph and shape)'}")
@@ -296,72 +464,46 @@ def main():
     roof = None
     if n_roof:
         a = (args.warmup + args.steps) * args.batch
-        qs = queries[a:a + args.batch]
-        g.set_option("reset_stats", 0)
-        use_graph = g.get_option("graph")
-        g.set_option("graph", 0)  # events bracket eager launches
-        g.set_option("time_steps", 1)
-        rr = g.solve_batch(list(qs), Ds[:len(qs)])
-        g.set_option("time_steps", 0)
-        g.set_option("graph", use_graph)
-        n_steps = max(1, g.get_option("batch_timed_steps"))
-        lean_ms = g.get_option("batch_lean_ns") / 1e6 / n_steps
-        heavy_ms = g.get_option("batch_heavy_ns") / 1e6 / n_steps
-        rk = [len(q) for q in qs]
-        lb = lean_bytes(g, rr, V, rk, 1 if args.D < 0 else args.D + 1)
-        hb = heavy_bytes(rr, V, rk, 1 if args.D < 0 else args.D + 1)
-        dom = "gst_lean_kernel" if lean_ms >= heavy_ms else "gst_heavy_kernel"
-        ms, byts = (lean_ms, lb) if dom == "gst_lean_kernel" else (heavy_ms, hb)
-        achieved = byts / n_steps / (ms * 1e-3) / 1e9 if (byts and ms > 0) else 0.0
-        traffic = profiled_traffic(dom) if (args.D < 0 and args.batch == 256 and args.k == 5
-                                            and not args.kmix) else None
-        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
-                "algorithmic_bytes_per_launch": int(byts / n_steps) if byts else None,
-                "avg_launch_ms": round(ms, 4), "launches": int(n_steps),
-                "other_kernel_avg_ms": round(heavy_ms if dom == "gst_lean_kernel" else lean_ms, 4),
-                "traffic_source": "profiles/step_kernels_rmat1.1M_k5_batch256.md (rocprofv3 FETCH_SIZE x2.0 "
-                                  "+ WRITE_SIZE x1.0 per launch, same workload)" if traffic else None,
-                "pass": f"{len(rr)} further queries of the same workload, events around every step kernel"}
-        log(f"roofline pass: {n_steps} steps, lean {lean_ms:.3f} ms/launch, heavy {heavy_ms:.3f} ms/launch, "
-            f"{dom} {achieved:.1f} GB/s algorithmic")
-        log("job counters: " + " ".join(f"{f}={g.get_option('batch_job_' + f)}" for f in
-                                        ("entries", "arcs", "relaxations", "improvements", "atomics")))
+        kern, n_steps, n_q = roofline_pass(g, queries[a:a + args.batch], Ds[:args.batch], V, args.D, log)
+        headline = args.D < 0 and args.batch == 256 and args.k == 5 and not args.kmix and args.scale == 1.0
+        for name, kv in kern.items():
+            kv["traffic"] = profiled(name, "traffic_bytes_per_launch") if headline else None
+            at = profiled(name, "l2_atomics_per_launch") if headline else None
+            kv["l2_atomics_per_launch"] = at
+            kv["l2_atomics_per_s"] = round(at / (kv["avg_launch_ms"] * 1e-3), 1) if at else None
+        dom = max(kern, key=lambda k: kern[k]["avg_launch_ms"])
+        d = kern[dom]
+        roof = {"bound": "hbm", "achieved": d["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": d["frac"], "traffic": d["traffic"], "kernel": dom,
+                "algorithmic_bytes_per_launch": d["algorithmic_bytes_per_launch"],
+                "avg_launch_ms": d["avg_launch_ms"], "launches": n_steps,
+                "atomics": {"kernel": dom, "l2_atomics_per_s": d.get("l2_atomics_per_s"),
+                            "peak_per_s": L2_ATOMIC_PEAK,
+                            "frac": round(d["l2_atomics_per_s"] / L2_ATOMIC_PEAK, 4)
+                            if d.get("l2_atomics_per_s") else None,
+                            "source": f"profiles/{PROFILE} (rocprofv3 TCC_ATOMIC_sum per launch, same "
+                                      "workload) over the live launch time; peak = 8 XCDs x 16 L2 "
+                                      "channels x 2.4 GHz, an estimate"},
+                "kernels": kern,
+                "traffic_source": f"profiles/{PROFILE} (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0 per "
+                                  "launch, same workload)" if d["traffic"] else None,
+                "pass": f"{n_q} further queries of the same workload, events around every step kernel"}
 
     # ---------------- serving: open-loop Poisson arrivals into the server
     serving = None
     if args.serve > 0 and args.D < 0:
-        import threading
-        from gpugst_amd import Server
         rate = args.serve_rate if args.serve_rate > 0 else 0.5 * batch_qps
-        n = args.serve
-        sq = data.make_queries(n, args.k, inst.meta["eligible"], 3000 + 4 + 10 * args.k + 7)
-        gaps = np.random.default_rng(11).exponential(1.0 / rate, n)
-        srv = Server(g)
-        tickets = [None] * n
-
-        def submitter():
-            t_next = time.perf_counter()
-            for i in range(n):
-                t_next += gaps[i]
-                dt = t_next - time.perf_counter()
-                if dt > 0:
-                    time.sleep(dt)
-                tickets[i] = srv.submit(sq[i], D=-1)
-
-        ts = time.perf_counter()
-        th = threading.Thread(target=submitter)
-        th.start()
-        th.join()
-        sres = [srv.wait(t) for t in tickets]
-        span = time.perf_counter() - ts
-        srv.close()
-        slat = [r.latency_ms for r in sres]
-        serving = {"offered_qps": round(rate, 1), "queries": n, "achieved_qps": round(n / span, 1),
-                   "p50_ms": round(pct(slat, 50), 2), "p99_ms": round(pct(slat, 99), 2),
+        sq = data.make_queries(args.serve, args.k, inst.meta["eligible"], 3000 + 4 + 10 * args.k + 7)
+        g.set_option("cache_mb", args.cache_mb)
+        ach, p50, p99, sres = poisson_serve(g, list(sq), [-1] * args.serve, rate, 11)
+        g.set_option("cache_mb", 0)
+        serving = {"offered_qps": round(rate, 1), "queries": args.serve, "achieved_qps": round(ach, 1),
+                   "p50_ms": round(p50, 2), "p99_ms": round(p99, 2),
+                   "p50_vs_plain_p50": round(p50 / pct(plain_lat, 50), 2) if plain_lat else None,
+                   "cache_mb": args.cache_mb, "cache_hit_rate": hit_rate(sres, [len(q) for q in sq]),
                    "arrivals": "Poisson (seeded), open loop, gst_server_* continuous batching"}
-        log(f"serving: offered {rate:.0f} q/s, {n} queries in {span:.2f}s -> {n/span:.0f} q/s, "
-            f"p50 {serving['p50_ms']} ms p99 {serving['p99_ms']} ms")
+        log(f"serving: offered {rate:.0f} q/s -> {ach:.0f} q/s, p50 {p50:.2f} ms p99 {p99:.2f} ms, "
+            f"cache hit rate {serving['cache_hit_rate']}")
 
     # ---------------- parity: plain vs batched on shared queries, trees valid
     all_b = bres + timed
@@ -377,38 +519,23 @@ def main():
             bad_trees += 0 if v["ok"] else 1
     infeas = sum(1 for r in timed if not r.feasible)
 
-    # ---------------- CPU oracle baseline on a bounded sample (also parity)
-    cpu = None
-    oracle_mism = 0
-    oracle_checked = 0
-    if not args.no_cpu:
-        import oracle
-        oracle.lib()
-        tc = time.perf_counter()
-        n = 0
-        while n < len(all_b) and (time.perf_counter() - tc) < args.cpu_seconds:
-            groups = [inst.group(int(x)) for x in queries[n]]
-            if args.D < 0:
-                c = oracle.dpbf_pruned(inst.rowptr, inst.col, inst.wt, groups)[0]
-            else:
-                c = oracle.dpbf(inst.rowptr, inst.col, inst.wt, groups, D=args.D)[0]
-            oracle_mism += int(c != all_b[n].cost)
-            n += 1
-        cs = time.perf_counter() - tc
-        oracle_checked = n
-        algo = ("exact pruned best-first DPBF (orc_dpbf_pruned, the exact-mode PrunedDP++ "
-                "restatement)" if args.D < 0 else "exact hop-bounded DP (orc_dpbf, HOP_cpu restatement)")
-        cpu = {"value": n / cs, "unit": "queries/s", "cores": 1, "kind": "port",
-               "sample": f"first {n} of the same {kdesc} queries, {algo} in oracle/gst_oracle.c, "
-                         f"single thread, {cs:.1f} s"}
-        log(f"cpu oracle: {n} queries in {cs:.1f}s, mismatches {oracle_mism}")
+    # ---------------- the other configurations of the workload, on one GPU
+    configs = {}
+    if not args.no_extra and args.scale == 1.0 and not args.kmix and args.D < 0:
+        configs["d4"] = hop_leg(g, inst, log)
+        configs["kmix_4096"] = kmix_leg(g, inst, log)
 
-    # ---------------- serving on the 3M-vertex serving configuration (optional;
-    # the first graph's engines give their device memory back first)
+    # ---------------- CPU baseline on the host cores (also parity)
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_leg(inst, queries, all_b, args.D, args.cpu_seconds, log)
+
+    # ---------------- serving on the 3M-vertex serving configuration (the
+    # first graph's engines give their device memory back first)
     serving_c5 = None
-    if args.serve_c5:
+    if not args.no_extra and args.scale == 1.0:
         g.close()
-        serving_c5 = serve_config5(log)
+        serving_c5 = serve_config5(log, cache_mb=args.cache_mb)
 
     lat = [r.latency_ms for r in timed]
     dev = [r.device_ms for r in timed]
@@ -436,15 +563,18 @@ def main():
                        "batched_device_p50": pct(dev, 50), "batched_device_p99": pct(dev, 99),
                        "batched_submit_p50": pct(lat, 50), "batched_submit_p99": pct(lat, 99)},
         "parity": {"plain_vs_batched_mismatches": mism, "invalid_trees": bad_trees,
-                   "oracle_checked": oracle_checked, "oracle_mismatches": oracle_mism,
+                   "oracle_checked": cpu["oracle_checked"] if cpu else 0,
+                   "oracle_mismatches": cpu["oracle_mismatches"] if cpu else 0,
                    "infeasible": infeas},
         "serving": serving,
+        "configs": configs,
         "serving_c5": serving_c5,
         "roofline": roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
-    g.close()
+    if serving_c5 is None:
+        g.close()
 
 
 if __name__ == "__main__":

@@ -33,8 +33,9 @@ GRID = {
 KEYS = ("chunk", "blocks_per_cu", "graph", "max_slots")
 
 
-def bucket(V: int, arcs: int, k: int, D: int) -> str:
-    """Graphs are bucketed by powers of two of V and of the arc count."""
+def bucket(V: int, arcs: int, k, D: int) -> str:
+    """Graphs are bucketed by powers of two of V and of the arc count; k is a
+    query size or "mix" for a mixed-k workload."""
     return f"V2^{max(V, 1).bit_length()}_A2^{max(arcs, 1).bit_length()}_k{k}_D{D}"
 
 
@@ -45,8 +46,9 @@ def load_db(path: str = DEFAULT_DB) -> Dict[str, dict]:
     return {}
 
 
-def apply(graph, k: int, D: int = -1, path: str = DEFAULT_DB) -> Optional[dict]:
-    """Set the tuned options for this graph and query shape, if known."""
+def apply(graph, k, D: int = -1, path: str = DEFAULT_DB) -> Optional[dict]:
+    """Set the tuned options for this graph and query shape (k, or "mix"), if
+    known."""
     db = load_db(path)
     b = db.get(bucket(graph.V, graph.n_arcs, k, D))
     if not b:
@@ -58,7 +60,7 @@ def apply(graph, k: int, D: int = -1, path: str = DEFAULT_DB) -> Optional[dict]:
 
 
 def tune(graph, queries: Sequence, D: int = -1, grid: Optional[dict] = None,
-         path: Optional[str] = DEFAULT_DB, repeats: int = 1, log=print) -> dict:
+         path: Optional[str] = DEFAULT_DB, repeats: int = 1, log=print, kkey=None) -> dict:
     """Grid search; returns {'best': {...}, 'qps': float, 'trials': [...]}."""
     grid = grid or GRID
     keys = list(grid)
@@ -88,7 +90,7 @@ def tune(graph, queries: Sequence, D: int = -1, grid: Optional[dict] = None,
     for key in keys:
         graph.set_option(key, out["best"][key])
     if path:
-        k = len(queries[0])
+        k = kkey if kkey is not None else len(queries[0])
         db = load_db(path)
         db[bucket(graph.V, graph.n_arcs, k, D)] = out
         with open(path, "w") as f:

@@ -78,7 +78,6 @@ enum Phase : int32_t {
   PH_WAITJ = 10, // query waiting for its group-distance job (no work items)
   PH_STAR = 11,    // job: star bound of every member query
   PH_COPY = 12,    // hop job: height level starts as a copy of height level - 1
-  PH_COUNT = 13,   // optional: count the finite states (statistics, after FINAL)
   PH_CACHEIN = 14, // job: copy cached group-distance columns into its rows
   PH_CACHEOUT = 15 // job: copy its new labels' distances out into cache columns
 };
@@ -133,12 +132,6 @@ struct alignas(64) Slot {
   uint32_t* jmem;       // job: member slot indices
   uint32_t* vsrc;       // vertices LIVE examines (null: all V); job: the vertices it reached
   uint32_t nvsrc;       // entries of vsrc
-  uint32_t ncached;     // job: labels whose distances come from cache columns (not seeded)
-  // job: 2 * L column pointers; entry l (l < L) is the cache column of label l
-  // to copy in (null: computed by the job), entry L + l the column its
-  // distances are copied out to (null: not cached)
-  uint32_t* const* ccol;
-  uint32_t nout;        // job: labels copied out to the cache
   uint32_t pad3;
   uint32_t* live;       // V: vertices that can lie on a tree cheaper than the bound
   uint32_t* lvb;        // V bits: live flag per vertex
@@ -170,9 +163,16 @@ struct alignas(64) Slot {
   uint32_t vv[kMaxK * kMaxK];
   int32_t out_cost, out_root, out_nedges, out_d;
   unsigned long long ctr[6];
+  // ---- fields read off the hot paths (kept after the others) ----
   uint32_t maxq;        // largest frontier (vertices) of one of its rounds
-  uint32_t count;       // 1: run PH_COUNT before RECON
-  unsigned long long nfinite;  // PH_COUNT: finite states (singletons over all heights + T)
+  uint32_t count;       // 1: count its finite states when it has finished
+  unsigned long long nfinite;  // finite states (singletons over all heights + T)
+  // job: 2 * L column pointers; entry l (l < L) is the cache column of label l
+  // to copy in (null: computed by the job), entry L + l the column its
+  // distances are copied out to (null: not cached)
+  uint32_t* const* ccol;
+  uint32_t ncached;     // job: labels whose distances come from cache columns (not seeded)
+  uint32_t nout;        // job: labels copied out to the cache
 };
 
 struct alignas(64) Ctrl {

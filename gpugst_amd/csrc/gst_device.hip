@@ -1471,24 +1471,31 @@ __device__ void item_recon(const GraphDev& g, Slot* sp) {
 // code/GPUGST/src/DPQ.cu:225-240, D-GPUGST :128-148): per vertex of a tile,
 // the finite singleton states over every height (from S1 or the job's rows)
 // and, at live vertices (the only ones whose T segment is initialised), the
-// finite states of two or more groups.
+// finite states of two or more groups.  Run by its own kernel on a finished
+// query before its memory is released, so the step kernels carry none of it.
 __device__ void item_count(Slot* sp, uint32_t tile, uint64_t V) {
   const int lane = lane_id();
-  const States S = states_of(sp);
-  const int k = sp->k, W = sp->W, ND = sp->ND, NS = sp->NS;
+  const uint32_t k = (uint32_t)sp->k, W = (uint32_t)sp->W, ND = (uint32_t)sp->ND, NS = (uint32_t)sp->NS;
   const uint64_t v0 = (uint64_t)tile * kDistTV;
   const uint64_t v1 = v0 + kDistTV < V ? v0 + kDistTV : V;
-  uint64_t n = 0;
+  // singletons: S1 words (cost in the high half) or the job's distance rows
+  const uint32_t* S1 = (const uint32_t*)sp->S1;
+  const uint32_t* J = sp->J;
+  const uint32_t* T = (const uint32_t*)sp->T;
+  uint32_t n = 0;
   for (uint64_t v = v0 + (uint64_t)lane; v < v1; v += 64) {
-    for (int d = 0; d < ND; d++)
-      for (int i = 0; i < k; i++) n += (S.get((uint32_t)v, 1u << i, (uint32_t)d) >> 32) < kInf ? 1u : 0u;
+    for (uint32_t d = 0; d < ND; d++)
+      for (uint32_t i = 0; i < k; i++) {
+        const uint32_t c = S1 ? S1[2 * ((v * ND + d) * k + i) + 1] : J[v * sp->jvs + (uint64_t)d * sp->jds + i];
+        n += c < kInf ? 1u : 0u;
+      }
     if (!((sp->lvb[v >> 5] >> (v & 31u)) & 1u)) continue;
-    for (int d = 0; d < NS; d++)
-      for (int p = 3; p < W; p++)
-        if (p & (p - 1)) n += (S.get((uint32_t)v, (uint32_t)p, (uint32_t)d) >> 32) < kInf ? 1u : 0u;
+    for (uint32_t d = 0; d < NS; d++)
+      for (uint32_t p = 3; p < W; p++)
+        if (p & (p - 1u)) n += T[2 * ((v * NS + d) * W + p) + 1] < kInf ? 1u : 0u;
   }
-  n = wave_sum_u64(n);
-  if (lane == 0 && n) atomicAdd(&sp->nfinite, (unsigned long long)n);
+  const uint64_t t = wave_sum_u64(n);
+  if (lane == 0 && t) atomicAdd(&sp->nfinite, (unsigned long long)t);
 }
 
 // -------------------------------------------------------- state machine ----
@@ -1591,9 +1598,6 @@ __device__ void complete_step(Slot* slots, Slot* s, Ctrl* ctrl) {
         else s->phase = PH_ROUND;
         break;
       case PH_FINAL:
-        s->phase = s->count ? PH_COUNT : PH_RECON;
-        break;
-      case PH_COUNT:
         s->phase = PH_RECON;
         break;
       case PH_RECON:
@@ -1629,15 +1633,16 @@ __device__ uint32_t plan_step(Slot* s, uint64_t V) {
       if (!job) cls = kClsHeavy;  // the per-query grow carries the lower-bound code
       break;
     case PH_STAR:
-    case PH_COPY:
+    case PH_COPY: work = ceil_div(V, kStarTV); break;
+    // cache copies run in the heavy kernel: they would push the lean one
+    // (job rounds) over its register budget
     case PH_CACHEIN:
-    case PH_CACHEOUT: work = ceil_div(V, kStarTV); break;
+    case PH_CACHEOUT: work = ceil_div(V, kStarTV); cls = kClsHeavy; break;
     case PH_DIST: work = (s->S1 ? ceil_div(V, kDistTV) : 0u) + 1u; cls = kClsHeavy; break;
     case PH_LIVE: work = ceil_div(s->vsrc ? (uint64_t)s->nvsrc : V, kDistTV); cls = kClsHeavy; break;
     case PH_MERGE: work = ceil_div(s->level >= 3 ? s->nact : s->nlive, kMergeTV); cls = kClsHeavy; break;
     case PH_FINAL: work = ceil_div(s->k >= 3 ? s->nact : s->nlive, kMergeTV); cls = kClsHeavy; break;
     case PH_RECON: work = 1; cls = kClsHeavy; break;
-    case PH_COUNT: work = ceil_div(V, kDistTV); cls = kClsHeavy; break;
     default: return (s->n_work = s->n_clr = 0);
   }
   // every active step runs at least one (possibly empty) item so that it completes
@@ -1812,8 +1817,6 @@ __device__ void step_body(const GraphDev& g, Slot* slots, Ctrl* ctrl, LDS* L) {
             case PH_ROUND: item_jround(g, sp, local, (FrontLDS*)L); break;
             case PH_STAR: item_jstar(slots, sp, local, V); break;
             case PH_COPY: item_jcopy(sp, local, V); break;
-            case PH_CACHEIN: item_jcache(sp, local, V, false); break;
-            case PH_CACHEOUT: item_jcache(sp, local, V, true); break;
             default: break;
           }
         } else {
@@ -1835,7 +1838,8 @@ __device__ void step_body(const GraphDev& g, Slot* slots, Ctrl* ctrl, LDS* L) {
           case PH_MERGE: item_merge(g, sp, local, (WaveLDS*)L); break;
           case PH_FINAL: item_final(g, sp, local, (WaveLDS*)L); break;
           case PH_RECON: item_recon(g, sp); break;
-          case PH_COUNT: item_count(sp, local, V); break;
+          case PH_CACHEIN: item_jcache(sp, local, V, false); break;
+          case PH_CACHEOUT: item_jcache(sp, local, V, true); break;
           default: break;
         }
       }
@@ -1867,6 +1871,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
 // Between two steps, one block advances every slot's state machine and
 // publishes the next step's work items.  Once it has set exit_flag it leaves
 // everything untouched, so the host may enqueue more steps than are needed.
+// Finite-state count of one finished query: one wave per tile of kDistTV
+// vertices.
+__global__ __launch_bounds__(64) void gst_count_kernel(Slot* slot, uint64_t V) {
+  item_count(slot, blockIdx.x, V);
+}
+
 __global__ __launch_bounds__(kBlock) void gst_plan_kernel(Slot* slots, Ctrl* ctrl, uint64_t V) {
   __shared__ uint32_t sh_items[kNumCls][kMaxSlots];
   __shared__ uint32_t sh_scan[kNumCls][2][kBlock];
@@ -1889,6 +1899,12 @@ extern "C" hipError_t gst_launch_step(int cls, const gst::GraphDev* g, gst::Slot
 
 extern "C" hipError_t gst_launch_plan(gst::Slot* slots, gst::Ctrl* ctrl, uint64_t V, hipStream_t stream) {
   hipLaunchKernelGGL(gst::gst_plan_kernel, dim3(1), dim3(gst::kBlock), 0, stream, slots, ctrl, V);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t gst_launch_count(gst::Slot* slot, uint64_t V, hipStream_t stream) {
+  const uint32_t tiles = (uint32_t)((V + gst::kDistTV - 1) / gst::kDistTV);
+  hipLaunchKernelGGL(gst::gst_count_kernel, dim3(tiles), dim3(64), 0, stream, slot, V);
   return hipGetLastError();
 }
 

@@ -29,6 +29,7 @@ extern "C" hipError_t gst_launch_step(int cls, const gst::GraphDev* g, gst::Slot
 extern "C" hipError_t gst_launch_plan(gst::Slot* slots, gst::Ctrl* ctrl, uint64_t V, hipStream_t stream);
 extern "C" hipError_t gst_step_occupancy(int cls, int* blocks_per_cu);
 extern "C" hipError_t gst_scratch_setup(int device, void** ptr);
+extern "C" hipError_t gst_launch_count(gst::Slot* slot, uint64_t V, hipStream_t stream);
 
 namespace {
 
@@ -650,6 +651,12 @@ class Engine {
           }
           cache_misses_++;
           if (it != cache_.end() || outs.count(gid)) continue;  // being written already
+          // a group earns a column on its second request: one-off groups
+          // (most of a uniform workload) cost no copy-out
+          if (seen_.insert(gid).second) {
+            if (seen_.size() > kSeenCap) seen_.clear();
+            continue;
+          }
           size_t coff = 0;
           if (!reserve_column(&coff)) continue;
           CacheEntry e;
@@ -713,6 +720,7 @@ class Engine {
   void sync_cache_version() {
     if (cache_version_ == g_->groups_version) return;
     cache_version_ = g_->groups_version;
+    seen_.clear();
     for (auto it = cache_.begin(); it != cache_.end();) {
       if (it->second.refs == 0) {
         arena_.release(it->second.off, column_bytes());
@@ -747,6 +755,7 @@ class Engine {
   void clear_cache() {
     for (auto& kv : cache_) arena_.release(kv.second.off, column_bytes());
     cache_.clear();
+    seen_.clear();
     cache_bytes_ = 0;
     job_cache_refs_.clear();
   }
@@ -1025,7 +1034,17 @@ class Engine {
     out->counters.active_vertices = (int64_t)d.nact;
     out->counters.max_frontier = (int64_t)std::max(d.maxq, h.jmaxq);
     out->counters.cached_labels = h.cached;
-    out->counters.states_finite = d.count ? (int64_t)d.nfinite : -1;
+    out->counters.states_finite = -1;
+    if (d.count && d.status == 0 && d.phase == gst::PH_DONE) {
+      // the reference counts finite states after its main loop (count_set);
+      // here a pass of its own over the finished query's tables
+      unsigned long long nf = 0;
+      gst::Slot* ds = d_slots_ + s;
+      hipError_t e = gst_launch_count(ds, (uint64_t)g_->V, stream_);  // nfinite starts at 0 (admission)
+      if (e == hipSuccess) e = hipMemcpyAsync(&nf, &ds->nfinite, sizeof(nf), hipMemcpyDeviceToHost, stream_);
+      if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+      if (e == hipSuccess) out->counters.states_finite = (int64_t)nf;
+    }
     gst_status st = GST_OK;
     if (d.status == gst::kStatusCancelled) {
       out->status = GST_CANCELLED;
@@ -1142,6 +1161,8 @@ class Engine {
   // group-distance cache: group id -> column in the arena
   std::unordered_map<int32_t, CacheEntry> cache_;
   std::map<int, std::vector<std::pair<int32_t, bool>>> job_cache_refs_;  // job slot -> (group, writes it)
+  std::unordered_set<int32_t> seen_;  // groups requested once since they last left the cache
+  static constexpr size_t kSeenCap = size_t(1) << 22;
   size_t cache_bytes_ = 0;
   uint64_t cache_clock_ = 0;
   int64_t cache_version_ = -1;

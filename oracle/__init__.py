@@ -111,7 +111,8 @@ def dpbf(rowptr, col, wt, groups: Sequence[Sequence[int]], D: int = -1,
 
 
 def dpbf_pruned(rowptr, col, wt, groups: Sequence[Sequence[int]]):
-    """Exact optimum (unbounded) by the pruned best-first search; cost only."""
+    """Exact optimum (unbounded) by the pruned best-first search; cost only.
+    stats: pops, heap capacity, precompute ns, main-loop ns."""
     rowptr, col, wt = _csr(rowptr, col, wt)
     V = rowptr.shape[0] - 1
     k = len(groups)
@@ -121,7 +122,7 @@ def dpbf_pruned(rowptr, col, wt, groups: Sequence[Sequence[int]]):
     gmem = np.ascontiguousarray(np.concatenate([np.asarray(g, dtype=np.int32) for g in groups]),
                                 dtype=np.int32)
     cost = ctypes.c_int32()
-    stats = np.zeros(2, dtype=np.int64)
+    stats = np.zeros(4, dtype=np.int64)
     rc = lib().orc_dpbf_pruned(V, _ptr(rowptr), _ptr(col), _ptr(wt), k, _ptr(gptr), _ptr(gmem),
                                ctypes.byref(cost), _ptr(stats))
     if rc < 0:

@@ -39,6 +39,13 @@
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
+
+static int64_t orc_now_ns(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return (int64_t)ts.tv_sec * 1000000000 + ts.tv_nsec;
+}
 
 #define GST_ORC_INF 0x3FFFFFFF
 
@@ -378,7 +385,10 @@ int orc_steiner_dw(int64_t V, const int64_t *rowptr, const int32_t *col,
  * merges.  Deviations that keep it exact: no approximation-ratio exit
  * (:730-755), no 1/2 and 2/3 decomposition rules, and a candidate is dropped
  * only when cost + lb > best (strict), which never removes a state of an
- * optimal witness.  Returns the optimal cost only. */
+ * optimal witness.  Returns the optimal cost only.  stats (4 entries): pops,
+ * heap capacity, ns of the precompute (distances, route tables, bounds) and
+ * ns of the main loop -- the reference times its main loop alone
+ * (CPUNONHOP.h:634-665). */
 static uint32_t orc_lb(const int32_t *c, int k, uint32_t x, const int32_t *R) {
   int64_t lb1 = 0;
   for (int i = 0; i < k; i++)
@@ -405,9 +415,11 @@ int orc_dpbf_pruned(int64_t V, const int64_t *rowptr, const int32_t *col, const 
                     int32_t k, const int64_t *gptr, const int32_t *gmem, int32_t *cost_out,
                     int64_t *stats) {
   if (k < 1 || k > 16 || V <= 0) return -2;
+  const int64_t t_begin = orc_now_ns();
   const int64_t W = (int64_t)1 << k;
   const int32_t full = (int32_t)(W - 1);
   *cost_out = GST_ORC_INF;
+  if (stats) stats[0] = stats[1] = stats[2] = stats[3] = 0;
   for (int i = 0; i < k; i++)
     if (gptr[i + 1] == gptr[i]) return 0; /* empty group: infeasible */
   if (k == 1) {
@@ -479,6 +491,7 @@ int orc_dpbf_pruned(int64_t V, const int64_t *rowptr, const int32_t *col, const 
     }
   }
   int32_t answer = GST_ORC_INF;
+  const int64_t t_loop = orc_now_ns();
   while (h.n) {
     uint64_t key = heap_pop(&h);
     int64_t s = (int64_t)(key & STATE_MASK);
@@ -525,8 +538,11 @@ int orc_dpbf_pruned(int64_t V, const int64_t *rowptr, const int32_t *col, const 
   }
   *cost_out = answer;
   if (stats) {
+    const int64_t t_end = orc_now_ns();
     stats[0] = pops;
     stats[1] = (int64_t)h.cap;
+    stats[2] = t_loop - t_begin;
+    stats[3] = t_end - t_loop;
   }
   free(h.a);
   free(C); free(dist); free(R); free(cost); free(done); free(gmask);

@@ -84,6 +84,22 @@ extern "C" hipError_t gst_launch_step(int cls, const gst::GraphDev* g, gst::Slot
   return hipSuccess;
 }
 
+extern "C" hipError_t gst_launch_count(gst::Slot* slot, uint64_t V, hipStream_t) {
+  const uint32_t tiles = (uint32_t)((V + gst::kDistTV - 1) / gst::kDistTV);
+  for (uint32_t t = 0; t < tiles; t++) {
+    gsim::Wave w;
+    std::vector<std::thread> th;
+    for (int l = 0; l < 64; l++)
+      th.emplace_back([&, l]() {
+        gsim::lane = l;
+        gsim::wave = &w;
+        gst::item_count(slot, t, V);
+      });
+    for (auto& x : th) x.join();
+  }
+  return hipSuccess;
+}
+
 extern "C" hipError_t gst_step_occupancy(int, int* blocks_per_cu) {
   *blocks_per_cu = 1;
   return hipSuccess;

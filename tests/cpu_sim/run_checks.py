@@ -312,10 +312,10 @@ def check_robust():
 
 
 def check_cache():
-    """Group-distance cache across batches (unbounded jobs): a second batch
-    over the same groups reads columns written by the first (hits, labels
-    counted per query); single-query jobs reuse the columns of earlier ones;
-    new group lists retire the cache; every cost equals the oracle."""
+    """Group-distance cache across batches (unbounded jobs): a group gets a
+    column at its second request and is read from it afterwards (hits,
+    labels counted per query); single-query jobs do the same; new group lists
+    retire the cache; every cost equals the oracle."""
     rng, rp, col, wt, groups = random_instance(150, 29)
     G = len(groups)
     b1 = [rng.sample(range(G), 4) for _ in range(12)]
@@ -325,20 +325,23 @@ def check_cache():
     g = Graph(rp, col, wt)
     g.set_groups(*groups_csr(groups))
     g.set_option("cache_mb", 1)
-    r1 = g.solve_batch(b1)
+    r1 = g.solve_batch(b1)   # first requests: no columns yet
     h1 = g.get_option("batch_cache_hits")
-    r2 = g.solve_batch(b2)
-    h2 = g.get_option("batch_cache_hits") - h1
-    out = {"hits_first": h1, "hits_second": h2, "inserts": g.get_option("batch_cache_inserts"),
-           "per_query": sum(r.counters["cached_labels"] for r in r2)}
-    out["exact"] = [r.cost for r in r1] == exp1 and [r.cost for r in r2] == exp2
+    i1 = g.get_option("batch_cache_inserts")
+    r2 = g.solve_batch(b2)   # second requests of shared groups: columns written
+    r3 = g.solve_batch(b1)   # third: read back
+    h3 = g.get_option("batch_cache_hits") - h1
+    out = {"hits_first": h1, "inserts_first": i1, "hits_later": h3, "inserts": g.get_option("batch_cache_inserts"),
+           "per_query": sum(r.counters["cached_labels"] for r in r2 + r3)}
+    out["exact"] = ([r.cost for r in r1] == exp1 and [r.cost for r in r2] == exp2
+                    and [r.cost for r in r3] == exp1)
     # one query per batch: a job of one query fills the cache for the next
     g2 = Graph(rp, col, wt)
     g2.set_groups(*groups_csr(groups))
     g2.set_option("cache_mb", 1)
     g2.set_option("job_min_queries", 1)
-    r3 = [g2.solve_batch([q])[0].cost for q in b1[:4] + b1[:2]]
-    out["single_exact"] = r3 == exp1[:4] + exp1[:2] and g2.get_option("batch_cache_hits") >= 8
+    r4 = [g2.solve_batch([q])[0].cost for q in b1[:2] * 3]
+    out["single_exact"] = r4 == exp1[:2] * 3 and g2.get_option("batch_cache_hits") >= 8
     # new group lists: the cache must not serve the old distances
     g.set_option("cache_mb", 1)
     g.solve_batch(b1)

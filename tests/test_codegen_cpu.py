@@ -91,3 +91,23 @@ def test_claim_and_broadcast_share_one_loop(device_asm, cls):
     _, hb, db = blk[br]
     assert (hc, dc) == (hb, db), (blk[c], blk[br])
     assert dc == 1, blk[c]
+
+
+def _kernel_meta(text, sym):
+    """The amdhsa metadata block of one kernel (.vgpr_count, spill counts)."""
+    i = text.index(f".name:           {sym}")
+    j = text.find("  - .", i + 1)
+    block = text[text.rfind("  - .", 0, i):j if j > 0 else len(text)]
+    return {k: int(v) for k, v in re.findall(r"\.(vgpr_count|vgpr_spill_count|sgpr_spill_count):\s+(\d+)", block)}
+
+
+@pytest.mark.parametrize("cls,max_vgpr,max_vspill", [("lean", 96, 2), ("heavy", 168, 0)])
+def test_step_kernel_register_budget(device_asm, cls, max_vgpr, max_vspill):
+    """Register budgets of the step kernels: the lean kernel (job rounds) at
+    five waves per SIMD with no more than its two long-standing VGPR spills,
+    the heavy kernel at three waves per SIMD with none.  Code added to rare
+    items has pushed both over before (a statistics pass is a kernel of its
+    own for that reason)."""
+    m = _kernel_meta(device_asm, KERNELS[cls][0])
+    assert m["vgpr_count"] <= max_vgpr, m
+    assert m["vgpr_spill_count"] <= max_vspill, m

@@ -3,8 +3,8 @@ across batches and server epochs (option cache_mb).
 
 The reference recomputes every query's group distances
 (code/GPUGST/src/DPQ.cu:378-404, code/GPUGST+/src/GSTnonHop.cu:273); here a
-later batch, or a later server request, whose groups were seen before copies
-their columns instead.  A 1 MiB budget holds fewer columns than the batches
+group gets a column at its second request, and later batches or server
+requests copy that column instead of recomputing it.  A 1 MiB budget holds fewer columns than the batches
 use, so columns are evicted and re-inserted along the way.  Every cost must
 equal the oracle's; the cache must be hit, and the per-query count of cached
 labels must add up to the engine's hit counter.
@@ -44,18 +44,19 @@ def test_cache_across_batches_and_server():
             assert r.cost == exp[tuple(q)], (q, r.cost, exp[tuple(q)])
             groups = [inst.group(x) for x in q]
             assert validate_tree(inst.rowptr, inst.col, inst.wt, groups, r.cost, r.root, r.edges)["ok"]
-        if name != "A":
+        if name == "A again":
             assert hits > 0
     assert g.get_option("batch_cache_evictions") > 0
     # the server engine keeps its own cache across epochs (every admission
     # through a job, so that lone requests fill it too)
     g.set_option("job_min_queries", 1)
     srv = Server(g)
-    first = [srv.wait(t) for t in [srv.submit(q) for q in B[:8]]]
-    again = [srv.wait(t) for t in [srv.submit(q) for q in B[:8]]]
+    rounds = [[srv.wait(t) for t in [srv.submit(q) for q in B[:8]]] for _ in range(3)]
     srv.close()
-    for q, r1, r2 in zip(B[:8], first, again):
-        assert r1.cost == r2.cost == exp[tuple(q)]
+    again = rounds[2]
+    for rs in rounds:
+        for q, r in zip(B[:8], rs):
+            assert r.cost == exp[tuple(q)]
     print("cache: server cached labels", sum(r.counters["cached_labels"] for r in again), flush=True)
     assert sum(r.counters["cached_labels"] for r in again) > 0
     g.close()
