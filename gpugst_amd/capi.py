@@ -55,7 +55,8 @@ class c_counters(ctypes.Structure):
                 ("live_vertices", ctypes.c_int64), ("shared_entries", ctypes.c_int64),
                 ("shared_arcs", ctypes.c_int64), ("shared_relaxations", ctypes.c_int64),
                 ("active_vertices", ctypes.c_int64), ("max_frontier", ctypes.c_int64),
-                ("states_finite", ctypes.c_int64), ("cached_labels", ctypes.c_int64)]
+                ("states_finite", ctypes.c_int64), ("cached_labels", ctypes.c_int64),
+                ("lower_bound", ctypes.c_int64)]
 
 
 class c_result(ctypes.Structure):

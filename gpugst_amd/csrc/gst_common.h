@@ -173,6 +173,14 @@ struct alignas(64) Slot {
   uint32_t* const* ccol;
   uint32_t ncached;     // job: labels whose distances come from cache columns (not seeded)
   uint32_t nout;        // job: labels copied out to the cache
+  // progressive answers: stop once a walk-tree within anum / aden of a
+  // certified lower bound is known (anum == 0: exact)
+  uint32_t anum, aden;
+  uint32_t lbmin;       // LIVE: min over candidate vertices of the full-set lower bound
+  uint32_t cert;        // certified lower bound on the optimum of the answer given
+  unsigned long long star_pk;  // LIVE: cheapest star, (cost << 32) | centre vertex
+  uint32_t probe;       // the FINAL under way is a checkpoint between levels
+  uint32_t recon_star;  // RECON rebuilds the star of star_pk (shortest paths from its centre)
 };
 
 struct alignas(64) Ctrl {

@@ -999,6 +999,33 @@ __device__ __forceinline__ void stage_rows(const Slot* sp, const uint32_t* ids, 
   }
 }
 
+// Progressive answers need, per candidate vertex, the full-set bound itself
+// (its minimum over the candidates bounds the optimum, whose tree has its
+// vertices among them) and the cheapest star (arms within floor(D/2) hops).
+// Wave-collective (lanes without a vertex pass has = false); returns whether
+// the lane's vertex is live.  The minima go straight to the slot, so LIVE's
+// loop carries nothing extra.
+__device__ bool prog_bounds(Slot* sp, bool has, uint64_t v, uint32_t lim) {
+  uint32_t lb = kInf;
+  uint64_t pk = ~0ull;
+  if (has) {
+    const int k = sp->k;
+    g_cu32* cr = crow(sp, v, sp->D);
+    lb = lower_bound(cr, k, (uint32_t)sp->W - 1, (g_cu32*)sp->R);
+    g_cu32* cs = sp->D >= 0 ? crow(sp, v, sp->D / 2) : cr;
+    uint32_t st = 0;
+    for (int i = 0; i < k; i++) st = sat(st + cs[i]);
+    if (st < kInf) pk = ((uint64_t)st << 32) | v;
+  }
+  const uint32_t lbw = wave_min_u32(lb);
+  const uint64_t pkw = wave_min_u64(pk);
+  if (lane_id() == 0) {
+    if (lbw < kInf) atomicMin(&sp->lbmin, lbw);
+    if (pkw != ~0ull) atomicMin((unsigned long long*)&sp->star_pk, (unsigned long long)pkw);
+  }
+  return has && lb <= lim;
+}
+
 // Live list after level 1: a vertex whose lower bound for covering every
 // group (the tour bound over the full set) exceeds the current upper bound
 // cannot lie on a tree cheaper than that bound, so it never holds a useful
@@ -1012,16 +1039,19 @@ __device__ void item_live(Slot* sp, uint32_t tile, uint64_t V, WaveLDS* L) {
   const uint64_t n = sp->vsrc ? (uint64_t)sp->nvsrc : V;  // candidate vertices
   const uint64_t v0 = (uint64_t)tile * kDistTV;
   const uint64_t v1 = v0 + kDistTV < n ? v0 + kDistTV : n;
+  const bool prog = sp->anum != 0;
   for (uint64_t vb = v0; vb < v1; vb += 64) {
     const uint64_t idx = vb + lane;
     uint64_t v = 0;
     bool lv = false;
-    if (idx < v1) {
-      v = sp->vsrc ? (uint64_t)sp->vsrc[idx] : idx;
+    if (idx < v1) v = sp->vsrc ? (uint64_t)sp->vsrc[idx] : idx;
+    if (prog) {
+      lv = prog_bounds(sp, idx < v1, v, bst < kInf ? bst : kInf - 1);
+    } else if (idx < v1) {
       // lower bound on any tree through v that covers every group
       lv = lb_le(crow(sp, v, sp->D), k, (uint32_t)W - 1, (g_cu32*)sp->R, bst < kInf ? bst : kInf - 1);
-      if (lv) atomicOr(&sp->lvb[v >> 5], 1u << (uint32_t)(v & 31u));
     }
+    if (lv) atomicOr(&sp->lvb[v >> 5], 1u << (uint32_t)(v & 31u));
     const uint64_t ball = __ballot(lv);
     if (ball == 0) continue;
     const int nl = __popcll(ball);
@@ -1319,7 +1349,10 @@ __device__ void item_recon(const GraphDev& g, Slot* sp) {
   const States S = states_of(sp);
   const bool hop = D >= 0;
   const uint32_t full = (uint32_t)W - 1;
-  const uint64_t ans = sp->ans;
+  // the answer: the DP's best full-set state, or (progressive answers) the
+  // cheapest star, rebuilt as one shortest path per group from its centre
+  const bool star = sp->recon_star != 0;
+  const uint64_t ans = star ? (uint64_t)sp->star_pk : sp->ans;
   const uint32_t cost = (uint32_t)(ans >> 32);
   if (cost >= kInf) {
     if (lane == 0) {
@@ -1336,18 +1369,33 @@ __device__ void item_recon(const GraphDev& g, Slot* sp) {
   if (lane < NS) cd = (uint32_t)(S.get(root, full, (uint32_t)lane) >> 32);
   const uint64_t hit = __ballot(lane < NS && cd == cost);
   const int dstar = hit ? (__ffsll((long long)hit) - 1) : 0;
+
   const uint32_t qcap = (uint32_t)sp->edge_cap + 2 * kMaxK;
   uint32_t* qv = sp->rq;
   uint32_t* qpd = sp->rq + qcap;
   int32_t* qat = (int32_t*)(sp->rq + 2 * qcap);
-  if (lane == 0) {
+  if (!star && lane == 0) {
     qv[0] = root;
     qpd[0] = full | ((uint32_t)dstar << 16);
     qat[0] = -1;
   }
+  if (star && lane < sp->k) {
+    // one arm per group: its singleton at the lowest height h <= floor(D/2)
+    // that holds the distance within floor(D/2) hops
+    uint32_t sd = 0;
+    if (hop) {
+      const uint32_t want = crow(sp, root, D / 2)[lane];
+      sd = (uint32_t)(D / 2);
+      for (int h = D / 2; h >= 0; h--)
+        if ((uint32_t)(S.get(root, 1u << lane, (uint32_t)h) >> 32) == want) sd = (uint32_t)h;
+    }
+    qv[lane] = root;
+    qpd[lane] = (1u << lane) | (sd << 16);
+    qat[lane] = -1;
+  }
   __threadfence();
   wave_sync();
-  uint32_t head = 0, tail = 1, ne = 0;
+  uint32_t head = 0, tail = star ? (uint32_t)sp->k : 1u, ne = 0;
   bool overflow = false, broken = false;
   while (head < tail) {
     const uint32_t cnt = (tail - head) < 64u ? (tail - head) : 64u;
@@ -1502,12 +1550,37 @@ __device__ void item_count(Slot* sp, uint32_t tile, uint64_t V) {
 
 __device__ uint32_t ceil_div(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
+// Progressive answers: the candidate cost is within anum / aden of the
+// certified lower bound (the candidates' full-set bound, or the distance
+// between two groups, both at most the optimum), in exact integers.
+__device__ uint32_t certified_lb(const Slot* s) {
+  uint32_t lb = s->lbmin;
+  for (int i = 0; i < s->k; i++)
+    for (int j = 0; j < s->k; j++)
+      if (i != j && s->vv[i * s->k + j] < kInf && s->vv[i * s->k + j] > lb) lb = s->vv[i * s->k + j];
+  return lb;
+}
+
+__device__ bool approx_met(Slot* s, uint32_t cand) {
+  if (!s->anum || cand >= kInf) return false;
+  const uint32_t lb = certified_lb(s);
+  if (lb >= kInf || (uint64_t)cand * s->aden > (uint64_t)s->anum * lb) return false;
+  s->cert = lb;
+  return true;
+}
+
 __device__ void advance_level(Slot* s) {
   if (s->level == 1) {
     s->phase = PH_DIST;
   } else if (s->level + 1 < s->k) {
-    s->phase = PH_MERGE;
-    s->level++;
+    if (s->anum) {
+      // progressive answers: a FINAL over the levels done so far first
+      s->phase = PH_FINAL;
+      s->probe = 1;
+    } else {
+      s->phase = PH_MERGE;
+      s->level++;
+    }
   } else {
     s->phase = PH_FINAL;
   }
@@ -1586,7 +1659,10 @@ __device__ void complete_step(Slot* slots, Slot* s, Ctrl* ctrl) {
         s->phase = PH_LIVE;
         break;
       case PH_LIVE:
-        if (s->k == 2) {
+        if (approx_met(s, (uint32_t)(s->star_pk >> 32))) {
+          s->recon_star = 1;  // the star is close enough to the bound
+          s->phase = PH_RECON;
+        } else if (s->k == 2) {
           s->phase = PH_FINAL;
         } else {
           s->phase = PH_MERGE;
@@ -1598,7 +1674,20 @@ __device__ void complete_step(Slot* slots, Slot* s, Ctrl* ctrl) {
         else s->phase = PH_ROUND;
         break;
       case PH_FINAL:
-        s->phase = PH_RECON;
+        if (s->probe) {
+          s->probe = 0;
+          const uint32_t a = (uint32_t)(s->ans >> 32), st = (uint32_t)(s->star_pk >> 32);
+          if (approx_met(s, a < st ? a : st)) {
+            s->recon_star = st < a;
+            s->phase = PH_RECON;
+          } else {
+            s->phase = PH_MERGE;
+            s->level++;
+          }
+        } else {
+          s->cert = (uint32_t)(s->ans >> 32);  // the DP is complete: the answer is the optimum
+          s->phase = PH_RECON;
+        }
         break;
       case PH_RECON:
         s->phase = PH_DONE;

@@ -128,6 +128,7 @@ struct Options {
   int64_t job_min_queries = 2;  // fewer unbounded queries admitted together run their own level 1
   int64_t deadline_ms = 0;  // a query still running this long after its submission is cancelled (0: none)
   int64_t cache_mb = 0;  // device memory for cached group-distance columns (0: no cache)
+  int64_t approx_num = 1, approx_den = 1;  // progressive answers within num / den of a certified bound
   int64_t count_states = 0;  // count the finite DP states of every query (an extra pass)
 };
 
@@ -537,6 +538,13 @@ class Engine {
     d.rq = (uint32_t*)(base + L.rq);
     d.phase = gst::PH_FILL;
     d.count = g_->opt.count_states ? 1u : 0u;
+    if (g_->opt.approx_num > g_->opt.approx_den) {
+      d.anum = (uint32_t)g_->opt.approx_num;
+      d.aden = (uint32_t)g_->opt.approx_den;
+    }
+    d.lbmin = gst::kInf;
+    d.cert = gst::kInf;
+    d.star_pk = ~0ull;
     d.best = gst::kInf;
     d.ans = ~0ull;
     d.out_cost = (int32_t)gst::kInf;
@@ -1035,6 +1043,7 @@ class Engine {
     out->counters.max_frontier = (int64_t)std::max(d.maxq, h.jmaxq);
     out->counters.cached_labels = h.cached;
     out->counters.states_finite = -1;
+    out->counters.lower_bound = d.cert >= gst::kInf ? GST_INFEASIBLE : (int64_t)d.cert;
     if (d.count && d.status == 0 && d.phase == gst::PH_DONE) {
       // the reference counts finite states after its main loop (count_set);
       // here a pass of its own over the finished query's tables
@@ -1483,6 +1492,8 @@ static int64_t* option_ref(gst_graph* g, const std::string& k) {
   if (k == "deadline_ms") return &g->opt.deadline_ms;
   if (k == "count_states") return &g->opt.count_states;
   if (k == "cache_mb") return &g->opt.cache_mb;
+  if (k == "approx_num") return &g->opt.approx_num;
+  if (k == "approx_den") return &g->opt.approx_den;
   return nullptr;
 }
 
@@ -1504,6 +1515,8 @@ gst_status gst_set_option(gst_graph* g, const char* key, int64_t value) {
   if ((k == "budget_mb" || k == "blocks_per_cu" || k == "timeout_ms" || k == "trace_slot" || k == "deadline_ms" ||
        k == "count_states" || k == "cache_mb") && value < 0)
     return fail(GST_INVALID_ARG, "negative value");
+  if ((k == "approx_num" || k == "approx_den") && (value < 1 || value > (1 << 20)))
+    return fail(GST_INVALID_ARG, "%s outside [1, 2^20]", key);
   if (k == "job_min_queries" && (value < 1 || value > gst::kJobLabels))
     return fail(GST_INVALID_ARG, "job_min_queries outside [1,%d]", gst::kJobLabels);
   std::lock_guard<std::mutex> lk(g->dev_mu);

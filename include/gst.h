@@ -95,6 +95,10 @@ typedef struct {
                                plus the stored states of two or more groups; -1 unless the option
                                "count_states" was set when the query was admitted */
   int64_t cached_labels;    /* its groups whose distances came from the group-distance cache */
+  int64_t lower_bound;      /* certified lower bound on the optimum: the cost itself when the
+                               answer is optimal; with a progressive ratio num / den the answer
+                               satisfies cost * den <= num * lower_bound (GST_INFEASIBLE when
+                               no tree exists) */
 } gst_counters;
 
 typedef struct {
@@ -167,7 +171,14 @@ void gst_result_free(gst_result *r);
  * query of gst_solve / gst_solve_batch / gst_submit still running this long
  * after its submission is stopped on the device at its next step and
  * answered with status GST_CANCELLED), "count_states" (1: fill
- * counters.states_finite with an extra pass), "cache_mb" (device memory of
+ * counters.states_finite with an extra pass), "approx_num" and
+ * "approx_den" (progressive answers, default 1 / 1 = exact: a query stops
+ * as soon as the cheapest tree it has found costs at most num / den times a
+ * certified lower bound -- checked in exact integers after its group
+ * distances and after each level of the DP -- and returns that tree with the
+ * bound in counters.lower_bound; the reference's maximum_return_app_ratio
+ * early exit, code/PrunedDP++/include/CPUNONHOP.h:730-755, whose int
+ * division is replaced by this exact test), "cache_mb" (device memory of
  * an engine's group-distance cache: columns of exact distances of one group
  * to every vertex, kept across batches and server epochs, least recently used
  * first out, evicted before admission fails; 0: no cache; used by queries

@@ -354,13 +354,46 @@ def check_cache():
     return out
 
 
+def check_progressive():
+    """Progressive answers (approx_num / approx_den): every answer is a valid
+    tree (walk-trees allowed) whose cost is at least the optimum and at most
+    num/den times the certified lower bound it reports, which is at most the
+    optimum; ratio 1/1 gives the optimum; plain and batched, with and
+    without a hop bound."""
+    rng, rp, col, wt, groups = random_instance(150, 37, deg=3, gmax=4)
+    G = len(groups)
+    g = Graph(rp, col, wt)
+    g.set_groups(*groups_csr(groups))
+    qs = [rng.sample(range(G), rng.randint(3, 5)) for _ in range(6)]
+    Ds = [-1, -1, 3, -1, 2, -1]
+    opt = [O.dpbf(rp, col, wt, [groups[x] for x in q], D=d)[0] for q, d in zip(qs, Ds)]
+    bad, early = [], 0
+    for num, den in ((1, 1), (3, 2), (2, 1), (4, 1)):
+        g.set_option("approx_num", num)
+        g.set_option("approx_den", den)
+        rs = [g.solve(q, D=d) for q, d in zip(qs, Ds)] + g.solve_batch(qs, Ds)
+        for i, r in enumerate(rs):
+            q, d, o = qs[i % 6], Ds[i % 6], opt[i % 6]
+            lb = r.counters["lower_bound"]
+            ok = r.cost >= o and lb <= o and r.cost * den <= num * lb
+            if num == 1:
+                ok = ok and r.cost == o
+            if ok and r.feasible:
+                ok = validate_tree(rp, col, wt, [groups[x] for x in q], r.cost, r.root, r.edges, D=d)["ok"]
+            early += int(r.cost != lb)
+            if not ok:
+                bad.append([num, den, i, d, r.cost, lb, o])
+    g.close()
+    return {"bad": bad, "early": early}
+
+
 def main():
     stride = int(sys.argv[1]) if len(sys.argv) > 1 else 3
     nq = int(sys.argv[2]) if len(sys.argv) > 2 else 60
     nh = int(sys.argv[3]) if len(sys.argv) > 3 else 24
     out = {"tiny": check_tiny(stride), "jobs": check_jobs(150, nq, 7), "hop": check_hop(150, nh, 11),
            "branching": check_branching(), "graph": check_graph(), "static": check_static(),
-           "robust": check_robust(), "cache": check_cache()}
+           "robust": check_robust(), "cache": check_cache(), "progressive": check_progressive()}
     print(json.dumps(out), flush=True)
 
 

@@ -84,3 +84,8 @@ def test_group_distance_cache_on_cpu_build(cpu_results):
     assert c["exact"] and c["single_exact"] and c["regroup_exact"], c
     assert c["hits_first"] == 0, c  # columns become readable only when their job ends
     assert c["inserts"] > 0 and c["hits_later"] > 0 and c["per_query"] == c["hits_later"], c
+
+
+def test_progressive_answers_on_cpu_build(cpu_results):
+    p = cpu_results["progressive"]
+    assert p["bad"] == [], p
