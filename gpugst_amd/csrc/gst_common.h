@@ -42,7 +42,10 @@ constexpr uint32_t kArcMask = (1u << 30) - 1;
 constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / 64;
 constexpr uint32_t kFillWords = 16384;  // u64 words per FILL item
-constexpr uint32_t kMergeTV = 64;       // vertices per MERGE / FINAL item
+constexpr uint32_t kMergeTV = 64;       // default vertices per MERGE / FINAL item (option merge_tv)
+// waves per block of the step kernels: a schedule option ("block"); the plan
+// kernel always runs kBlock threads
+constexpr int kWaveChoices[3] = {2, 4, 8};
 constexpr uint32_t kDistTV = 256;       // vertices per DIST item
 constexpr uint32_t kClearPer = 1024;    // mask clears per CLEAR item
 constexpr int kStageWords = 1152;       // per-wave LDS staging (u64): 64*ND rows or the W*k*k route table
@@ -181,6 +184,8 @@ struct alignas(64) Slot {
   unsigned long long star_pk;  // LIVE: cheapest star, (cost << 32) | centre vertex
   uint32_t probe;       // the FINAL under way is a checkpoint between levels
   uint32_t recon_star;  // RECON rebuilds the star of star_pk (shortest paths from its centre)
+  uint32_t mtv;         // live vertices per MERGE / FINAL item (schedule option merge_tv)
+  uint32_t pad6;
 };
 
 struct alignas(64) Ctrl {

@@ -1108,8 +1108,9 @@ __device__ void item_merge(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* 
   // of two or more groups at the vertex, so only active vertices can hold one
   const uint32_t* vl = level >= 3 ? sp->act : sp->live;
   const uint64_t nl = level >= 3 ? sp->nact : sp->nlive;
-  const uint64_t v0 = (uint64_t)tile * kMergeTV;
-  const uint64_t v1 = v0 + kMergeTV < nl ? v0 + kMergeTV : nl;
+  const uint64_t mtv = sp->mtv;
+  const uint64_t v0 = (uint64_t)tile * mtv;
+  const uint64_t v1 = v0 + mtv < nl ? v0 + mtv : nl;
   const uint32_t bst0 = __hip_atomic_load(&sp->best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   uint32_t ubmin = kInf;
   uint64_t n_eval = 0;
@@ -1264,8 +1265,9 @@ __device__ void item_final(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* 
   // with k >= 3 one part of every split has two or more groups: active vertices
   const uint32_t* vl = k >= 3 ? sp->act : sp->live;
   const uint64_t nl = k >= 3 ? sp->nact : sp->nlive;
-  const uint64_t v0 = (uint64_t)tile * kMergeTV;
-  const uint64_t v1 = v0 + kMergeTV < nl ? v0 + kMergeTV : nl;
+  const uint64_t mtv = sp->mtv;
+  const uint64_t v0 = (uint64_t)tile * mtv;
+  const uint64_t v1 = v0 + mtv < nl ? v0 + mtv : nl;
   uint64_t ansmin = ~0ull;
   uint64_t n_eval = 0;
   for (uint64_t vb = v0; vb < v1; vb += VPF) {
@@ -1729,8 +1731,8 @@ __device__ uint32_t plan_step(Slot* s, uint64_t V) {
     case PH_CACHEOUT: work = ceil_div(V, kStarTV); cls = kClsHeavy; break;
     case PH_DIST: work = (s->S1 ? ceil_div(V, kDistTV) : 0u) + 1u; cls = kClsHeavy; break;
     case PH_LIVE: work = ceil_div(s->vsrc ? (uint64_t)s->nvsrc : V, kDistTV); cls = kClsHeavy; break;
-    case PH_MERGE: work = ceil_div(s->level >= 3 ? s->nact : s->nlive, kMergeTV); cls = kClsHeavy; break;
-    case PH_FINAL: work = ceil_div(s->k >= 3 ? s->nact : s->nlive, kMergeTV); cls = kClsHeavy; break;
+    case PH_MERGE: work = ceil_div(s->level >= 3 ? s->nact : s->nlive, s->mtv); cls = kClsHeavy; break;
+    case PH_FINAL: work = ceil_div(s->k >= 3 ? s->nact : s->nlive, s->mtv); cls = kClsHeavy; break;
     case PH_RECON: work = 1; cls = kClsHeavy; break;
     default: return (s->n_work = s->n_clr = 0);
   }
@@ -1853,7 +1855,7 @@ __device__ void phase_b(Slot* slots, Ctrl* ctrl, uint64_t V, uint32_t (*sh_items
 
 // One engine step of one work class for one wave: pull items of the current
 // plan until they are exhausted.
-template <int CLS, class LDS>
+template <int CLS, int WPB, class LDS>
 __device__ void step_body(const GraphDev& g, Slot* slots, Ctrl* ctrl, LDS* L) {
   const int lane = lane_id();
   const uint64_t V = (uint64_t)g.V;
@@ -1862,7 +1864,7 @@ __device__ void step_body(const GraphDev& g, Slot* slots, Ctrl* ctrl, LDS* L) {
   // waves beyond the item count have nothing to claim: leave before touching
   // the shared counter (a small step then costs one atomic per item, not one
   // per wave of the grid)
-  if (blockIdx.x * (uint32_t)kWaves + (threadIdx.x >> 6) >= total) return;
+  if (blockIdx.x * (uint32_t)WPB + (threadIdx.x >> 6) >= total) return;
   const uint32_t nact = ctrl->n_active[CLS];
   const uint32_t* prefix = ctrl->prefix[CLS];
   const uint32_t* active = ctrl->active[CLS];
@@ -1870,7 +1872,7 @@ __device__ void step_body(const GraphDev& g, Slot* slots, Ctrl* ctrl, LDS* L) {
   // claim from one counter, whose atomics serialise at its L2 bank, so a step
   // with many small items (LIVE: one per 256 vertices of every query) would
   // otherwise queue on it.  The run leaves about 8 claims per wave.
-  const uint32_t nwaves = gridDim.x * (uint32_t)kWaves;
+  const uint32_t nwaves = gridDim.x * (uint32_t)WPB;
   uint32_t run = total / (8u * nwaves);
   run = run < 1u ? 1u : (run > 16u ? 16u : run);
   for (;;) {
@@ -1942,19 +1944,25 @@ __device__ void step_body(const GraphDev& g, Slot* slots, Ctrl* ctrl, LDS* L) {
 // the kernels end, and the next plan kernel sees everything they wrote.
 // Five waves per SIMD (96 VGPRs) without spills: held there so that code
 // added to rare items cannot lower the job rounds' occupancy.
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) void gst_lean_kernel(GraphDev g, Slot* slots,
-                                                                                              Ctrl* ctrl) {
-  __shared__ FrontLDS lds[kWaves];
-  step_body<kClsLean>(g, slots, ctrl, &lds[threadIdx.x >> 6]);
+// Blocks of WPB waves (a schedule option); the claim loop is per wave, so the
+// block shape sets only how waves share a CU's LDS and how the grid fills it.
+template <int WPB>
+__global__ __launch_bounds__(WPB * 64) __attribute__((amdgpu_waves_per_eu(5))) void gst_lean_kernel(GraphDev g,
+                                                                                                Slot* slots,
+                                                                                                Ctrl* ctrl) {
+  __shared__ FrontLDS lds[WPB];
+  step_body<kClsLean, WPB>(g, slots, ctrl, &lds[threadIdx.x >> 6]);
 }
 
 // Three waves per SIMD is what its LDS allows (3 blocks of 4 waves per CU);
 // the register budget is held to match (one loop-invariant 16-byte value is
 // kept in scratch).
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void gst_heavy_kernel(GraphDev g, Slot* slots,
-                                                                                               Ctrl* ctrl) {
-  __shared__ WaveLDS lds[kWaves];
-  step_body<kClsHeavy>(g, slots, ctrl, &lds[threadIdx.x >> 6]);
+template <int WPB>
+__global__ __launch_bounds__(WPB * 64) __attribute__((amdgpu_waves_per_eu(3))) void gst_heavy_kernel(GraphDev g,
+                                                                                                 Slot* slots,
+                                                                                                 Ctrl* ctrl) {
+  __shared__ WaveLDS lds[WPB];
+  step_body<kClsHeavy, WPB>(g, slots, ctrl, &lds[threadIdx.x >> 6]);
 }
 
 // Between two steps, one block advances every slot's state machine and
@@ -1977,13 +1985,27 @@ __global__ __launch_bounds__(kBlock) void gst_plan_kernel(Slot* slots, Ctrl* ctr
 }  // namespace gst
 
 #ifndef GST_CPU_SIM
-extern "C" hipError_t gst_launch_step(int cls, const gst::GraphDev* g, gst::Slot* slots, gst::Ctrl* ctrl,
-                                      int grid, hipStream_t stream) {
+namespace {
+template <int WPB>
+hipError_t launch_step_wpb(int cls, const gst::GraphDev* g, gst::Slot* slots, gst::Ctrl* ctrl, int grid,
+                           hipStream_t stream) {
   if (cls == gst::kClsLean)
-    hipLaunchKernelGGL(gst::gst_lean_kernel, dim3(grid), dim3(gst::kBlock), 0, stream, *g, slots, ctrl);
+    hipLaunchKernelGGL(gst::gst_lean_kernel<WPB>, dim3(grid), dim3(WPB * 64), 0, stream, *g, slots, ctrl);
   else
-    hipLaunchKernelGGL(gst::gst_heavy_kernel, dim3(grid), dim3(gst::kBlock), 0, stream, *g, slots, ctrl);
+    hipLaunchKernelGGL(gst::gst_heavy_kernel<WPB>, dim3(grid), dim3(WPB * 64), 0, stream, *g, slots, ctrl);
   return hipGetLastError();
+}
+}  // namespace
+
+// wpb: waves per block, one of kWaveChoices
+extern "C" hipError_t gst_launch_step(int cls, int wpb, const gst::GraphDev* g, gst::Slot* slots, gst::Ctrl* ctrl,
+                                      int grid, hipStream_t stream) {
+  switch (wpb) {
+    case 2: return launch_step_wpb<2>(cls, g, slots, ctrl, grid, stream);
+    case 4: return launch_step_wpb<4>(cls, g, slots, ctrl, grid, stream);
+    case 8: return launch_step_wpb<8>(cls, g, slots, ctrl, grid, stream);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 extern "C" hipError_t gst_launch_plan(gst::Slot* slots, gst::Ctrl* ctrl, uint64_t V, hipStream_t stream) {
@@ -1997,9 +2019,15 @@ extern "C" hipError_t gst_launch_count(gst::Slot* slot, uint64_t V, hipStream_t 
   return hipGetLastError();
 }
 
-extern "C" hipError_t gst_step_occupancy(int cls, int* blocks_per_cu) {
-  const void* f = cls == gst::kClsLean ? (const void*)gst::gst_lean_kernel : (const void*)gst::gst_heavy_kernel;
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, f, gst::kBlock, 0);
+extern "C" hipError_t gst_step_occupancy(int cls, int wpb, int* blocks_per_cu) {
+  const void* f = nullptr;
+  switch (wpb) {
+    case 2: f = cls == gst::kClsLean ? (const void*)gst::gst_lean_kernel<2> : (const void*)gst::gst_heavy_kernel<2>; break;
+    case 4: f = cls == gst::kClsLean ? (const void*)gst::gst_lean_kernel<4> : (const void*)gst::gst_heavy_kernel<4>; break;
+    case 8: f = cls == gst::kClsLean ? (const void*)gst::gst_lean_kernel<8> : (const void*)gst::gst_heavy_kernel<8>; break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, f, wpb * 64, 0);
 }
 #endif  // GST_CPU_SIM
 

@@ -24,10 +24,10 @@
 #include "gst.h"
 #include "gst_common.h"
 
-extern "C" hipError_t gst_launch_step(int cls, const gst::GraphDev* g, gst::Slot* slots, gst::Ctrl* ctrl,
+extern "C" hipError_t gst_launch_step(int cls, int wpb, const gst::GraphDev* g, gst::Slot* slots, gst::Ctrl* ctrl,
                                       int grid, hipStream_t stream);
 extern "C" hipError_t gst_launch_plan(gst::Slot* slots, gst::Ctrl* ctrl, uint64_t V, hipStream_t stream);
-extern "C" hipError_t gst_step_occupancy(int cls, int* blocks_per_cu);
+extern "C" hipError_t gst_step_occupancy(int cls, int wpb, int* blocks_per_cu);
 extern "C" hipError_t gst_scratch_setup(int device, void** ptr);
 extern "C" hipError_t gst_launch_count(gst::Slot* slot, uint64_t V, hipStream_t stream);
 
@@ -129,15 +129,22 @@ struct Options {
   int64_t deadline_ms = 0;  // a query still running this long after its submission is cancelled (0: none)
   int64_t cache_mb = 0;  // device memory for cached group-distance columns (0: no cache)
   int64_t approx_num = 1, approx_den = 1;  // progressive answers within num / den of a certified bound
+  // schedule (the autotuner's knobs besides chunk / blocks_per_cu / graph / max_slots)
+  int64_t block = 2;      // waves per block of the step kernels: 2, 4 or 8 (2: measured fastest)
+  int64_t merge_tv = gst::kMergeTV;  // live vertices per MERGE / FINAL work item
+  int64_t job_lanes = 0;  // lanes sharing an arc in job rounds (power of two; 0: the smallest with <= 8 chunks each)
   int64_t count_states = 0;  // count the finite DP states of every query (an extra pass)
 };
 
 }  // namespace
 
+// index of a block shape (waves per block) in kWaveChoices
+static int wpb_index(int64_t wpb) { return wpb == 2 ? 0 : (wpb == 8 ? 2 : 1); }
+
 struct gst_graph {
   int device = 0;
   int num_cu = 0;
-  int occ[gst::kNumCls] = {1, 1};  // blocks per CU of the lean and heavy step kernels
+  int occ[3][gst::kNumCls] = {{1, 1}, {1, 1}, {1, 1}};  // blocks per CU of the step kernels, per block shape
   int64_t V = 0, n_arcs = 0;
   int64_t n_groups = 0;
   int64_t groups_version = 0;  // bumped by every gst_groups_set (group ids change meaning)
@@ -243,6 +250,7 @@ JobLayout job_layout(const gst_graph* g, int L, int nmem, int D) {
   const int nch = J.Lp / gst::kJobChunkLabels;
   J.G = 1;
   while (J.G * gst::kJobChunksPerLane < nch) J.G <<= 1;
+  if (g->opt.job_lanes > 0) J.G = (int)g->opt.job_lanes;
   J.chunk = (int)gst::kJobArcs;
   J.istride = istride_for(g, J.chunk);
   size_t off = 0;
@@ -538,6 +546,7 @@ class Engine {
     d.rq = (uint32_t*)(base + L.rq);
     d.phase = gst::PH_FILL;
     d.count = g_->opt.count_states ? 1u : 0u;
+    d.mtv = (uint32_t)g_->opt.merge_tv;
     if (g_->opt.approx_num > g_->opt.approx_den) {
       d.anum = (uint32_t)g_->opt.approx_num;
       d.aden = (uint32_t)g_->opt.approx_den;
@@ -809,7 +818,7 @@ class Engine {
     int chunk = use_graph ? kGraphChunk : 4;
     for (;;) {
       if (use_graph) {
-        if (gexec_ && (graph_grid_[0] != grid_l || graph_grid_[1] != grid_h)) drop_graph();
+        if (gexec_ && (graph_grid_[0] != grid_l || graph_grid_[1] != grid_h || graph_wpb_ != wpb())) drop_graph();
         if (!gexec_) {
           gst_status st = capture_chunk(grid_l, grid_h);
           if (st != GST_OK) return st;
@@ -818,9 +827,9 @@ class Engine {
       } else {
         for (int i = 0; i < chunk; i++) {
           if (timed) HIP_TRY(hipEventRecord(tev_[4 * i], stream_));
-          le = gst_launch_step(gst::kClsLean, &gd, d_slots_, d_ctrl_, grid_l, stream_);
+          le = gst_launch_step(gst::kClsLean, wpb(), &gd, d_slots_, d_ctrl_, grid_l, stream_);
           if (timed && le == hipSuccess) le = hipEventRecord(tev_[4 * i + 1], stream_);
-          if (le == hipSuccess) le = gst_launch_step(gst::kClsHeavy, &gd, d_slots_, d_ctrl_, grid_h, stream_);
+          if (le == hipSuccess) le = gst_launch_step(gst::kClsHeavy, wpb(), &gd, d_slots_, d_ctrl_, grid_h, stream_);
           if (timed && le == hipSuccess) le = hipEventRecord(tev_[4 * i + 2], stream_);
           if (le == hipSuccess) le = gst_launch_plan(d_slots_, d_ctrl_, (uint64_t)g_->V, stream_);
           if (le != hipSuccess) return fail(GST_DEVICE_ERROR, "step launch failed: %s", hipGetErrorString(le));
@@ -992,8 +1001,8 @@ class Engine {
     HIP_TRY(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
     hipError_t le = hipSuccess;
     for (int i = 0; i < kGraphChunk && le == hipSuccess; i++) {
-      le = gst_launch_step(gst::kClsLean, &gd, d_slots_, d_ctrl_, grid_l, stream_);
-      if (le == hipSuccess) le = gst_launch_step(gst::kClsHeavy, &gd, d_slots_, d_ctrl_, grid_h, stream_);
+      le = gst_launch_step(gst::kClsLean, wpb(), &gd, d_slots_, d_ctrl_, grid_l, stream_);
+      if (le == hipSuccess) le = gst_launch_step(gst::kClsHeavy, wpb(), &gd, d_slots_, d_ctrl_, grid_h, stream_);
       if (le == hipSuccess) le = gst_launch_plan(d_slots_, d_ctrl_, (uint64_t)g_->V, stream_);
     }
     hipGraph_t graph = nullptr;
@@ -1008,6 +1017,7 @@ class Engine {
     }
     graph_grid_[0] = grid_l;
     graph_grid_[1] = grid_h;
+    graph_wpb_ = wpb();
     return GST_OK;
   }
 
@@ -1151,9 +1161,12 @@ class Engine {
   }
   size_t largest_free() const { return arena_.largest(); }
 
+  int wpb() const { return gst::kWaveChoices[wpb_index(g_->opt.block)]; }
+
   int grid_size(int cls) const {
-    int bpc = g_->occ[cls];
-    if (g_->opt.blocks_per_cu > 0) bpc = std::min<int>((int)g_->opt.blocks_per_cu, g_->occ[cls]);
+    const int occ = g_->occ[wpb_index(g_->opt.block)][cls];
+    int bpc = occ;
+    if (g_->opt.blocks_per_cu > 0) bpc = std::min<int>((int)g_->opt.blocks_per_cu, occ);
     return std::max(1, bpc) * g_->num_cu;
   }
 
@@ -1196,6 +1209,7 @@ class Engine {
   std::vector<hipEvent_t> tev_;  // per-step class timing (option time_steps)
   hipGraphExec_t gexec_ = nullptr;
   int graph_grid_[2] = {0, 0};
+  int graph_wpb_ = 0;
   int64_t kernel_ns_ = 0, launches_ = 0, steps_total_ = 0;
   int64_t cls_ns_[2] = {0, 0}, cls_launches_ = 0, jobs_done_ = 0;
   int64_t job_ctr_[6] = {0, 0, 0, 0, 0, 0};
@@ -1409,10 +1423,11 @@ gst_status gst_graph_create(const int64_t* beg_pos, int64_t V, const int32_t* ad
   hipDeviceProp_t prop;
   HIP_TRY(hipGetDeviceProperties(&prop, g->device));
   g->num_cu = prop.multiProcessorCount;
-  for (int c = 0; c < gst::kNumCls; c++) {
-    int occ = 1;
-    if (gst_step_occupancy(c, &occ) == hipSuccess && occ > 0) g->occ[c] = occ;
-  }
+  for (int b = 0; b < 3; b++)
+    for (int c = 0; c < gst::kNumCls; c++) {
+      int occ = 1;
+      if (gst_step_occupancy(c, gst::kWaveChoices[b], &occ) == hipSuccess && occ > 0) g->occ[b][c] = occ;
+    }
   g->V = V;
   g->n_arcs = n_arcs;
   {
@@ -1492,6 +1507,9 @@ static int64_t* option_ref(gst_graph* g, const std::string& k) {
   if (k == "deadline_ms") return &g->opt.deadline_ms;
   if (k == "count_states") return &g->opt.count_states;
   if (k == "cache_mb") return &g->opt.cache_mb;
+  if (k == "block") return &g->opt.block;
+  if (k == "merge_tv") return &g->opt.merge_tv;
+  if (k == "job_lanes") return &g->opt.job_lanes;
   if (k == "approx_num") return &g->opt.approx_num;
   if (k == "approx_den") return &g->opt.approx_den;
   return nullptr;
@@ -1515,6 +1533,11 @@ gst_status gst_set_option(gst_graph* g, const char* key, int64_t value) {
   if ((k == "budget_mb" || k == "blocks_per_cu" || k == "timeout_ms" || k == "trace_slot" || k == "deadline_ms" ||
        k == "count_states" || k == "cache_mb") && value < 0)
     return fail(GST_INVALID_ARG, "negative value");
+  if (k == "block" && value != 2 && value != 4 && value != 8)
+    return fail(GST_INVALID_ARG, "block (waves per block) must be 2, 4 or 8");
+  if (k == "merge_tv" && (value < 1 || value > 4096)) return fail(GST_INVALID_ARG, "merge_tv outside [1, 4096]");
+  if (k == "job_lanes" && (value < 0 || value > 64 || (value & (value - 1))))
+    return fail(GST_INVALID_ARG, "job_lanes must be 0 or a power of two up to 64");
   if ((k == "approx_num" || k == "approx_den") && (value < 1 || value > (1 << 20)))
     return fail(GST_INVALID_ARG, "%s outside [1, 2^20]", key);
   if (k == "job_min_queries" && (value < 1 || value > gst::kJobLabels))
@@ -1561,11 +1584,12 @@ gst_status gst_get_option(gst_graph* g, const char* key, int64_t* value) {
     }
   }
   auto grid_of = [&](int c) {
-    const int64_t b = g->opt.blocks_per_cu > 0 ? std::min<int64_t>(g->opt.blocks_per_cu, g->occ[c]) : g->occ[c];
+    const int occ = g->occ[wpb_index(g->opt.block)][c];
+    const int64_t b = g->opt.blocks_per_cu > 0 ? std::min<int64_t>(g->opt.blocks_per_cu, occ) : occ;
     return std::max<int64_t>(1, b) * g->num_cu;
   };
-  if (k == "occupancy") { *value = g->occ[gst::kClsLean]; return GST_OK; }
-  if (k == "occupancy_heavy") { *value = g->occ[gst::kClsHeavy]; return GST_OK; }
+  if (k == "occupancy") { *value = g->occ[wpb_index(g->opt.block)][gst::kClsLean]; return GST_OK; }
+  if (k == "occupancy_heavy") { *value = g->occ[wpb_index(g->opt.block)][gst::kClsHeavy]; return GST_OK; }
   if (k == "grid") { *value = grid_of(gst::kClsLean); return GST_OK; }
   if (k == "grid_heavy") { *value = grid_of(gst::kClsHeavy); return GST_OK; }
   int64_t* r = option_ref(g, key);

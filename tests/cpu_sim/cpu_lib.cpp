@@ -48,10 +48,10 @@ extern "C" hipError_t gst_launch_plan(gst::Slot* slots, gst::Ctrl* ctrl, uint64_
   return hipSuccess;
 }
 
-template <int CLS, class LDS>
+template <int CLS, int WPB, class LDS>
 static void run_class(const gst::GraphDev* g, gst::Slot* slots, gst::Ctrl* ctrl, int grid) {
   if (ctrl->exit_flag || ctrl->item_total[CLS] == 0) return;
-  int n_waves = grid * gst::kWaves;
+  int n_waves = grid * WPB;
   if ((uint32_t)n_waves > ctrl->item_total[CLS]) n_waves = (int)ctrl->item_total[CLS];
   std::vector<gsim::Wave> waves(n_waves);
   std::vector<LDS> lds(n_waves);
@@ -62,25 +62,35 @@ static void run_class(const gst::GraphDev* g, gst::Slot* slots, gst::Ctrl* ctrl,
       th.emplace_back([&, w, l]() {
         gsim::lane = l;
         gsim::wave = &waves[w];
-        gsim::thread_idx.x = (unsigned)((w % gst::kWaves) * 64 + l);
-        gsim::block_idx.x = (unsigned)(w / gst::kWaves);
-        gsim::grid_dim.x = (unsigned)((n_waves + gst::kWaves - 1) / gst::kWaves);
-        gst::step_body<CLS>(gd, slots, ctrl, &lds[w]);
+        gsim::thread_idx.x = (unsigned)((w % WPB) * 64 + l);
+        gsim::block_idx.x = (unsigned)(w / WPB);
+        gsim::grid_dim.x = (unsigned)((n_waves + WPB - 1) / WPB);
+        gst::step_body<CLS, WPB>(gd, slots, ctrl, &lds[w]);
       });
   for (auto& x : th) x.join();
 }
 
-extern "C" hipError_t gst_launch_step(int cls, const gst::GraphDev* g, gst::Slot* slots, gst::Ctrl* ctrl,
+template <int WPB>
+static void run_step(int cls, const gst::GraphDev* g, gst::Slot* slots, gst::Ctrl* ctrl, int grid) {
+  if (cls == gst::kClsLean)
+    run_class<gst::kClsLean, WPB, gst::FrontLDS>(g, slots, ctrl, grid);
+  else
+    run_class<gst::kClsHeavy, WPB, gst::WaveLDS>(g, slots, ctrl, grid);
+}
+
+extern "C" hipError_t gst_launch_step(int cls, int wpb, const gst::GraphDev* g, gst::Slot* slots, gst::Ctrl* ctrl,
                                       int grid, hipStream_t s) {
   if (gsim_capturing()) {
     const gst::GraphDev gd = *g;  // kernel arguments are copied at capture
-    gsim_record([=]() { gst_launch_step(cls, &gd, slots, ctrl, grid, s); });
+    gsim_record([=]() { gst_launch_step(cls, wpb, &gd, slots, ctrl, grid, s); });
     return hipSuccess;
   }
-  if (cls == gst::kClsLean)
-    run_class<gst::kClsLean, gst::FrontLDS>(g, slots, ctrl, grid);
-  else
-    run_class<gst::kClsHeavy, gst::WaveLDS>(g, slots, ctrl, grid);
+  switch (wpb) {
+    case 2: run_step<2>(cls, g, slots, ctrl, grid); break;
+    case 4: run_step<4>(cls, g, slots, ctrl, grid); break;
+    case 8: run_step<8>(cls, g, slots, ctrl, grid); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipSuccess;
 }
 
@@ -100,7 +110,7 @@ extern "C" hipError_t gst_launch_count(gst::Slot* slot, uint64_t V, hipStream_t)
   return hipSuccess;
 }
 
-extern "C" hipError_t gst_step_occupancy(int, int* blocks_per_cu) {
+extern "C" hipError_t gst_step_occupancy(int, int, int* blocks_per_cu) {
   *blocks_per_cu = 1;
   return hipSuccess;
 }

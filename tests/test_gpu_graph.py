@@ -40,3 +40,30 @@ def test_graph_replays_equal_eager():
                                   a.edges, D=Ds[i])
                 assert v["ok"], (rep, i, v)
     g.close()
+
+
+@pytest.mark.parametrize("opts", [{"block": 2}, {"block": 8}, {"merge_tv": 32}, {"merge_tv": 128},
+                                  {"job_lanes": 4}, {"job_lanes": 16}, {"block": 8, "graph": 1}])
+def test_schedules_equal_default(opts):
+    """The autotuner's knobs (waves per block, merge work-item size, lanes per
+    arc in job rounds, graph replay) change how work is cut, never the
+    answers: every schedule returns the default schedule's costs, with valid
+    trees."""
+    inst = data.make_config(2, scale=0.02, nq=8)
+    g = Graph(inst.rowptr, inst.col, inst.wt)
+    g.set_groups(inst.gptr, inst.gmem)
+    qs = list(inst.queries)
+    Ds = [-1 if i % 4 else 3 for i in range(len(qs))]
+    base = g.solve_batch(qs, Ds)
+    for k, v in opts.items():
+        g.set_option(k, v)
+    print("schedule", opts, flush=True)
+    got = g.solve_batch(qs, Ds)
+    single = g.solve(qs[1], D=Ds[1])
+    assert single.cost == base[1].cost
+    for i, (a, b) in enumerate(zip(got, base)):
+        assert a.status == 0 and a.cost == b.cost, (opts, i, a.cost, b.cost)
+        if a.feasible:
+            assert validate_tree(inst.rowptr, inst.col, inst.wt, inst.query_groups(i), a.cost, a.root,
+                                 a.edges, D=Ds[i])["ok"]
+    g.close()
