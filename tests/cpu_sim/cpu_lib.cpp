@@ -110,7 +110,7 @@ extern "C" hipError_t gst_launch_count(gst::Slot* slot, uint64_t V, hipStream_t)
   return hipSuccess;
 }
 
-extern "C" hipError_t gst_step_occupancy(int, int, int* blocks_per_cu) {
-  *blocks_per_cu = 1;
+extern "C" hipError_t gst_step_occupancy(int, int wpb, int* blocks_per_cu) {
+  *blocks_per_cu = wpb >= 4 ? 1 : 4 / wpb;  // at least four emulated waves per launch
   return hipSuccess;
 }

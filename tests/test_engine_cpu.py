@@ -27,9 +27,9 @@ def cpu_results():
     except (OSError, subprocess.CalledProcessError) as e:  # no g++
         pytest.skip(f"CPU build unavailable: {e}")
     env = dict(os.environ, GST_LIB=lib)
-    # every 20th tiny case (all D; the GPU tests take all of them), 54 k=5
+    # every 30th tiny case (all D; the GPU tests take all of them), 54 k=5
     # queries = 270 labels = two jobs, and 16 hop-bounded queries
-    r = subprocess.run([sys.executable, os.path.join(ROOT, "tests", "cpu_sim", "run_checks.py"), "20", "54", "16"],
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "tests", "cpu_sim", "run_checks.py"), "30", "54", "16"],
                        env=env, capture_output=True, text=True, timeout=900)
     assert r.returncode == 0, r.stderr[-2000:]
     return json.loads(r.stdout.strip().splitlines()[-1])
