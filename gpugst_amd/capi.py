@@ -31,7 +31,7 @@ STATUS_NAMES = {0: "OK", 1: "INVALID_ARG", 2: "OUT_OF_MEMORY", 3: "DEVICE_ERROR"
                 4: "CANCELLED", 5: "NOT_READY", 6: "QUEUE_FULL"}
 
 # every function include/gst.h declares
-EXPORTED = ("gst_graph_create", "gst_graph_destroy", "gst_groups_set", "gst_solve",
+EXPORTED = ("gst_graph_create", "gst_graph_destroy", "gst_backbone_get", "gst_groups_set", "gst_solve",
             "gst_solve_static", "gst_solve_batch", "gst_result_free", "gst_set_option", "gst_get_option",
             "gst_query_bytes", "gst_server_create", "gst_submit", "gst_poll", "gst_wait",
             "gst_cancel", "gst_server_destroy", "gst_last_error", "gst_version")
@@ -93,6 +93,7 @@ def load():
     sig = {
         "gst_graph_create": [P, I64, P, P, I64, ctypes.POINTER(P)],
         "gst_graph_destroy": [P],
+        "gst_backbone_get": [P, P, P, P, ctypes.POINTER(I64)],
         "gst_groups_set": [P, P, P, I64],
         "gst_solve": [P, P, I32, I32, ctypes.POINTER(c_result)],
         "gst_solve_batch": [P, I64, P, P],
@@ -183,6 +184,18 @@ class Graph:
             gmem = np.zeros(1, dtype=np.int32)
         _check(load().gst_groups_set(self._h, _ptr(gptr), _ptr(gmem), gptr.shape[0] - 1))
         self.n_groups = gptr.shape[0] - 1
+
+    def backbone(self):
+        """The arcs that queries without a hop bound run on (gst_backbone_get):
+        (rowptr int64[V+1], col int32, wt int32)."""
+        L = load()
+        rp = np.zeros(self.V + 1, dtype=np.int64)
+        n = ctypes.c_int64()
+        _check(L.gst_backbone_get(self._h, _ptr(rp), None, None, ctypes.byref(n)))
+        col = np.zeros(max(n.value, 1), dtype=np.int32)
+        wt = np.zeros(max(n.value, 1), dtype=np.int32)
+        _check(L.gst_backbone_get(self._h, _ptr(rp), _ptr(col), _ptr(wt), ctypes.byref(n)))
+        return rp, col[:n.value], wt[:n.value]
 
     def set_option(self, key: str, value: int):
         _check(load().gst_set_option(self._h, key.encode(), int(value)))

@@ -60,6 +60,11 @@ constexpr uint32_t kJobArcs = 512;      // arcs per job ROUND item (swept: 128 /
 constexpr int kJobChunksPerLane = 8;    // chunks one lane relaxes per arc (swept: 2 / 4 / 8)
 constexpr uint32_t kStarTV = 64;        // vertices per job STAR item
 
+// Shortest-path backbone, built once per graph: exact distances from kBbHubs
+// hub vertices (one 256-byte row per vertex) prove arcs redundant.
+constexpr int kBbHubs = 64;
+constexpr uint32_t kBbArcs = 256;       // arcs per backbone work item
+
 // work classes: each has its own kernel, so the arc-parallel items do not run
 // at the register budget (and occupancy) of the merge items
 constexpr int kClsLean = 0;   // FILL, SEED, job ROUND, STAR, CLEAR
@@ -95,15 +100,35 @@ enum SlotKind : int32_t {
   KIND_JOB = 1
 };
 
+// The graph's arcs are followed by those of its shortest-path backbone (the
+// arcs (u, v, w) with no u-v path shorter than w, same row order): col / wt /
+// rev hold n_arcs + backbone arcs, rowptr holds the graph's V + 1 row starts,
+// then the backbone's (into the same arrays, so from n_arcs on).  A slot with
+// Slot::bb set (unbounded queries and jobs) reads rows through the second
+// half of rowptr; its arc indices (grow predecessors, frontier rows) are
+// indices into the same col / wt / rev.
 struct GraphDev {
-  const int64_t* rowptr;  // V + 1
-  const int32_t* col;     // n_arcs
-  const int32_t* wt;      // n_arcs
-  const int32_t* rev;     // n_arcs: index of the reverse arc
+  const int64_t* rowptr;  // V + 1 (+ V + 1 of the backbone)
+  const int32_t* col;     // n_arcs (+ backbone arcs)
+  const int32_t* wt;      // as col
+  const int32_t* rev;     // as col: index of the reverse arc
   const int64_t* gptr;    // n_groups + 1
   const int32_t* gmem;    // members
   int64_t V;
   int64_t n_arcs;
+};
+
+// Arguments of the backbone construction passes (hub rounds, arc test).
+struct BbArgs {
+  const int64_t* rowptr;
+  const int32_t* col;
+  const int32_t* wt;
+  uint64_t V, n_arcs;
+  uint32_t* dist;       // V rows of kBbHubs labels (256 bytes)
+  const uint8_t* dcur;  // per vertex: its row changed in the round before
+  uint8_t* dnxt;        // per vertex: its row changed in this round
+  uint32_t* changed;    // set when any row changed in this round
+  uint8_t* keep;        // per arc: 0 when proved redundant
 };
 
 struct EdgeDev {
@@ -185,7 +210,7 @@ struct alignas(64) Slot {
   uint32_t probe;       // the FINAL under way is a checkpoint between levels
   uint32_t recon_star;  // RECON rebuilds the star of star_pk (shortest paths from its centre)
   uint32_t mtv;         // live vertices per MERGE / FINAL item (schedule option merge_tv)
-  uint32_t pad6;
+  uint32_t bb;          // 1: the slot's arcs are the graph's shortest-path backbone
 };
 
 struct alignas(64) Ctrl {

@@ -181,6 +181,25 @@ __device__ __forceinline__ States states_of(const Slot* sp) {
   return st;
 }
 
+// Arcs of a slot: the graph's shortest-path backbone (Slot::bb) for
+// unbounded queries and jobs, every arc otherwise.  Only the row starts
+// differ: the backbone's arcs follow the graph's in col / wt / rev.
+struct Arcs {
+  const int64_t* rowptr;
+  const int32_t* col;
+  const int32_t* wt;
+  const int32_t* rev;
+};
+
+__device__ __forceinline__ Arcs arcs_of(const GraphDev& g, const Slot* sp) {
+  Arcs a;
+  a.rowptr = g.rowptr + (sp->bb ? (uint64_t)g.V + 1u : 0u);
+  a.col = g.col;
+  a.wt = g.wt;
+  a.rev = g.rev;
+  return a;
+}
+
 // Row of vertex v in the distance table C.  A hop query's table holds D + 1
 // cumulative rows per vertex, row h the group distances within h hops (its own
 // table, or its job's cumulative rows); an unbounded query has
@@ -258,12 +277,12 @@ __device__ __forceinline__ bool lb_le(g_cu32* __restrict__ Crow, int k, uint32_t
 // holds an item's first arc records its list position in istart, so the item
 // starts without a search.
 __device__ __forceinline__ void wave_append(bool flag, uint32_t u, Slot* sp, int buf,
-                                            const GraphDev& g, uint64_t V) {
+                                            const int64_t* rowptr, uint64_t V) {
   const uint64_t ball = __ballot(flag);
   if (ball == 0) return;
   const int lane = lane_id();
   uint32_t deg = 0;
-  if (flag) deg = (uint32_t)(g.rowptr[u + 1] - g.rowptr[u]);
+  if (flag) deg = (uint32_t)(rowptr[u + 1] - rowptr[u]);
   const uint32_t excl = wave_excl_scan(deg);
   const uint32_t total = __shfl(excl + deg, 63, 64);
   const int leader = __ffsll((long long)ball) - 1;
@@ -291,18 +310,19 @@ __device__ __forceinline__ void wave_append(bool flag, uint32_t u, Slot* sp, int
 // LDS and are published in batches of 64 or more with one atomicAdd, so the
 // waves of a dense round do not queue on the frontier counter once per
 // iteration.  flush_append must run before the item ends.
-__device__ void flush_append(FrontLDS* L, uint32_t& apn, Slot* sp, int buf, const GraphDev& g, uint64_t V) {
+__device__ void flush_append(FrontLDS* L, uint32_t& apn, Slot* sp, int buf, const int64_t* rowptr,
+                             uint64_t V) {
   if (apn == 0) return;
   wave_sync();
   const int lane = lane_id();
   uint32_t u0 = 0, u1 = 0, d0 = 0, d1 = 0;
   if ((uint32_t)lane < apn) {
     u0 = L->av[lane];
-    d0 = (uint32_t)(g.rowptr[u0 + 1] - g.rowptr[u0]);
+    d0 = (uint32_t)(rowptr[u0 + 1] - rowptr[u0]);
   }
   if ((uint32_t)lane + 64 < apn) {
     u1 = L->av[lane + 64];
-    d1 = (uint32_t)(g.rowptr[u1 + 1] - g.rowptr[u1]);
+    d1 = (uint32_t)(rowptr[u1 + 1] - rowptr[u1]);
   }
   const uint32_t e0 = wave_excl_scan(d0);
   const uint32_t t0 = __shfl(e0 + d0, 63, 64);
@@ -333,12 +353,12 @@ __device__ void flush_append(FrontLDS* L, uint32_t& apn, Slot* sp, int buf, cons
 }
 
 __device__ __forceinline__ void stage_append(bool flag, uint32_t u, FrontLDS* L, uint32_t& apn, Slot* sp,
-                                             int buf, const GraphDev& g, uint64_t V) {
+                                             int buf, const int64_t* rowptr, uint64_t V) {
   const uint64_t ball = __ballot(flag);
   if (ball == 0) return;
   if (flag) L->av[apn + __popcll(ball & __lanemask_lt())] = u;
   apn += (uint32_t)__popcll(ball);
-  if (apn >= 64) flush_append(L, apn, sp, buf, g, V);
+  if (apn >= 64) flush_append(L, apn, sp, buf, rowptr, V);
 }
 
 // Sets the bits `imp` of the next round's dirty mask of u; true when u had
@@ -419,6 +439,7 @@ __device__ void item_fill(Slot* sp, uint32_t idx, uint64_t V) {
 }
 
 __device__ void item_seed(const GraphDev& g, Slot* sp, uint32_t i) {
+  const Arcs A = arcs_of(g, sp);
   const int ND = sp->ND;
   const int nxt = (sp->step + 1) % 3;
   const uint64_t V = (uint64_t)g.V;
@@ -438,7 +459,7 @@ __device__ void item_seed(const GraphDev& g, Slot* sp, uint32_t i) {
       app = (old == 0);
       vmin = v < vmin ? v : vmin;
     }
-    wave_append(app, v, sp, nxt, g, V);
+    wave_append(app, v, sp, nxt, A.rowptr, V);
   }
   if (k == 1) {
     // a member alone is an optimal tree; the smallest member id is the root
@@ -459,7 +480,7 @@ __device__ void item_clear(Slot* sp, uint32_t idx, uint64_t V) {
 
 // Stages the 64 frontier entries from list index i on: vertex, arc offset,
 // row start and dirty mask; entries past the end get offset arcs_total.
-__device__ __forceinline__ void stage_front(const GraphDev& g, FrontLDS* L, uint32_t i, uint32_t n,
+__device__ __forceinline__ void stage_front(const int64_t* rowptr, FrontLDS* L, uint32_t i, uint32_t n,
                                             uint32_t arcs_total, const uint32_t* lst, const uint32_t* lof,
                                             const uint64_t* mcur) {
   const int lane = lane_id();
@@ -468,7 +489,7 @@ __device__ __forceinline__ void stage_front(const GraphDev& g, FrontLDS* L, uint
   const uint32_t v = valid ? lst[idx] : 0u;
   L->wv[lane] = v;
   L->woff[lane] = valid ? lof[idx] : arcs_total;
-  L->wrs[lane] = valid ? (uint32_t)g.rowptr[v] : 0u;
+  L->wrs[lane] = valid ? (uint32_t)rowptr[v] : 0u;
   L->wm[lane] = valid ? mcur[v] : 0ull;
 }
 
@@ -484,6 +505,7 @@ __device__ __forceinline__ int find_entry(const FrontLDS* L, uint32_t pos) {
 // One chunk of arcs of the current frontier: grow T(v,p,d) -> T(u,p,d') for
 // every arc v->u in the chunk and every dirty subset p of v.
 __device__ void item_round(const GraphDev& g, Slot* sp, uint32_t chunk, FrontLDS* L) {
+  const Arcs A = arcs_of(g, sp);
   const int lane = lane_id();
   const uint64_t V = (uint64_t)g.V;
   const int k = sp->k, W = sp->W, D = sp->D;
@@ -513,7 +535,7 @@ __device__ void item_round(const GraphDev& g, Slot* sp, uint32_t chunk, FrontLDS
   uint32_t apn = 0;
   uint32_t a = a0;
   for (uint32_t i = sp->istart[(uint64_t)cur * sp->istride + chunk]; a < a1; i += 64) {
-    stage_front(g, L, i, n, arcs_total, lst, lof, mcur);
+    stage_front(A.rowptr, L, i, n, arcs_total, lst, lof, mcur);
     const uint32_t wend = (i + 64 < n) ? lof[i + 64] : arcs_total;
     wave_sync();
     const uint32_t hia = a1 < wend ? a1 : wend;
@@ -523,8 +545,8 @@ __device__ void item_round(const GraphDev& g, Slot* sp, uint32_t chunk, FrontLDS
     if (a + lane < hia) {
       no = find_entry(L, a + lane);
       ne = L->wrs[no] + (a + lane - L->woff[no]);
-      nu = (uint32_t)g.col[ne];
-      nw = (uint32_t)g.wt[ne];
+      nu = (uint32_t)A.col[ne];
+      nw = (uint32_t)A.wt[ne];
     }
     for (uint32_t base = a; base < hia; base += 64) {
       const uint32_t pos = base + lane;
@@ -535,8 +557,8 @@ __device__ void item_round(const GraphDev& g, Slot* sp, uint32_t chunk, FrontLDS
       if (pos + 64 < hia) {
         no = find_entry(L, pos + 64);
         ne = L->wrs[no] + (pos + 64 - L->woff[no]);
-        nu = (uint32_t)g.col[ne];
-        nw = (uint32_t)g.wt[ne];
+        nu = (uint32_t)A.col[ne];
+        nw = (uint32_t)A.wt[ne];
       }
       if (pos < hia) {
         const uint32_t vs = L->wv[o];
@@ -574,7 +596,7 @@ __device__ void item_round(const GraphDev& g, Slot* sp, uint32_t chunk, FrontLDS
             g_u64* tp = S.at(u, p, td);
             const uint32_t tc = (uint32_t)(*tp >> 32);
             if (cand >= tc) continue;
-            if (revarc == 0xFFFFFFFFu) revarc = (uint32_t)g.rev[e];
+            if (revarc == 0xFFFFFFFFu) revarc = (uint32_t)A.rev[e];
             const unsigned long long nv = ((unsigned long long)cand << 32) | (kPredGrow | revarc);
             const unsigned long long old = __hip_atomic_fetch_min(tp, nv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if ((uint32_t)(old >> 32) > cand) {
@@ -601,12 +623,12 @@ __device__ void item_round(const GraphDev& g, Slot* sp, uint32_t chunk, FrontLDS
         held = improved != 0;
       }
       if (level >= 2) mark_hold(held, u, sp);
-      stage_append(app, u, L, apn, sp, nxt, g, V);
+      stage_append(app, u, L, apn, sp, nxt, A.rowptr, V);
     }
     a = hia;
     wave_sync();
   }
-  flush_append(L, apn, sp, nxt, g, V);
+  flush_append(L, apn, sp, nxt, A.rowptr, V);
   n_relax = wave_sum_u64(n_relax);
   n_impr = wave_sum_u64(n_impr);
   if (lane == 0) {
@@ -622,6 +644,7 @@ __device__ void item_round(const GraphDev& g, Slot* sp, uint32_t chunk, FrontLDS
 // chunk bit is only ever set by the other labels of its chunk (relaxing its
 // exact distances again never lowers them).
 __device__ void item_jseed(const GraphDev& g, Slot* sp, uint32_t l) {
+  const Arcs A = arcs_of(g, sp);
   if (sp->ccol && sp->ccol[l]) return;
   const int nxt = (sp->step + 1) % 3;
   const uint64_t V = (uint64_t)g.V;
@@ -639,7 +662,7 @@ __device__ void item_jseed(const GraphDev& g, Slot* sp, uint32_t l) {
       const unsigned long long old = atomicOr((unsigned long long*)&mn[v], 1ull << (l / kJobChunkLabels));
       app = (old == 0);
     }
-    wave_append(app, v, sp, nxt, g, V);
+    wave_append(app, v, sp, nxt, A.rowptr, V);
   }
 }
 
@@ -670,6 +693,7 @@ __device__ __forceinline__ bool jrelax(uint32_t s, uint32_t t, uint32_t w, g_u32
 // the distances within level hops, and a round's frontier is only the chunks
 // that the previous height improved.
 __device__ void item_jround(const GraphDev& g, Slot* sp, uint32_t item, FrontLDS* L) {
+  const Arcs A = arcs_of(g, sp);
   const int lane = lane_id();
   const uint64_t V = (uint64_t)g.V;
   const uint32_t r = sp->step;
@@ -697,7 +721,7 @@ __device__ void item_jround(const GraphDev& g, Slot* sp, uint32_t item, FrontLDS
   uint32_t n_at = 0, apn = 0;
   uint32_t a = a0;
   for (uint32_t i = sp->istart[(uint64_t)cur * sp->istride + item]; a < a1; i += 64) {
-    stage_front(g, L, i, n, arcs_total, lst, lof, mcur);
+    stage_front(A.rowptr, L, i, n, arcs_total, lst, lof, mcur);
     const uint32_t wend = (i + 64 < n) ? lof[i + 64] : arcs_total;
     wave_sync();
     const uint32_t hia = a1 < wend ? a1 : wend;
@@ -707,8 +731,8 @@ __device__ void item_jround(const GraphDev& g, Slot* sp, uint32_t item, FrontLDS
     if (a + (uint32_t)sub < hia) {
       no = find_entry(L, a + (uint32_t)sub);
       const uint32_t e = L->wrs[no] + (a + (uint32_t)sub - L->woff[no]);
-      nu = (uint32_t)g.col[e];
-      nw = (uint32_t)g.wt[e];
+      nu = (uint32_t)A.col[e];
+      nw = (uint32_t)A.wt[e];
     }
     for (uint32_t base = a; base < hia; base += (uint32_t)APW) {
       const uint32_t pos = base + (uint32_t)sub;
@@ -719,8 +743,8 @@ __device__ void item_jround(const GraphDev& g, Slot* sp, uint32_t item, FrontLDS
       if (pos + (uint32_t)APW < hia) {
         no = find_entry(L, pos + (uint32_t)APW);
         const uint32_t e = L->wrs[no] + (pos + (uint32_t)APW - L->woff[no]);
-        nu = (uint32_t)g.col[e];
-        nw = (uint32_t)g.wt[e];
+        nu = (uint32_t)A.col[e];
+        nw = (uint32_t)A.wt[e];
       }
       if (pos < hia) {
         const uint32_t vs = L->wv[o];
@@ -746,12 +770,12 @@ __device__ void item_jround(const GraphDev& g, Slot* sp, uint32_t item, FrontLDS
       // the improved chunks of one arc, gathered on the group's first lane
       for (int o2 = 1; o2 < G; o2 <<= 1) imp |= (uint64_t)__shfl_xor((unsigned long long)imp, o2, 64);
       const bool app = (pos < hia && rr == 0 && imp) ? mark_dirty(mnxt, u, imp) : false;
-      stage_append(app, u, L, apn, sp, nxt, g, V);
+      stage_append(app, u, L, apn, sp, nxt, A.rowptr, V);
     }
     a = hia;
     wave_sync();
   }
-  flush_append(L, apn, sp, nxt, g, V);
+  flush_append(L, apn, sp, nxt, A.rowptr, V);
   n_relax = wave_sum_u64(n_relax);
   n_impr = wave_sum_u64(n_impr);
   const uint64_t n_atom = wave_sum_u64(n_at);
@@ -1080,6 +1104,7 @@ __device__ void item_live(Slot* sp, uint32_t tile, uint64_t V, WaveLDS* L) {
 // vertex, subset) tuples of the staged vertices in turn, height-major so that
 // neighbouring lanes run the same height-pair loops.
 __device__ void item_merge(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* L) {
+  const Arcs A = arcs_of(g, sp);
   const int lane = lane_id();
   const uint64_t V = (uint64_t)g.V;
   const int k = sp->k, W = sp->W, ND = sp->ND, NS = sp->NS, D = sp->D;
@@ -1233,7 +1258,7 @@ __device__ void item_merge(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* 
       }
     }
     if (level == 2) mark_hold(app, v, sp);  // above level 2 the vertex is active already
-    wave_append(app, v, sp, nxt, g, V);
+    wave_append(app, v, sp, nxt, A.rowptr, V);
     wave_sync();
   }
   ubmin = wave_min_u32(ubmin);
@@ -1346,6 +1371,7 @@ __device__ void item_final(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* 
 // J[u][i] + w == J[v][i] (for a hop job, J at height d - 1 against height d),
 // found by a wave-wide scan of v's row (it exists because the rows are exact).
 __device__ void item_recon(const GraphDev& g, Slot* sp) {
+  const Arcs A = arcs_of(g, sp);
   const int lane = lane_id();
   const int W = sp->W, NS = sp->NS, D = sp->D;
   const States S = states_of(sp);
@@ -1426,14 +1452,14 @@ __device__ void item_recon(const GraphDev& g, Slot* sp) {
       const uint32_t li = (uint32_t)(__ffs(__shfl(p, l, 64)) - 1);
       const uint32_t lc = __shfl(sc, l, 64);
       const uint32_t ld = __shfl(d, l, 64);
-      const uint32_t e0 = (uint32_t)g.rowptr[lv], e1 = (uint32_t)g.rowptr[lv + 1];
+      const uint32_t e0 = (uint32_t)A.rowptr[lv], e1 = (uint32_t)A.rowptr[lv + 1];
       uint32_t found = 0xFFFFFFFFu;
       for (uint32_t b = e0; b < e1; b += 64) {
         const uint32_t e = b + lane;
         bool ok = false;
         if (e < e1) {
-          const uint32_t u = (uint32_t)g.col[e];
-          ok = S.jcost(u, li, hop ? ld - 1u : 0u) + (uint32_t)g.wt[e] == lc;
+          const uint32_t u = (uint32_t)A.col[e];
+          ok = S.jcost(u, li, hop ? ld - 1u : 0u) + (uint32_t)A.wt[e] == lc;
         }
         const uint64_t hb = __ballot(ok);
         if (hb) {
@@ -1462,8 +1488,8 @@ __device__ void item_recon(const GraphDev& g, Slot* sp) {
     int32_t c1a = att;
     if (isg) {
       const uint32_t ea = pred & kArcMask;
-      const uint32_t u = (uint32_t)g.col[ea];
-      const int32_t w = g.wt[ea];
+      const uint32_t u = (uint32_t)A.col[ea];
+      const int32_t w = A.wt[ea];
       if (eidx < (uint32_t)sp->edge_cap) {
         EdgeDev ed;
         ed.u = (int32_t)v;
@@ -1938,7 +1964,126 @@ __device__ void step_body(const GraphDev& g, Slot* slots, Ctrl* ctrl, LDS* L) {
   }
 }
 
+// ------------------------------------------------- shortest-path backbone ----
+// An arc (u, v, w) with a u-v path shorter than w lies on no shortest path and
+// on no optimal group Steiner tree: replacing it by the shorter path, then
+// breaking the cycles that closes, would leave a cheaper connected subgraph
+// over the same vertices.  So the unbounded DP and the group->vertex distances
+// give the same answers on the remaining arcs (the backbone).  A hop bound can
+// need such an arc (the only route within D hops may be a heavy direct one),
+// so hop-bounded queries keep the whole graph.
+// Redundancy is proved through the exact distances d(h, .) from kBbHubs hub
+// vertices: d(h, u) + d(h, v) is the length of a real u-v walk, so
+// min_h d(h, u) + d(h, v) < w proves the arc redundant; an arc no hub proves
+// redundant is kept.  The hub distances come from label-correcting rounds over
+// all arcs, where a source whose row changed in the round before relaxes its
+// row into its targets.
+typedef uint32_t bchunk __attribute__((vector_size(16)));
+
+// the row holding arc e < n_arcs: rowptr[r] <= e < rowptr[r + 1]
+__device__ __forceinline__ uint64_t bb_row_of(const int64_t* rowptr, uint64_t V, uint64_t e) {
+  uint64_t lo = 0, hi = V;
+  while (hi - lo > 1) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if ((uint64_t)rowptr[mid] <= e) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// One hub round over the arcs of one item (kBbArcs consecutive arcs): 16
+// lanes per arc, four arcs at a time; lane r of a group relaxes hub labels
+// 4r .. 4r+3 of the source row into the target row.
+__device__ void bb_round_item(const BbArgs& a, uint64_t item) {
+  const int r = lane_id() & 15;
+  const uint64_t a0 = item * kBbArcs;
+  const uint64_t a1 = a0 + kBbArcs < a.n_arcs ? a0 + kBbArcs : a.n_arcs;
+  uint64_t e = a0 + (uint64_t)(lane_id() >> 4);
+  uint64_t s = e < a1 ? bb_row_of(a.rowptr, a.V, e) : 0;
+  bool any = false;
+  for (; e < a1; e += 4) {
+    while ((uint64_t)a.rowptr[s + 1] <= e) s++;
+    if (!a.dcur[s]) continue;
+    const uint64_t v = (uint64_t)a.col[e];
+    const uint32_t w = (uint32_t)a.wt[e];
+    const bchunk su = ((const bchunk*)(a.dist + s * kBbHubs))[r];
+    const bchunk tv = ((const bchunk*)(a.dist + v * kBbHubs))[r];
+    bool im = false;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      if (su[q] >= kInf) continue;
+      const uint32_t cand = su[q] + w;
+      if (cand >= kInf || cand >= tv[q]) continue;
+      atomicMin(a.dist + v * kBbHubs + 4 * r + q, cand);
+      im = true;
+    }
+    if (im) {
+      a.dnxt[v] = 1;
+      any = true;
+    }
+  }
+  if (any) *a.changed = 1;
+}
+
+// Arc test of one item: 16 lanes per arc, each the least d(h, u) + d(h, v)
+// over its four hubs, then the least over the group against the weight.
+__device__ void bb_test_item(const BbArgs& a, uint64_t item) {
+  const int r = lane_id() & 15;
+  const uint64_t a0 = item * kBbArcs;
+  const uint64_t a1 = a0 + kBbArcs < a.n_arcs ? a0 + kBbArcs : a.n_arcs;
+  const uint64_t e0 = a0 + (uint64_t)(lane_id() >> 4);
+  uint64_t s = e0 < a1 ? bb_row_of(a.rowptr, a.V, e0) : 0;
+  for (uint64_t base = a0; base < a1; base += 4) {
+    const uint64_t e = base + (uint64_t)(lane_id() >> 4);
+    const bool valid = e < a1;
+    uint32_t m = 0xFFFFFFFFu, w = 0;
+    if (valid) {
+      while ((uint64_t)a.rowptr[s + 1] <= e) s++;
+      const uint64_t v = (uint64_t)a.col[e];
+      w = (uint32_t)a.wt[e];
+      const bchunk su = ((const bchunk*)(a.dist + s * kBbHubs))[r];
+      const bchunk tv = ((const bchunk*)(a.dist + v * kBbHubs))[r];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const uint32_t x = su[q] + tv[q];  // both <= kInf: no overflow
+        m = x < m ? x : m;
+      }
+    }
+#pragma unroll
+    for (int o = 8; o > 0; o >>= 1) {
+      const uint32_t y = __shfl_xor(m, o, 64);
+      m = y < m ? y : m;
+    }
+    if (valid && r == 0) a.keep[e] = m < w ? 0 : 1;
+  }
+}
+
 #ifndef GST_CPU_SIM
+// Backbone construction (once per graph): a fill, the hub seeds, then the
+// hub rounds and the arc test as grid-stride loops over items, one wave per item.
+__global__ __launch_bounds__(256) void gst_bb_fill_kernel(uint32_t* p, uint64_t n, uint32_t val) {
+  const uint64_t stride = (uint64_t)gridDim.x * 256u;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x; i < n; i += stride) p[i] = val;
+}
+
+__global__ __launch_bounds__(64) void gst_bb_seed_kernel(uint32_t* dist, uint8_t* flags, const int32_t* hubs,
+                                                         int H) {
+  const int t = (int)threadIdx.x;
+  if (t < H) {
+    dist[(uint64_t)hubs[t] * kBbHubs + t] = 0u;
+    flags[hubs[t]] = 1;
+  }
+}
+
+__global__ __launch_bounds__(256) void gst_bb_round_kernel(BbArgs a, uint64_t n_items) {
+  const uint64_t nw = (uint64_t)gridDim.x * 4u;
+  for (uint64_t it = (uint64_t)blockIdx.x * 4u + (threadIdx.x >> 6); it < n_items; it += nw) bb_round_item(a, it);
+}
+
+__global__ __launch_bounds__(256) void gst_bb_test_kernel(BbArgs a, uint64_t n_items) {
+  const uint64_t nw = (uint64_t)gridDim.x * 4u;
+  for (uint64_t it = (uint64_t)blockIdx.x * 4u + (threadIdx.x >> 6); it < n_items; it += nw) bb_test_item(a, it);
+}
+
 // One engine step of each class: every wave pulls items until the class's
 // plan is exhausted.  There is no waiting between blocks; the step ends when
 // the kernels end, and the next plan kernel sees everything they wrote.
@@ -2028,6 +2173,41 @@ extern "C" hipError_t gst_step_occupancy(int cls, int wpb, int* blocks_per_cu) {
     default: return hipErrorInvalidValue;
   }
   return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, f, wpb * 64, 0);
+}
+
+namespace {
+int bb_grid(uint64_t waves) {
+  const uint64_t b = (waves + 3) / 4;
+  return (int)(b < 1 ? 1 : (b > 8192 ? 8192 : b));
+}
+}  // namespace
+
+// Backbone construction: n words of p set to val.
+extern "C" hipError_t gst_bb_fill(uint32_t* p, uint64_t n, uint32_t val, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  const uint64_t b = (n + 255) / 256;
+  hipLaunchKernelGGL(gst::gst_bb_fill_kernel, dim3((unsigned)(b > 8192 ? 8192 : b)), dim3(256), 0, stream, p, n, val);
+  return hipGetLastError();
+}
+
+// Hub l (l < H <= kBbHubs) at distance 0 from itself, its row marked changed.
+extern "C" hipError_t gst_bb_seed(uint32_t* dist, uint8_t* flags, const int32_t* hubs, int H, hipStream_t stream) {
+  if (H < 1 || H > gst::kBbHubs) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(gst::gst_bb_seed_kernel, dim3(1), dim3(64), 0, stream, dist, flags, hubs, H);
+  return hipGetLastError();
+}
+
+// what: 0 = one hub round, 1 = the arc test
+extern "C" hipError_t gst_bb_pass(int what, const gst::BbArgs* a, hipStream_t stream) {
+  const uint64_t items = (a->n_arcs + gst::kBbArcs - 1) / gst::kBbArcs;
+  if (items == 0) return hipSuccess;
+  if (what == 0)
+    hipLaunchKernelGGL(gst::gst_bb_round_kernel, dim3(bb_grid(items)), dim3(256), 0, stream, *a, items);
+  else if (what == 1)
+    hipLaunchKernelGGL(gst::gst_bb_test_kernel, dim3(bb_grid(items)), dim3(256), 0, stream, *a, items);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
 }
 #endif  // GST_CPU_SIM
 

@@ -30,6 +30,9 @@ extern "C" hipError_t gst_launch_plan(gst::Slot* slots, gst::Ctrl* ctrl, uint64_
 extern "C" hipError_t gst_step_occupancy(int cls, int wpb, int* blocks_per_cu);
 extern "C" hipError_t gst_scratch_setup(int device, void** ptr);
 extern "C" hipError_t gst_launch_count(gst::Slot* slot, uint64_t V, hipStream_t stream);
+extern "C" hipError_t gst_bb_fill(uint32_t* p, uint64_t n, uint32_t val, hipStream_t stream);
+extern "C" hipError_t gst_bb_seed(uint32_t* dist, uint8_t* flags, const int32_t* hubs, int H, hipStream_t stream);
+extern "C" hipError_t gst_bb_pass(int what, const gst::BbArgs* a, hipStream_t stream);
 
 namespace {
 
@@ -134,6 +137,7 @@ struct Options {
   int64_t merge_tv = gst::kMergeTV;  // live vertices per MERGE / FINAL work item
   int64_t job_lanes = 0;  // lanes sharing an arc in job rounds (power of two; 0: the smallest with <= 8 chunks each)
   int64_t count_states = 0;  // count the finite DP states of every query (an extra pass)
+  int64_t backbone = 1;  // unbounded queries and jobs admitted from now on run on the shortest-path backbone
 };
 
 }  // namespace
@@ -153,6 +157,12 @@ struct gst_graph {
   int32_t* d_col = nullptr;
   int32_t* d_wt = nullptr;
   int32_t* d_rev = nullptr;
+  // shortest-path backbone: its arcs follow the graph's in d_col / d_wt /
+  // d_rev and its row starts follow the graph's in d_rowptr (has_bb false
+  // when the graph has no arcs or its build did not fit)
+  bool has_bb = false;
+  int64_t bb_arcs = 0, bb_rounds = 0;
+  double bb_ms = 0;
   int64_t* d_gptr = nullptr;
   int32_t* d_gmem = nullptr;
   Options opt;
@@ -197,6 +207,10 @@ uint32_t istride_for(const gst_graph* g, int64_t chunk) {
 // True when the query can take its level 1 from a group-distance job (k >= 2;
 // k = 1 is answered at SEED).  A job serves queries of one hop bound.
 bool uses_job(const gst_graph* g, int k, int D) { return g->opt.group_pass != 0 && k >= 2; }
+
+// True when a query or job of hop bound D runs on the shortest-path backbone:
+// unbounded only (a hop bound can need an arc that no shortest path uses).
+bool uses_backbone(const gst_graph* g, int D) { return D < 0 && g->opt.backbone != 0 && g->has_bb; }
 
 // Device memory of one query; `own` = it computes its own level 1 (S1 and C
 // tables); a job-fed query reads both from its job's rows.
@@ -516,6 +530,7 @@ class Engine {
     d.W = 1 << k;
     d.ND = q.D < 0 ? 1 : q.D + 1;
     d.D = q.D;
+    d.bb = uses_backbone(g_, q.D) ? 1u : 0u;
     for (int i = 0; i < k; i++) d.qg[i] = q.groups[i];
     d.edge_cap = L.edge_cap;
     d.chunk = (int32_t)g_->opt.chunk;
@@ -621,6 +636,7 @@ class Engine {
     d.job = -1;
     d.D = D;
     d.ND = D < 0 ? 1 : D + 1;
+    d.bb = uses_backbone(g_, D) ? 1u : 0u;
     d.L = L;
     d.Lp = J.Lp;
     d.G = J.G;
@@ -1345,6 +1361,184 @@ size_t static_arena_bytes() {
   return gst::kScratchBytes - align256(2 * sizeof(gst::Slot)) - align256(sizeof(gst::Ctrl));
 }
 
+// Shortest-path backbone of a new graph: the arcs that the hub distances do
+// not prove redundant (see the backbone kernels), in the same row order, with
+// their own reverse-arc table.  Built once; unbounded queries and jobs run on
+// it (option backbone).  Device buffers of the build are released before it
+// returns; when they do not fit, the graph simply has no backbone.
+gst_status build_backbone(gst_graph* g, const int64_t* beg_pos, const std::vector<int32_t>& col,
+                          const std::vector<int32_t>& wt, const std::vector<int32_t>& rev) {
+  const double t0 = now_ms();
+  const int64_t V = g->V, E = g->n_arcs;
+  if (E == 0) return GST_OK;
+  // hubs: the vertices of highest degree (ties: lower id)
+  const int H = (int)std::min<int64_t>(gst::kBbHubs, V);
+  std::vector<int32_t> hubs(V);
+  for (int64_t v = 0; v < V; v++) hubs[v] = (int32_t)v;
+  std::partial_sort(hubs.begin(), hubs.begin() + H, hubs.end(), [&](int32_t a, int32_t b) {
+    const int64_t da = beg_pos[a + 1] - beg_pos[a], db = beg_pos[b + 1] - beg_pos[b];
+    return da != db ? da > db : a < b;
+  });
+  hubs.resize(H);
+  const size_t Vp = (size_t)(V + 3) & ~size_t(3), Ep = (size_t)(E + 3) & ~size_t(3);
+  uint32_t* dist = nullptr;
+  uint8_t* flags = nullptr;
+  uint8_t* keep = nullptr;
+  uint32_t* changed = nullptr;
+  int32_t* dhubs = nullptr;
+  hipStream_t st = nullptr;
+  auto release = [&]() {
+    if (st) (void)hipStreamDestroy(st);
+    (void)hipFree(dist);
+    (void)hipFree(flags);
+    (void)hipFree(keep);
+    (void)hipFree(changed);
+    (void)hipFree(dhubs);
+  };
+  if (hipMalloc(&dist, (size_t)V * gst::kBbHubs * 4) != hipSuccess || hipMalloc(&flags, 2 * Vp) != hipSuccess ||
+      hipMalloc(&keep, Ep) != hipSuccess || hipMalloc(&changed, 256) != hipSuccess ||
+      hipMalloc(&dhubs, sizeof(int32_t) * gst::kBbHubs) != hipSuccess) {
+    release();
+    (void)hipGetLastError();
+    return GST_OK;  // no room for the build: the graph runs without a backbone
+  }
+  std::vector<uint8_t> hk(E);
+  int64_t rounds = 0;
+  gst_status err = [&]() -> gst_status {
+    HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    HIP_TRY(gst_bb_fill(dist, (uint64_t)V * gst::kBbHubs, gst::kInf, st));
+    HIP_TRY(gst_bb_fill((uint32_t*)flags, 2 * Vp / 4, 0u, st));
+    HIP_TRY(hipMemcpyAsync(dhubs, hubs.data(), sizeof(int32_t) * H, hipMemcpyHostToDevice, st));
+    uint8_t* cur = flags;
+    uint8_t* nxt = flags + Vp;
+    HIP_TRY(gst_bb_seed(dist, cur, dhubs, H, st));
+    gst::BbArgs a;
+    a.rowptr = g->d_rowptr;
+    a.col = g->d_col;
+    a.wt = g->d_wt;
+    a.V = (uint64_t)V;
+    a.n_arcs = (uint64_t)E;
+    a.dist = dist;
+    a.changed = changed;
+    a.keep = keep;
+    // label-correcting rounds until no row changes.  A vertex whose shortest
+    // path from a hub has i arcs holds its distance after round i (its
+    // predecessor held its own after round i - 1, changed then and relaxed in
+    // round i), so at most V + 1 rounds: the hop depth of the hubs' trees + 1
+    for (;;) {
+      if (rounds > V + 1) return fail(GST_DEVICE_ERROR, "backbone hub rounds did not converge");
+      HIP_TRY(gst_bb_fill(changed, 1, 0u, st));
+      a.dcur = cur;
+      a.dnxt = nxt;
+      HIP_TRY(gst_bb_pass(0, &a, st));
+      uint32_t ch = 0;
+      HIP_TRY(hipMemcpyAsync(&ch, changed, sizeof(ch), hipMemcpyDeviceToHost, st));
+      HIP_TRY(hipStreamSynchronize(st));
+      rounds++;
+      if (!ch) break;
+      HIP_TRY(gst_bb_fill((uint32_t*)cur, Vp / 4, 0u, st));
+      std::swap(cur, nxt);
+    }
+    HIP_TRY(gst_bb_pass(1, &a, st));
+    HIP_TRY(hipMemcpyAsync(hk.data(), keep, (size_t)E, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return GST_OK;
+  }();
+  release();
+  if (err != GST_OK) return err;
+  // the test is symmetric (d(h,u) + d(h,v), and reverse arcs have equal
+  // weights), so an arc and its reverse are kept together
+  std::atomic<int64_t> asym{-1};
+  parallel_for(E, [&](int64_t b, int64_t e) {
+    for (int64_t i = b; i < e; i++)
+      if (hk[i] != hk[rev[i]]) asym = i;
+  });
+  if (asym >= 0)
+    return fail(GST_DEVICE_ERROR, "backbone test kept arc %lld but not its reverse", (long long)asym.load());
+  std::vector<int64_t> brow(V + 1, 0);
+  parallel_for_rows(beg_pos, V, [&](int64_t b, int64_t e) {
+    for (int64_t u = b; u < e; u++) {
+      int64_t c = 0;
+      for (int64_t i = beg_pos[u]; i < beg_pos[u + 1]; i++) c += hk[i];
+      brow[u + 1] = c;
+    }
+  });
+  for (int64_t u = 0; u < V; u++) brow[u + 1] += brow[u];
+  const int64_t nb = brow[V];
+  // arc indices of the backbone run on from E in the shared arrays, and a
+  // grow predecessor holds one in 30 bits
+  if (E + nb >= (int64_t(1) << 30)) return GST_OK;
+  std::vector<int32_t> nidx(E, -1), acol(E + nb), awt(E + nb), arev(E + nb);
+  std::vector<int64_t> arow(2 * (V + 1));
+  std::copy(col.begin(), col.end(), acol.begin());
+  std::copy(wt.begin(), wt.end(), awt.begin());
+  std::copy(rev.begin(), rev.end(), arev.begin());
+  for (int64_t v = 0; v <= V; v++) {
+    arow[v] = beg_pos[v];
+    arow[V + 1 + v] = E + brow[v];
+  }
+  parallel_for_rows(beg_pos, V, [&](int64_t b, int64_t e) {
+    for (int64_t u = b; u < e; u++) {
+      int64_t j = E + brow[u];
+      for (int64_t i = beg_pos[u]; i < beg_pos[u + 1]; i++) {
+        if (!hk[i]) continue;
+        nidx[i] = (int32_t)j;
+        acol[j] = col[i];
+        awt[j] = wt[i];
+        j++;
+      }
+    }
+  });
+  parallel_for(E, [&](int64_t b, int64_t e) {
+    for (int64_t i = b; i < e; i++)
+      if (hk[i]) arev[nidx[i]] = nidx[rev[i]];
+  });
+  int64_t* d_rowptr = nullptr;
+  int32_t *d_col = nullptr, *d_wt = nullptr, *d_rev = nullptr;
+  auto drop = [&]() {
+    (void)hipFree(d_rowptr);
+    (void)hipFree(d_col);
+    (void)hipFree(d_wt);
+    (void)hipFree(d_rev);
+    (void)hipGetLastError();
+  };
+  if (hipMalloc(&d_rowptr, sizeof(int64_t) * arow.size()) != hipSuccess ||
+      hipMalloc(&d_col, sizeof(int32_t) * acol.size()) != hipSuccess ||
+      hipMalloc(&d_wt, sizeof(int32_t) * awt.size()) != hipSuccess ||
+      hipMalloc(&d_rev, sizeof(int32_t) * arev.size()) != hipSuccess) {
+    drop();
+    return GST_OK;  // no room: the graph runs without a backbone
+  }
+  gst_status cst = [&]() -> gst_status {
+    HIP_TRY(hipMemcpy(d_rowptr, arow.data(), sizeof(int64_t) * arow.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d_col, acol.data(), sizeof(int32_t) * acol.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d_wt, awt.data(), sizeof(int32_t) * awt.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d_rev, arev.data(), sizeof(int32_t) * arev.size(), hipMemcpyHostToDevice));
+    return GST_OK;
+  }();
+  if (cst != GST_OK) {
+    drop();
+    return cst;
+  }
+  for (void* p : {(void*)g->d_rowptr, (void*)g->d_col, (void*)g->d_wt, (void*)g->d_rev}) (void)hipFree(p);
+  g->d_rowptr = d_rowptr;
+  g->d_col = d_col;
+  g->d_wt = d_wt;
+  g->d_rev = d_rev;
+  g->has_bb = true;
+  g->bb_arcs = nb;
+  g->bb_rounds = rounds;
+  g->bb_ms = now_ms() - t0;
+  return GST_OK;
+}
+
+// Device memory of a graph (its engines are gone already).
+void free_graph_memory(gst_graph* g) {
+  for (void* p : {(void*)g->d_rowptr, (void*)g->d_col, (void*)g->d_wt, (void*)g->d_rev, (void*)g->d_gptr,
+                  (void*)g->d_gmem})
+    if (p) (void)hipFree(p);
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------- C ABI
@@ -1446,7 +1640,27 @@ gst_status gst_graph_create(const int64_t* beg_pos, int64_t V, const int32_t* ad
     HIP_TRY(hipMemcpy(g->d_wt, wt.data(), sizeof(int32_t) * n_arcs, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(g->d_rev, rev.data(), sizeof(int32_t) * n_arcs, hipMemcpyHostToDevice));
   }
+  gst_status bst = build_backbone(g.get(), beg_pos, col, wt, rev);
+  if (bst != GST_OK) {
+    free_graph_memory(g.get());
+    return bst;
+  }
   *out = g.release();
+  return GST_OK;
+}
+
+gst_status gst_backbone_get(gst_graph* g, int64_t* beg_pos_out, int32_t* adj_out, int32_t* w_out,
+                            int64_t* n_arcs_out) {
+  if (!g || !beg_pos_out || !n_arcs_out) return fail(GST_INVALID_ARG, "null argument");
+  const bool bb = g->has_bb;
+  const int64_t n = bb ? g->bb_arcs : g->n_arcs, first = bb ? g->n_arcs : 0;
+  std::lock_guard<std::mutex> lk(g->dev_mu);
+  HIP_TRY(hipMemcpy(beg_pos_out, g->d_rowptr + (bb ? g->V + 1 : 0), sizeof(int64_t) * (g->V + 1),
+                    hipMemcpyDeviceToHost));
+  for (int64_t v = 0; v <= g->V; v++) beg_pos_out[v] -= first;
+  if (adj_out && n) HIP_TRY(hipMemcpy(adj_out, g->d_col + first, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+  if (w_out && n) HIP_TRY(hipMemcpy(w_out, g->d_wt + first, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+  *n_arcs_out = n;
   return GST_OK;
 }
 
@@ -1457,12 +1671,7 @@ gst_status gst_graph_destroy(gst_graph* g) {
     g->plain.reset();
     g->batch.reset();
   }
-  (void)hipFree(g->d_rowptr);
-  (void)hipFree(g->d_col);
-  (void)hipFree(g->d_wt);
-  (void)hipFree(g->d_rev);
-  if (g->d_gptr) (void)hipFree(g->d_gptr);
-  if (g->d_gmem) (void)hipFree(g->d_gmem);
+  free_graph_memory(g);
   delete g;
   return GST_OK;
 }
@@ -1512,6 +1721,7 @@ static int64_t* option_ref(gst_graph* g, const std::string& k) {
   if (k == "job_lanes") return &g->opt.job_lanes;
   if (k == "approx_num") return &g->opt.approx_num;
   if (k == "approx_den") return &g->opt.approx_den;
+  if (k == "backbone") return &g->opt.backbone;
   return nullptr;
 }
 
@@ -1531,7 +1741,7 @@ gst_status gst_set_option(gst_graph* g, const char* key, int64_t value) {
   if (k == "chunk" && (value < 64 || value > (1 << 20))) return fail(GST_INVALID_ARG, "chunk outside [64, 2^20]");
   if (k == "edge_cap" && (value < 64 || value > (1 << 26))) return fail(GST_INVALID_ARG, "edge_cap outside [64, 2^26]");
   if ((k == "budget_mb" || k == "blocks_per_cu" || k == "timeout_ms" || k == "trace_slot" || k == "deadline_ms" ||
-       k == "count_states" || k == "cache_mb") && value < 0)
+       k == "count_states" || k == "cache_mb" || k == "backbone") && value < 0)
     return fail(GST_INVALID_ARG, "negative value");
   if (k == "block" && value != 2 && value != 4 && value != 8)
     return fail(GST_INVALID_ARG, "block (waves per block) must be 2, 4 or 8");
@@ -1552,6 +1762,9 @@ gst_status gst_get_option(gst_graph* g, const char* key, int64_t* value) {
   if (!g || !key || !value) return fail(GST_INVALID_ARG, "null argument");
   const std::string k(key);
   if (k == "num_cu") { *value = g->num_cu; return GST_OK; }
+  if (k == "backbone_arcs") { *value = g->has_bb ? g->bb_arcs : g->n_arcs; return GST_OK; }
+  if (k == "backbone_rounds") { *value = g->bb_rounds; return GST_OK; }
+  if (k == "backbone_us") { *value = (int64_t)(g->bb_ms * 1e3); return GST_OK; }
   if (k.rfind("plain_", 0) == 0 || k.rfind("batch_", 0) == 0) {
     Engine* e = (k[0] == 'p') ? g->plain.get() : g->batch.get();
     const std::string f = k.substr(6);
@@ -1836,6 +2049,7 @@ void server_loop(gst_server* s) {
     if (!fin.empty()) s->cv_done.notify_all();
   }
 }
+
 
 }  // namespace
 

@@ -123,11 +123,26 @@ typedef struct {
  * weights of n_arcs arcs; undirected, both arcs present with equal weights
  * (code/GPUGST+/include/graph.hpp:12-118 loads the same three arrays from
  * X_beg_pos.bin / X_csr.bin / X_weight.bin).  Weights must be in [1, 2^20].
- * The library copies the arrays to the device and keeps them resident. */
+ * The library copies the arrays to the device and keeps them resident.
+ * It also builds the graph's shortest-path backbone once, on the device: the
+ * arcs (u, v, w) that no u-v path shorter than w is known to replace (exact
+ * distances from the 64 vertices of highest degree prove the others
+ * redundant).  No shortest path and no optimal group Steiner tree uses a
+ * redundant arc, so queries without a hop bound run on the backbone with the
+ * same costs (option "backbone"); hop-bounded queries use every arc.  The
+ * reference runs every query on the full CSR (code/GPUGST+/src/
+ * GSTnonHop.cu:168-180). */
 gst_status gst_graph_create(const int64_t *beg_pos, int64_t V,
                             const int32_t *adj, const int32_t *w,
                             int64_t n_arcs, gst_graph **out);
 gst_status gst_graph_destroy(gst_graph *g);
+
+/* Copies out the backbone: beg_pos_out[V+1] and, when non-null, adj_out /
+ * w_out of *n_arcs_out arcs (the count alone when they are null).  A graph
+ * without a backbone (no arcs, or no device memory for its build) reports
+ * every arc.  No counterpart in the reference (it has no backbone). */
+gst_status gst_backbone_get(gst_graph *g, int64_t *beg_pos_out, int32_t *adj_out,
+                            int32_t *w_out, int64_t *n_arcs_out);
 
 /* Groups: group g's members are members[grp_ptr[g] .. grp_ptr[g+1]), as read
  * from the .g file line "g{g+1}:" (code/GPUGST/include/
@@ -188,8 +203,11 @@ void gst_result_free(gst_result *r);
  * unbounded queries admitted together run their own level 1 instead of a
  * job), "time_steps" (time every step's kernels),
  * "trace_slot" (with time_steps: the engine slot whose phase each step's
- * time is binned by; slot 0 is the query of a one-query call).
- * Read-only: "num_cu", "occupancy", "occupancy_heavy", "grid",
+ * time is binned by; slot 0 is the query of a one-query call),
+ * "backbone" (default 1: queries without a hop bound admitted from then on
+ * run on the graph's shortest-path backbone; 0: on every arc).
+ * Read-only: "num_cu", "backbone_arcs", "backbone_rounds" (hub rounds of
+ * its build), "backbone_us" (build time), "occupancy", "occupancy_heavy", "grid",
  * "grid_heavy", "{plain,batch}_{kernel_ns,launches,steps,lean_ns,heavy_ns,
  * timed_steps,jobs,job_entries,job_arcs,job_relaxations,job_improvements,
  * job_atomics,cache_hits,cache_misses,cache_inserts,cache_evictions,

@@ -110,6 +110,44 @@ extern "C" hipError_t gst_launch_count(gst::Slot* slot, uint64_t V, hipStream_t)
   return hipSuccess;
 }
 
+// Backbone construction: the fill and seed as host loops, the hub rounds and
+// the arc test as emulated waves taking items in a grid-stride loop.
+extern "C" hipError_t gst_bb_fill(uint32_t* p, uint64_t n, uint32_t val, hipStream_t) {
+  for (uint64_t i = 0; i < n; i++) p[i] = val;
+  return hipSuccess;
+}
+
+extern "C" hipError_t gst_bb_seed(uint32_t* dist, uint8_t* flags, const int32_t* hubs, int H, hipStream_t) {
+  if (H < 1 || H > gst::kBbHubs) return hipErrorInvalidValue;
+  for (int t = 0; t < H; t++) {
+    dist[(uint64_t)hubs[t] * gst::kBbHubs + t] = 0u;
+    flags[hubs[t]] = 1;
+  }
+  return hipSuccess;
+}
+
+extern "C" hipError_t gst_bb_pass(int what, const gst::BbArgs* a, hipStream_t) {
+  if (what != 0 && what != 1) return hipErrorInvalidValue;
+  const uint64_t items = (a->n_arcs + gst::kBbArcs - 1) / gst::kBbArcs;
+  if (items == 0) return hipSuccess;
+  const int nw = (int)(items < 4 ? items : 4);
+  std::vector<gsim::Wave> waves(nw);
+  std::vector<std::thread> th;
+  const gst::BbArgs args = *a;
+  for (int w = 0; w < nw; w++)
+    for (int l = 0; l < 64; l++)
+      th.emplace_back([&, w, l]() {
+        gsim::lane = l;
+        gsim::wave = &waves[w];
+        for (uint64_t it = (uint64_t)w; it < items; it += (uint64_t)nw) {
+          if (what == 0) gst::bb_round_item(args, it);
+          else gst::bb_test_item(args, it);
+        }
+      });
+  for (auto& x : th) x.join();
+  return hipSuccess;
+}
+
 extern "C" hipError_t gst_step_occupancy(int, int wpb, int* blocks_per_cu) {
   *blocks_per_cu = wpb >= 4 ? 1 : 4 / wpb;  // at least four emulated waves per launch
   return hipSuccess;

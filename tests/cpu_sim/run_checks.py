@@ -103,13 +103,16 @@ def check_jobs(V, nq, seed):
     g = Graph(rp, col, wt)
     g.set_groups(*groups_csr(groups))
     rs = g.solve_batch(qs)
+    g.set_option("backbone", 0)  # the same batch on every arc
+    rf = g.solve_batch(qs)
+    g.set_option("backbone", 1)
     g.set_option("group_pass", 0)
     r0 = g.solve_batch(qs)
     bad = []
     for i, q in enumerate(qs):
         gl = [groups[x] for x in q]
         exp = O.dpbf(rp, col, wt, gl)[0]
-        ok = rs[i].cost == exp and r0[i].cost == exp
+        ok = rs[i].cost == exp and r0[i].cost == exp and rf[i].cost == exp
         if ok and rs[i].feasible:
             ok = validate_tree(rp, col, wt, gl, rs[i].cost, rs[i].root, rs[i].edges)["ok"]
         if not ok:
@@ -167,6 +170,61 @@ def check_branching():
     ok = r.cost == exp and validate_tree(rp, col, wt, groups, r.cost, r.root, r.edges)["ok"]
     g.close()
     return {"cost": r.cost, "expect": exp, "ok": bool(ok)}
+
+
+def check_backbone():
+    """The graph's shortest-path backbone against the exact one (arcs with
+    w == dist(u, v), from all-pairs distances): equal while every vertex is a
+    hub (V <= 64), a superset beyond that; an arc and its reverse together.
+    On two of the graphs, queries without a hop bound (one call each, batched,
+    and batched on every arc) and one with D = 3 equal the oracle."""
+    from scipy.sparse import csr_matrix
+    from scipy.sparse.csgraph import dijkstra
+    out = {"graphs": 0, "bad": [], "removed": 0, "queries": 0}
+    for gi, (V, seed) in enumerate([(40, 1), (64, 2), (150, 3), (220, 4)]):
+        rng, rp, col, wt, groups = random_instance(V, 500 + seed, deg=4, gmax=3)
+        dist = dijkstra(csr_matrix((wt.astype(np.float64), col, rp), shape=(V, V)))
+        src = np.repeat(np.arange(V), np.diff(rp))
+        exact = wt <= dist[src, col]
+        g = Graph(rp, col, wt)
+        g.set_groups(*groups_csr(groups))
+        brp, bcol, bwt = g.backbone()
+        bsrc = np.repeat(np.arange(V), np.diff(brp))
+        kept = set(zip(bsrc.tolist(), bcol.tolist(), bwt.tolist()))
+        want = set(zip(src[exact].tolist(), col[exact].tolist(), wt[exact].tolist()))
+        allarcs = set(zip(src.tolist(), col.tolist(), wt.tolist()))
+        out["graphs"] += 1
+        out["removed"] += len(col) - len(bcol)
+        if V <= 64 and kept != want:
+            out["bad"].append([gi, "not exact", len(kept), len(want)])
+        if not want <= kept or not kept <= allarcs:
+            out["bad"].append([gi, "not a superset"])
+        if any((v, u, w) not in kept for (u, v, w) in kept):
+            out["bad"].append([gi, "asymmetric"])
+        if g.get_option("backbone_arcs") != len(bcol):
+            out["bad"].append([gi, "count"])
+        if V not in (64, 220):  # queries on two of the graphs (the GPU tests take many more)
+            g.close()
+            continue
+        G = len(groups)
+        qs = [[rng.randrange(G) for _ in range(rng.randint(2, 5))] for _ in range(5)]
+        Ds = [-1] * 4 + [3]
+        g.set_option("backbone", 0)
+        full = g.solve_batch(qs, Ds)
+        g.set_option("backbone", 1)
+        got = list(zip([g.solve(q, D=d) for q, d in zip(qs, Ds)], g.solve_batch(qs, Ds), full))
+        for i, (q, d) in enumerate(zip(qs, Ds)):
+            gl = [groups[x] for x in q]
+            exp = O.dpbf(rp, col, wt, gl, D=d)[0]
+            ok = all(r.cost == exp for r in got[i])
+            for r in got[i]:
+                if ok and r.feasible:
+                    ok = validate_tree(rp, col, wt, gl, r.cost, r.root, r.edges, D=d)["ok"]
+            out["queries"] += 1
+            if not ok:
+                out["bad"].append([gi, i, d, [r.cost for r in got[i]], exp])
+        g.close()
+    return out
 
 
 def check_graph():
@@ -393,7 +451,8 @@ def main():
     nh = int(sys.argv[3]) if len(sys.argv) > 3 else 24
     out = {"tiny": check_tiny(stride), "jobs": check_jobs(150, nq, 7), "hop": check_hop(150, nh, 11),
            "branching": check_branching(), "graph": check_graph(), "static": check_static(),
-           "robust": check_robust(), "cache": check_cache(), "progressive": check_progressive()}
+           "robust": check_robust(), "cache": check_cache(), "progressive": check_progressive(),
+           "backbone": check_backbone()}
     print(json.dumps(out), flush=True)
 
 

@@ -103,13 +103,15 @@ def _kernel_meta(text, sym):
     return {k: int(v) for k, v in re.findall(r"\.(vgpr_count|vgpr_spill_count|sgpr_spill_count):\s+(\d+)", block)}
 
 
-@pytest.mark.parametrize("cls,max_vgpr,max_vspill", [("lean", 96, 2), ("heavy", 168, 2)])
+@pytest.mark.parametrize("cls,max_vgpr,max_vspill", [("lean", 96, 4), ("heavy", 168, 2)])
 def test_step_kernel_register_budget(device_asm, cls, max_vgpr, max_vspill):
     """Register budgets of the step kernels: the lean kernel (job rounds) at
-    five waves per SIMD, the heavy kernel at three, each with at most two
-    VGPR spills.  Code added to rare items has pushed both further before (a
-    statistics pass is a kernel of its own for that reason, and the
-    progressive bounds carry nothing across LIVE's loop)."""
+    five waves per SIMD with at most four VGPR spills (two loop-invariant
+    8-byte values, reloaded once per work item, outside the arc loops), the
+    heavy kernel at three with at most two.  Code added to rare items has
+    pushed both further before (a statistics pass is a kernel of its own for
+    that reason, and the progressive bounds carry nothing across LIVE's
+    loop)."""
     m = _kernel_meta(device_asm, KERNELS[cls][0])
     assert m["vgpr_count"] <= max_vgpr, m
     assert m["vgpr_spill_count"] <= max_vspill, m

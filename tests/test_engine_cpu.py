@@ -89,3 +89,10 @@ def test_group_distance_cache_on_cpu_build(cpu_results):
 def test_progressive_answers_on_cpu_build(cpu_results):
     p = cpu_results["progressive"]
     assert p["bad"] == [], p
+
+
+def test_backbone_on_cpu_build(cpu_results):
+    r = cpu_results["backbone"]
+    assert r["graphs"] == 4 and r["queries"] == 10, r
+    assert r["removed"] > 0, r
+    assert r["bad"] == [], r["bad"]
