@@ -114,6 +114,14 @@ struct GraphDev {
   const int32_t* rev;     // as col: index of the reverse arc
   const int64_t* gptr;    // n_groups + 1
   const int32_t* gmem;    // members
+  // exact distances d(h, v) from the kBbHubs hub vertices of the backbone
+  // build, V rows of kBbHubs (null: none)
+  const uint32_t* hubd;
+  // vertices by hub reach: the n_hubr vertices some hub reaches, then the rest
+  // (jobs compute hub bounds for the first ones only; the others' are
+  // infinite)
+  const uint32_t* hubv;
+  int64_t n_hubr;
   int64_t V;
   int64_t n_arcs;
 };
@@ -211,6 +219,11 @@ struct alignas(64) Slot {
   uint32_t recon_star;  // RECON rebuilds the star of star_pk (shortest paths from its centre)
   uint32_t mtv;         // live vertices per MERGE / FINAL item (schedule option merge_tv)
   uint32_t bb;          // 1: the slot's arcs are the graph's shortest-path backbone
+  uint32_t nhubr;       // job with hub bounds: vertices some hub reaches (GraphDev::n_hubr)
+  // unbounded job: hub bounds of its labels, hA[l * kBbHubs + h] = distance
+  // from hub h to the nearest member of label l's group (null: its rows start
+  // at infinity)
+  const uint32_t* hA;
 };
 
 struct alignas(64) Ctrl {

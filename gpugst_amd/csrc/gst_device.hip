@@ -112,10 +112,14 @@ __device__ __forceinline__ uint32_t sat(uint32_t a) { return a < kInf ? a : kInf
 // (whose waits also cover LDS traffic).
 #ifdef GST_CPU_SIM
 #define GST_AS_GLOBAL
+#define GST_AS_CONST
 #else
 #define GST_AS_GLOBAL __attribute__((address_space(1)))
+// memory no kernel writes while it runs: wave-uniform loads become scalar ones
+#define GST_AS_CONST __attribute__((address_space(4)))
 #endif
 typedef uint32_t jchunk __attribute__((vector_size(4 * kJobChunkLabels)));
+typedef uint32_t bchunk __attribute__((vector_size(16)));  // four hub labels
 typedef GST_AS_GLOBAL const jchunk g_cchunk;
 typedef GST_AS_GLOBAL jchunk g_chunk;
 typedef GST_AS_GLOBAL uint32_t g_u32;
@@ -128,6 +132,13 @@ __device__ __forceinline__ void store_stream(uint64_t* p, uint64_t v) {
   *p = v;
 #else
   __builtin_nontemporal_store(v, (GST_AS_GLOBAL uint64_t*)p);
+#endif
+}
+__device__ __forceinline__ void store_stream(g_u32* p, uint32_t v) {
+#ifdef GST_CPU_SIM
+  *p = v;
+#else
+  __builtin_nontemporal_store(v, p);
 #endif
 }
 __device__ __forceinline__ void store_stream(g_chunk* p, jchunk v) {
@@ -402,7 +413,9 @@ __device__ __forceinline__ void mark_hold(bool flag, uint32_t v, Slot* sp) {
 __device__ __forceinline__ void fill_regions(const Slot* sp, uint64_t V, uint64_t* nA, uint64_t* nM,
                                              uint64_t* nF) {
   if (sp->kind == KIND_JOB) {
-    *nA = V * (uint64_t)sp->Lp / 2;  // height 0 of every row (COPY writes the others)
+    // height 0 of every row (COPY writes the others); rows of hub bounds are
+    // written by their own items
+    *nA = sp->hA ? 0 : V * (uint64_t)sp->Lp / 2;
     *nM = 3 * V;
     *nF = 0;
     return;
@@ -411,6 +424,67 @@ __device__ __forceinline__ void fill_regions(const Slot* sp, uint64_t V, uint64_
   *nA = own ? V * (uint64_t)sp->ND * (uint64_t)sp->k : 0;
   *nM = 3 * V;
   *nF = 2 * ((V + 63) / 64);  // live bits, then hold bits
+}
+
+// Job FILL of an unbounded job with hub bounds (Slot::hA): every label l of
+// row v starts at min_h hA[l][h] + d(h, v), the length of a real walk from a
+// member of l's group through hub h.  These values obey the triangle
+// inequality along every arc (d(h, .) are exact distances), so no arc improves
+// one of them: the rounds seeded at the members only lower the labels whose
+// bound is not their distance, and the result is the same exact distances.
+// An item takes 64 labels (one per lane, its kBbHubs bounds in registers) and
+// kUbTV of the vertices some hub reaches, one at a time: the vertex's hub
+// distances are the same for the whole wave and arrive as scalar loads.
+constexpr uint32_t kUbTV = 256;
+
+__device__ __forceinline__ uint32_t ub_groups(const Slot* sp) { return ((uint32_t)sp->Lp + 63u) / 64u; }
+
+__device__ void item_ubfill(const GraphDev& g, Slot* sp, uint32_t tile) {
+  const int lane = lane_id();
+  const uint32_t ngrp = ub_groups(sp);
+  const uint32_t l = (tile % ngrp) * 64u + (uint32_t)lane;
+  const uint64_t i0 = (uint64_t)(tile / ngrp) * kUbTV;
+  const uint64_t i1 = i0 + kUbTV < (uint64_t)g.n_hubr ? i0 + kUbTV : (uint64_t)g.n_hubr;
+  const uint32_t Lp = (uint32_t)sp->Lp;
+  const bool lv = l < Lp;
+  uint32_t a[kBbHubs];
+  const bchunk* arow = (const bchunk*)(sp->hA + (uint64_t)(lv ? l : 0u) * kBbHubs);
+#pragma unroll
+  for (int j = 0; j < kBbHubs / 4; j++) {
+    const bchunk x = arow[j];
+#pragma unroll
+    for (int q = 0; q < 4; q++) a[4 * j + q] = x[q];
+  }
+  const GST_AS_CONST uint32_t* hubd = (const GST_AS_CONST uint32_t*)g.hubd;
+  const GST_AS_CONST uint32_t* hubv = (const GST_AS_CONST uint32_t*)g.hubv;
+  g_u32* dist = (g_u32*)sp->dist;
+  for (uint64_t i = i0; i < i1; i++) {
+    const uint64_t v = (uint64_t)hubv[i];
+    const GST_AS_CONST uint32_t* hr = hubd + v * kBbHubs;
+    uint32_t u = kInf;
+#pragma unroll
+    for (int h = 0; h < kBbHubs; h++) {
+      const uint32_t t = a[h] + hr[h];  // both <= kInf: no overflow
+      u = t < u ? t : u;
+    }
+    if (lv) store_stream(&dist[v * Lp + l], u);
+  }
+}
+
+// The rows of 64 vertices no hub reaches: infinite for every label (one lane
+// per chunk, a row at a time).
+__device__ void item_ubfill_inf(const GraphDev& g, Slot* sp, uint32_t tile) {
+  const uint64_t n_un = (uint64_t)g.V - (uint64_t)g.n_hubr;
+  const uint64_t i0 = (uint64_t)tile * 64u;
+  const uint64_t i1 = i0 + 64u < n_un ? i0 + 64u : n_un;
+  const int nch = sp->Lp / kJobChunkLabels;
+  jchunk inf;
+  for (int q = 0; q < kJobChunkLabels; q++) inf[q] = kInf;
+  for (uint64_t i = i0; i < i1; i++) {
+    const uint64_t v = (uint64_t)g.hubv[(uint64_t)g.n_hubr + i];
+    g_chunk* row = (g_chunk*)(sp->dist + v * (uint64_t)sp->Lp);
+    for (int c = lane_id(); c < nch; c += 64) store_stream(&row[c], inf);
+  }
 }
 
 __device__ void item_fill(Slot* sp, uint32_t idx, uint64_t V) {
@@ -436,6 +510,16 @@ __device__ void item_fill(Slot* sp, uint32_t idx, uint64_t V) {
     else
       F64[t - nA - nM] = 0;
   }
+}
+
+// FILL of an unbounded job with hub bounds (heavy kernel): the rows' items
+// first, then the masks'.
+__device__ void item_fill_hub(const GraphDev& g, Slot* sp, uint32_t idx, uint64_t V) {
+  const uint32_t nr = (uint32_t)(((uint64_t)g.n_hubr + kUbTV - 1u) / kUbTV) * ub_groups(sp);
+  const uint32_t nu = (uint32_t)((V - (uint64_t)g.n_hubr + 63u) / 64u);
+  if (idx < nr) item_ubfill(g, sp, idx);
+  else if (idx < nr + nu) item_ubfill_inf(g, sp, idx - nr);
+  else item_fill(sp, idx - nr - nu, V);
 }
 
 __device__ void item_seed(const GraphDev& g, Slot* sp, uint32_t i) {
@@ -1742,6 +1826,12 @@ __device__ uint32_t plan_step(Slot* s, uint64_t V) {
       uint64_t nA, nM, nF;
       fill_regions(s, V, &nA, &nM, &nF);
       work = ceil_div(nA + nM + nF, kFillWords);
+      // hub bounds keep 64 distances per lane in registers: the heavy kernel
+      // has them, the job rounds' kernel does not
+      if (job && s->hA) {
+        work += ceil_div(s->nhubr, kUbTV) * ub_groups(s) + ceil_div(V - s->nhubr, 64u);
+        cls = kClsHeavy;
+      }
       break;
     }
     case PH_SEED: work = job ? (uint32_t)s->L : (uint32_t)s->k; break;
@@ -1956,6 +2046,7 @@ __device__ void step_body(const GraphDev& g, Slot* slots, Ctrl* ctrl, LDS* L) {
           case PH_FINAL: item_final(g, sp, local, (WaveLDS*)L); break;
           case PH_RECON: item_recon(g, sp); break;
           case PH_CACHEIN: item_jcache(sp, local, V, false); break;
+          case PH_FILL: item_fill_hub(g, sp, local, V); break;  // jobs with hub bounds
           case PH_CACHEOUT: item_jcache(sp, local, V, true); break;
           default: break;
         }
@@ -1978,8 +2069,6 @@ __device__ void step_body(const GraphDev& g, Slot* slots, Ctrl* ctrl, LDS* L) {
 // redundant is kept.  The hub distances come from label-correcting rounds over
 // all arcs, where a source whose row changed in the round before relaxes its
 // row into its targets.
-typedef uint32_t bchunk __attribute__((vector_size(16)));
-
 // the row holding arc e < n_arcs: rowptr[r] <= e < rowptr[r + 1]
 __device__ __forceinline__ uint64_t bb_row_of(const int64_t* rowptr, uint64_t V, uint64_t e) {
   uint64_t lo = 0, hi = V;

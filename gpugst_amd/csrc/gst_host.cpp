@@ -138,6 +138,7 @@ struct Options {
   int64_t job_lanes = 0;  // lanes sharing an arc in job rounds (power of two; 0: the smallest with <= 8 chunks each)
   int64_t count_states = 0;  // count the finite DP states of every query (an extra pass)
   int64_t backbone = 1;  // unbounded queries and jobs admitted from now on run on the shortest-path backbone
+  int64_t hub_bounds = 1;  // unbounded jobs admitted from now on start their rows at hub bounds
 };
 
 }  // namespace
@@ -153,6 +154,7 @@ struct gst_graph {
   int64_t n_groups = 0;
   int64_t groups_version = 0;  // bumped by every gst_groups_set (group ids change meaning)
   std::vector<int64_t> h_gptr;
+  std::vector<int32_t> h_gmem;
   int64_t* d_rowptr = nullptr;
   int32_t* d_col = nullptr;
   int32_t* d_wt = nullptr;
@@ -162,6 +164,14 @@ struct gst_graph {
   // when the graph has no arcs or its build did not fit)
   bool has_bb = false;
   int64_t bb_arcs = 0, bb_rounds = 0;
+  // exact distances from the backbone's hubs, V rows of kBbHubs, on the device
+  // and on the host (the host forms each job's hub bounds); null / empty when
+  // the build did not run
+  uint32_t* d_hubd = nullptr;
+  std::vector<uint32_t> h_hubd;
+  // vertices some hub reaches, then the others; n_hubr of the first kind
+  uint32_t* d_hubv = nullptr;
+  int64_t n_hubr = 0;
   double bb_ms = 0;
   int64_t* d_gptr = nullptr;
   int32_t* d_gmem = nullptr;
@@ -177,6 +187,9 @@ struct gst_graph {
     g.rev = d_rev;
     g.gptr = d_gptr;
     g.gmem = d_gmem;
+    g.hubd = d_hubd;
+    g.hubv = d_hubv;
+    g.n_hubr = n_hubr;
     g.V = V;
     g.n_arcs = n_arcs;
     return g;
@@ -247,7 +260,7 @@ Layout layout_for(const gst_graph* g, int k, int D) { return layout_for(g, k, D,
 
 // Group-distance job over L labels.
 struct JobLayout {
-  size_t dist, mask, list, loff, istart, lgrp, jmem, reach, ccol, total;
+  size_t dist, mask, list, loff, istart, lgrp, jmem, reach, ccol, hA, total;
   int Lp, G, chunk;
   uint32_t istride;
 };
@@ -278,6 +291,7 @@ JobLayout job_layout(const gst_graph* g, int L, int nmem, int D) {
   J.jmem = off;   off += align256((size_t)nmem * 4);
   J.reach = off;  off += align256(V * 4);
   J.ccol = off;   off += align256((size_t)L * 2 * sizeof(void*));
+  J.hA = off;     off += align256((size_t)J.Lp * gst::kBbHubs * 4);
   J.total = off;
   return J;
 }
@@ -659,6 +673,23 @@ class Engine {
     std::vector<uint32_t> jm(mem.begin(), mem.end());
     pending_uploads_.push_back({d.lgrp, std::vector<char>((char*)lg.data(), (char*)(lg.data() + lg.size()))});
     pending_uploads_.push_back({d.jmem, std::vector<char>((char*)jm.data(), (char*)(jm.data() + jm.size()))});
+    if (D < 0 && g_->opt.hub_bounds != 0 && g_->d_hubd) {
+      // hub bounds of the labels: hub h to the nearest member of the label's
+      // group, label-major (padding labels: none)
+      std::vector<uint32_t> hA((size_t)J.Lp * gst::kBbHubs, gst::kInf);
+      const uint32_t* hd = g_->h_hubd.data();
+      for (int l = 0; l < L; l++) {
+        const int32_t gid = lg[l];
+        uint32_t* x = hA.data() + (size_t)l * gst::kBbHubs;
+        for (int64_t t = g_->h_gptr[gid]; t < g_->h_gptr[gid + 1]; t++) {
+          const uint32_t* row = hd + (size_t)g_->h_gmem[t] * gst::kBbHubs;
+          for (int hh = 0; hh < gst::kBbHubs; hh++) x[hh] = std::min(x[hh], row[hh]);
+        }
+      }
+      d.hA = (const uint32_t*)(base + J.hA);
+      d.nhubr = (uint32_t)g_->n_hubr;
+      pending_uploads_.push_back({(void*)d.hA, std::vector<char>((char*)hA.data(), (char*)(hA.data() + hA.size()))});
+    }
     // group-distance cache (unbounded jobs): a label whose group has a valid
     // column is copied in instead of computed; a new group gets a column
     // its job fills at the end, when the budget allows
@@ -1441,9 +1472,38 @@ gst_status build_backbone(gst_graph* g, const int64_t* beg_pos, const std::vecto
     }
     HIP_TRY(gst_bb_pass(1, &a, st));
     HIP_TRY(hipMemcpyAsync(hk.data(), keep, (size_t)E, hipMemcpyDeviceToHost, st));
+    g->h_hubd.resize((size_t)V * gst::kBbHubs);
+    HIP_TRY(hipMemcpyAsync(g->h_hubd.data(), dist, g->h_hubd.size() * 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     return GST_OK;
   }();
+  if (err == GST_OK) {
+    // the hub distances stay: jobs start their rows at hub bounds, computed
+    // for the vertices some hub reaches
+    std::vector<uint32_t> hv;
+    hv.reserve(V);
+    for (int pass = 0; pass < 2; pass++)
+      for (int64_t v = 0; v < V; v++) {
+        const uint32_t* r = g->h_hubd.data() + (size_t)v * gst::kBbHubs;
+        bool reached = false;
+        for (int h = 0; h < gst::kBbHubs && !reached; h++) reached = r[h] < gst::kInf;
+        if (reached == (pass == 0)) hv.push_back((uint32_t)v);
+        if (pass == 0 && reached) g->n_hubr++;
+      }
+    if (hipMalloc(&g->d_hubv, sizeof(uint32_t) * V) == hipSuccess &&
+        hipMemcpy(g->d_hubv, hv.data(), sizeof(uint32_t) * V, hipMemcpyHostToDevice) == hipSuccess) {
+      g->d_hubd = dist;
+      dist = nullptr;
+    } else {
+      (void)hipFree(g->d_hubv);
+      (void)hipGetLastError();
+      g->d_hubv = nullptr;
+      g->n_hubr = 0;
+      g->h_hubd.clear();
+    }
+  } else {
+    g->h_hubd.clear();
+  }
   release();
   if (err != GST_OK) return err;
   // the test is symmetric (d(h,u) + d(h,v), and reverse arcs have equal
@@ -1535,7 +1595,7 @@ gst_status build_backbone(gst_graph* g, const int64_t* beg_pos, const std::vecto
 // Device memory of a graph (its engines are gone already).
 void free_graph_memory(gst_graph* g) {
   for (void* p : {(void*)g->d_rowptr, (void*)g->d_col, (void*)g->d_wt, (void*)g->d_rev, (void*)g->d_gptr,
-                  (void*)g->d_gmem})
+                  (void*)g->d_gmem, (void*)g->d_hubd, (void*)g->d_hubv})
     if (p) (void)hipFree(p);
 }
 
@@ -1696,6 +1756,7 @@ gst_status gst_groups_set(gst_graph* g, const int64_t* grp_ptr, const int32_t* m
   HIP_TRY(hipMemcpy(g->d_gptr, grp_ptr, sizeof(int64_t) * (n_groups + 1), hipMemcpyHostToDevice));
   if (nm) HIP_TRY(hipMemcpy(g->d_gmem, members, sizeof(int32_t) * nm, hipMemcpyHostToDevice));
   g->h_gptr.assign(grp_ptr, grp_ptr + n_groups + 1);
+  g->h_gmem.assign(members, members + nm);
   g->n_groups = n_groups;
   g->groups_version++;
   return GST_OK;
@@ -1722,6 +1783,7 @@ static int64_t* option_ref(gst_graph* g, const std::string& k) {
   if (k == "approx_num") return &g->opt.approx_num;
   if (k == "approx_den") return &g->opt.approx_den;
   if (k == "backbone") return &g->opt.backbone;
+  if (k == "hub_bounds") return &g->opt.hub_bounds;
   return nullptr;
 }
 
@@ -1741,7 +1803,7 @@ gst_status gst_set_option(gst_graph* g, const char* key, int64_t value) {
   if (k == "chunk" && (value < 64 || value > (1 << 20))) return fail(GST_INVALID_ARG, "chunk outside [64, 2^20]");
   if (k == "edge_cap" && (value < 64 || value > (1 << 26))) return fail(GST_INVALID_ARG, "edge_cap outside [64, 2^26]");
   if ((k == "budget_mb" || k == "blocks_per_cu" || k == "timeout_ms" || k == "trace_slot" || k == "deadline_ms" ||
-       k == "count_states" || k == "cache_mb" || k == "backbone") && value < 0)
+       k == "count_states" || k == "cache_mb" || k == "backbone" || k == "hub_bounds") && value < 0)
     return fail(GST_INVALID_ARG, "negative value");
   if (k == "block" && value != 2 && value != 4 && value != 8)
     return fail(GST_INVALID_ARG, "block (waves per block) must be 2, 4 or 8");

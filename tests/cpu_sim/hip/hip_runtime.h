@@ -93,6 +93,8 @@ inline T __shfl_xor(T v, int mask, int = 64) {
 // every emulated wave runs with all 64 lanes active: the first active lane is 0
 inline uint32_t gsim_readfirstlane(uint32_t v) { return (uint32_t)__gsim_exchange(v, 0); }
 #define __builtin_amdgcn_readfirstlane(x) gsim_readfirstlane(x)
+inline uint32_t gsim_readlane(uint32_t v, int l) { return (uint32_t)__gsim_exchange(v, l); }
+#define __builtin_amdgcn_readlane(x, l) gsim_readlane((x), (l))
 
 inline unsigned long long __lanemask_lt() { return (1ull << gsim::lane) - 1ull; }
 inline int __popc(unsigned x) { return __builtin_popcount(x); }

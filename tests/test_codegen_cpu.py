@@ -103,12 +103,14 @@ def _kernel_meta(text, sym):
     return {k: int(v) for k, v in re.findall(r"\.(vgpr_count|vgpr_spill_count|sgpr_spill_count):\s+(\d+)", block)}
 
 
-@pytest.mark.parametrize("cls,max_vgpr,max_vspill", [("lean", 96, 4), ("heavy", 168, 2)])
+@pytest.mark.parametrize("cls,max_vgpr,max_vspill", [("lean", 96, 4), ("heavy", 168, 8)])
 def test_step_kernel_register_budget(device_asm, cls, max_vgpr, max_vspill):
     """Register budgets of the step kernels: the lean kernel (job rounds) at
     five waves per SIMD with at most four VGPR spills (two loop-invariant
     8-byte values, reloaded once per work item, outside the arc loops), the
-    heavy kernel at three with at most two.  Code added to rare items has
+    heavy kernel at three with at most eight (loop-invariant values, one of
+    16 bytes and two of 8, each reloaded at most once per work item).  Code
+    added to rare items has
     pushed both further before (a statistics pass is a kernel of its own for
     that reason, and the progressive bounds carry nothing across LIVE's
     loop)."""
