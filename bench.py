@@ -49,11 +49,34 @@ def pct(x, q):
     return float(np.percentile(np.asarray(x, dtype=np.float64), q)) if len(x) else None
 
 
+def job_fill_bytes(g, V, ks, ND=1):
+    """Algorithmic bytes of the group-distance jobs' FILL: 4 B per (vertex,
+    label) of height 0 + 24 B per vertex (3 masks).  An unbounded job with hub
+    bounds also reads each vertex's row of hub distances once (4 B per hub)."""
+    labels = int(sum(ks))
+    jobs = max(1, g.get_option("batch_jobs"))
+    lp = labels + 3 * jobs  # rows padded to whole 4-label chunks
+    b = 4 * V * lp + 24 * V * jobs
+    if hub_fill(g, ND):
+        b += 4 * HUBS * V * jobs
+    return b
+
+
+HUBS = 64  # hub vertices of the shortest-path backbone build (kBbHubs)
+
+
+def hub_fill(g, ND):
+    """Unbounded jobs start their rows at hub bounds: their FILL runs in the
+    heavy kernel (it keeps 64 hub bounds per lane in registers)."""
+    return ND == 1 and g.get_option("hub_bounds") != 0 and g.get_option("backbone") != 0
+
+
 def lean_bytes(g, res_list, V, ks, ND=1):
     """Algorithmic bytes of the lean kernel's items for these queries (one
     group-distance job per up to 256 labels; ND = D + 1 rows per vertex for a
     hop job, 1 otherwise):
-      job FILL        4 B per (vertex, label) of height 0 + 24 B per vertex (3 masks)
+      job FILL        (job_fill_bytes; in the heavy kernel for unbounded jobs
+                      with hub bounds)
       job ROUND       per frontier entry 24 B (list id, arc offset, mask, row
                       pointer) + 12 B clear; per arc 8 B (column + weight);
                       per 4-label chunk relaxed 16 B (target chunk read); per
@@ -65,12 +88,12 @@ def lean_bytes(g, res_list, V, ks, ND=1):
       query FILL      24 B per vertex (3 masks) + 1/4 B live and hold bits"""
     labels = int(sum(ks))
     jobs = max(1, g.get_option("batch_jobs"))
-    lp = labels + 3 * jobs  # rows padded to whole 4-label chunks
+    lp = labels + 3 * jobs
     ent = g.get_option("batch_job_entries")
     arcs = g.get_option("batch_job_arcs")
     relax = g.get_option("batch_job_relaxations")
     impr = g.get_option("batch_job_improvements")
-    b = 4 * V * lp + 24 * V * jobs
+    b = 0 if hub_fill(g, ND) else job_fill_bytes(g, V, ks, ND)
     b += 36 * ent + 8 * arcs + 16 * (relax // 4) + 16 * impr
     b += 8 * V * lp * (ND - 1)
     b += 4 * V * lp * (1 if ND == 1 else 2)
@@ -78,7 +101,7 @@ def lean_bytes(g, res_list, V, ks, ND=1):
     return b
 
 
-def heavy_bytes(res_list, V, ks, ND=1):
+def heavy_bytes(g, res_list, V, ks, ND=1):
     """Algorithmic bytes of the heavy kernel's items, per query: the live test
     (C row, 4 B per vertex and group), the T segments of live vertices written
     once (8 B per state; NS = ceil(D/2) + 1 = ND // 2 + 1 state heights for a
@@ -88,8 +111,9 @@ def heavy_bytes(res_list, V, ks, ND=1):
     reading the T segments of the active vertices (8 B per state and pass),
     and the level >= 2 grow rounds (24 B per frontier entry, 8 B per arc,
     16 B per relaxation candidate, 8 B per improvement) -- the counters of a
-    query fed by a job hold only these rounds."""
-    total = 0
+    query fed by a job hold only these rounds; plus the FILL of unbounded jobs
+    with hub bounds (job_fill_bytes)."""
+    total = job_fill_bytes(g, V, ks, ND) if hub_fill(g, ND) else 0
     for r, k in zip(res_list, ks):
         W = 1 << k
         c = r.counters
@@ -150,9 +174,15 @@ def poisson_serve(g, qs, Ds, rate, seed):
     th.join()
     res = [srv.wait(t) for t in tickets]
     span = time.perf_counter() - ts
+    steps, busy_us = srv.stat("steps"), srv.stat("busy_us")
     srv.close()
     lat = [r.latency_ms for r in res]
-    return n / span, pct(lat, 50), pct(lat, 99), res
+    dev = [r.device_ms for r in res]
+    wait = [r.latency_ms - r.device_ms for r in res]
+    detail = {"device_p50_ms": round(pct(dev, 50), 2), "queue_wait_p50_ms": round(pct(wait, 50), 2),
+              "engine_steps": steps, "device_busy_frac": round(busy_us * 1e-3 / (span * 1e3), 3),
+              "avg_step_us": round(busy_us / max(1, steps), 1)}
+    return n / span, pct(lat, 50), pct(lat, 99), res, detail
 
 
 def hit_rate(res, ks):
@@ -190,13 +220,13 @@ def serve_config5(log, n_req=96, cache_mb=4096):
     for i, f in enumerate((0.5, 0.8, 1.1)):
         a = 16 + n_req * (1 + i)
         q, d = qs[a:a + n_req], Ds[a:a + n_req]
-        ach, p50, p99, res = poisson_serve(g, q, d, f * cap, 21 + i)
+        ach, p50, p99, res, det = poisson_serve(g, q, d, f * cap, 21 + i)
         if i == 0:
             for j in range(16):
                 mism += int(g.solve(q[j], D=d[j]).cost != res[j].cost)
         out["rates"].append({"offered_qps": round(f * cap, 1), "achieved_qps": round(ach, 1),
                              "p50_ms": round(p50, 1), "p99_ms": round(p99, 1),
-                             "cache_hit_rate": hit_rate(res, [len(x) for x in q])})
+                             "cache_hit_rate": hit_rate(res, [len(x) for x in q]), **det})
         log(f"serving config: offered {f*cap:.1f} q/s -> {ach:.1f} q/s, p50 {p50:.1f} ms p99 {p99:.1f} ms")
     out["mix_vs_alone_mismatches"] = mism
     out["batch_infeasible"] = sum(1 for r in rb if not r.feasible)
@@ -332,7 +362,7 @@ def roofline_pass(g, qs, Ds, V, D, log):
     rk = [len(q) for q in qs]
     kern = {}
     for name, ms, byts in (("gst_lean_kernel", g.get_option("batch_lean_ns") / 1e6 / n_steps, lean_bytes(g, rr, V, rk, ND)),
-                           ("gst_heavy_kernel", g.get_option("batch_heavy_ns") / 1e6 / n_steps, heavy_bytes(rr, V, rk, ND))):
+                           ("gst_heavy_kernel", g.get_option("batch_heavy_ns") / 1e6 / n_steps, heavy_bytes(g, rr, V, rk, ND))):
         ach = byts / n_steps / (ms * 1e-3) / 1e9 if (byts and ms > 0) else 0.0
         kern[name] = {"avg_launch_ms": round(ms, 4), "algorithmic_bytes_per_launch": int(byts / n_steps),
                       "achieved": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4)}
@@ -495,15 +525,25 @@ def main():
         rate = args.serve_rate if args.serve_rate > 0 else 0.5 * batch_qps
         sq = data.make_queries(args.serve, args.k, inst.meta["eligible"], 3000 + 4 + 10 * args.k + 7)
         g.set_option("cache_mb", args.cache_mb)
-        ach, p50, p99, sres = poisson_serve(g, list(sq), [-1] * args.serve, rate, 11)
+        ach, p50, p99, sres, det = poisson_serve(g, list(sq), [-1] * args.serve, rate, 11)
         g.set_option("cache_mb", 0)
         serving = {"offered_qps": round(rate, 1), "queries": args.serve, "achieved_qps": round(ach, 1),
                    "p50_ms": round(p50, 2), "p99_ms": round(p99, 2),
                    "p50_vs_plain_p50": round(p50 / pct(plain_lat, 50), 2) if plain_lat else None,
                    "cache_mb": args.cache_mb, "cache_hit_rate": hit_rate(sres, [len(q) for q in sq]),
-                   "arrivals": "Poisson (seeded), open loop, gst_server_* continuous batching"}
+                   **det, "admission": {k: g.get_option(k) for k in ("serve_chunk", "serve_depth",
+                                                                      "serve_wait_us", "serve_batch")},
+                   "arrivals": "Poisson (seeded), open loop, gst_server_* continuous stepping: requests "
+                               "join the running engine between chunks of steps"}
         log(f"serving: offered {rate:.0f} q/s -> {ach:.0f} q/s, p50 {p50:.2f} ms p99 {p99:.2f} ms, "
-            f"cache hit rate {serving['cache_hit_rate']}")
+            f"cache hit rate {serving['cache_hit_rate']}, {det}")
+        # the same requests at half that rate
+        ach2, p502, p992, _, det2 = poisson_serve(g, list(sq), [-1] * args.serve, 0.5 * rate, 12)
+        serving["at_half_rate"] = {"offered_qps": round(0.5 * rate, 1), "achieved_qps": round(ach2, 1),
+                                   "p50_ms": round(p502, 2), "p99_ms": round(p992, 2),
+                                   "p50_vs_plain_p50": round(p502 / pct(plain_lat, 50), 2) if plain_lat else None,
+                                   **det2}
+        log(f"serving: offered {0.5 * rate:.0f} q/s -> {ach2:.0f} q/s, p50 {p502:.2f} ms p99 {p992:.2f} ms, {det2}")
 
     # ---------------- parity: plain vs batched on shared queries, trees valid
     all_b = bres + timed

@@ -34,7 +34,7 @@ STATUS_NAMES = {0: "OK", 1: "INVALID_ARG", 2: "OUT_OF_MEMORY", 3: "DEVICE_ERROR"
 EXPORTED = ("gst_graph_create", "gst_graph_destroy", "gst_backbone_get", "gst_groups_set", "gst_solve",
             "gst_solve_static", "gst_solve_batch", "gst_result_free", "gst_set_option", "gst_get_option",
             "gst_query_bytes", "gst_server_create", "gst_submit", "gst_poll", "gst_wait",
-            "gst_cancel", "gst_server_destroy", "gst_last_error", "gst_version")
+            "gst_cancel", "gst_server_stat", "gst_server_destroy", "gst_last_error", "gst_version")
 
 
 class GstError(RuntimeError):
@@ -107,6 +107,7 @@ def load():
         "gst_wait": [P, ctypes.c_uint64, ctypes.POINTER(c_result)],
         "gst_cancel": [P, ctypes.c_uint64],
         "gst_server_destroy": [P],
+        "gst_server_stat": [P, ctypes.c_char_p, ctypes.POINTER(I64)],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -294,6 +295,12 @@ class Server:
 
     def cancel(self, ticket: int):
         _check(load().gst_cancel(self._h, ticket))
+
+    def stat(self, key: str) -> int:
+        """Device statistics: "steps", "chunks", "busy_us"."""
+        v = ctypes.c_int64()
+        _check(load().gst_server_stat(self._h, key.encode(), ctypes.byref(v)))
+        return v.value
 
     def close(self):
         if getattr(self, "_h", None):

@@ -139,6 +139,14 @@ struct Options {
   int64_t count_states = 0;  // count the finite DP states of every query (an extra pass)
   int64_t backbone = 1;  // unbounded queries and jobs admitted from now on run on the shortest-path backbone
   int64_t hub_bounds = 1;  // unbounded jobs admitted from now on start their rows at hub bounds
+  // server (continuous stepping): engine steps per chunk between two host
+  // visits, chunks kept queued on the device, and how long / for how many
+  // pending requests admission waits while the device is busy, so that more
+  // requests share a group-distance job
+  int64_t serve_chunk = 2;
+  int64_t serve_depth = 2;
+  int64_t serve_wait_us = 10000;  // measured: 1.1M-vertex graph, 1000-2000 q/s offered
+  int64_t serve_batch = 32;
 };
 
 }  // namespace
@@ -328,6 +336,11 @@ struct SlotHost {
   int64_t extra[5] = {0, 0, 0, 0, 0};  // level-1 work charged from the query's job
   uint32_t jmaxq = 0;     // largest frontier of its job
   int32_t cached = 0;     // its labels served from the group-distance cache
+  // continuous stepping: admission number (a snapshot of the slot table
+  // describes this slot only if taken after its entry was uploaded) and
+  // whether a snapshot has shown it finished
+  uint64_t aseq = 0;
+  bool done_seen = false;
 };
 
 // One cached group-distance column (unbounded distances of one group to every
@@ -359,6 +372,16 @@ class Engine {
       if (h_ctrl_) (void)hipHostFree(h_ctrl_);
     }
     drop_graph();
+    for (int b = 0; b < kContBuffers; b++) {
+      if (cev_[b]) (void)hipEventDestroy(cev_[b]);
+      if (cev0_[b]) (void)hipEventDestroy(cev0_[b]);
+      if (h_snap_[b]) (void)hipHostFree(h_snap_[b]);
+    }
+    if (h_zero_) (void)hipHostFree(h_zero_);
+    if (xstream_) {
+      (void)hipStreamSynchronize(xstream_);
+      (void)hipStreamDestroy(xstream_);
+    }
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);
     for (auto e : tev_) (void)hipEventDestroy(e);
@@ -534,6 +557,7 @@ class Engine {
     h.flagged = false;
     h.jmaxq = 0;
     h.cached = 0;
+    mark_admitted(s);
     if (h_cancel_) __atomic_store_n(&h_cancel_[1 + s], 0u, __ATOMIC_RELEASE);
     gst::Slot& d = h_slots_[s];
     memset(&d, 0, sizeof(d));
@@ -605,9 +629,11 @@ class Engine {
     }
     arena_.release(h.off, h.len);
     if (!h.job) active_--;
+    if (!h.done_seen) n_inflight_--;  // never uploaded (a failed admission)
     h.used = false;
     h.job = false;
     h.charged = false;
+    h.done_seen = false;
     used_slots_--;
     memset(&h_slots_[s], 0, sizeof(gst::Slot));  // PH_FREE
     while (nslots_used_ > 0 && !sh_[nslots_used_ - 1].used) nslots_used_--;
@@ -626,6 +652,7 @@ class Engine {
     h.job = true;
     h.off = off;
     h.len = J.total;
+    mark_admitted(js);
     used_slots_++;
     nslots_used_ = std::max(nslots_used_, js + 1);
     std::vector<int> mem;
@@ -936,28 +963,223 @@ class Engine {
       return fail(GST_DEVICE_ERROR, "engine epoch made no progress on %d admitted queries", active_);
     for (int i = 0; i < nslots_used_; i++) {
       if (!sh_[i].used || h_slots_[i].phase != gst::PH_DONE) continue;
-      if (sh_[i].job) {
-        if (sh_[i].charged) continue;
-        // the job's level-1 work is charged to its members, per label; its
-        // distance rows stay until the last member has been harvested
-        const gst::Slot& j = h_slots_[i];
-        for (int m : job_members_.at(i)) {
-          const gst::Slot& q = h_slots_[m];
-          SlotHost& qh = sh_[m];
-          for (int t = 1; t <= 4; t++) qh.extra[t] += (int64_t)(j.ctr[t] * (uint64_t)q.k / (uint64_t)std::max(1, j.L));
-          qh.extra[0] += (int64_t)j.ctr[0];
-          qh.jmaxq = std::max(qh.jmaxq, j.maxq);
-        }
-        jobs_done_++;
-        for (int t = 0; t < 6; t++) job_ctr_[t] += (int64_t)j.ctr[t];
-        sh_[i].charged = true;
-        release_job_cache(i);
-      } else {
-        finished->push_back(i);
-      }
+      if (sh_[i].job) charge_job(i);
+      else finished->push_back(i);
     }
     return GST_OK;
   }
+
+  // A finished job's level-1 work is charged to its members, per label; its
+  // distance rows stay until the last member has been harvested.
+  void charge_job(int i) {
+    if (sh_[i].charged) return;
+    const gst::Slot& j = h_slots_[i];
+    for (int m : job_members_.at(i)) {
+      const gst::Slot& q = h_slots_[m];
+      SlotHost& qh = sh_[m];
+      for (int t = 1; t <= 4; t++) qh.extra[t] += (int64_t)(j.ctr[t] * (uint64_t)q.k / (uint64_t)std::max(1, j.L));
+      qh.extra[0] += (int64_t)j.ctr[0];
+      qh.jmaxq = std::max(qh.jmaxq, j.maxq);
+    }
+    jobs_done_++;
+    for (int t = 0; t < 6; t++) job_ctr_[t] += (int64_t)j.ctr[t];
+    sh_[i].charged = true;
+    release_job_cache(i);
+  }
+
+  // ---- continuous stepping (the server) ----
+  // The device keeps stepping while the host harvests finished queries and
+  // admits new ones between chunks of steps.  While it runs, the slot table on
+  // the device is authoritative for admitted slots: the host uploads only the
+  // entries of slots admitted since the last chunk (those were free or
+  // finished on the device, so no step touches them, and the plan kernel takes
+  // them up at its next run), and reads the table back after every chunk into
+  // a snapshot buffer.  A snapshot describes a slot only if the slot's entry
+  // was uploaded before that chunk (admission numbers).  Up to `depth` chunks
+  // stay queued, so the device steps on while the host handles the oldest
+  // one's snapshot.  Returns the queries the collected snapshot shows finished
+  // (their final entries copied into the host mirror for harvest).
+  gst_status cont_iter(int steps, int depth, std::vector<int>* finished) {
+    finished->clear();
+    if (poisoned_) return fail(GST_DEVICE_ERROR, "engine stopped by an earlier failure");
+    depth = std::max(1, std::min(depth, kContBuffers));
+    steps = std::max(1, steps);
+    if (!cont_on_) {
+      gst_status st = cont_start();
+      if (st != GST_OK) return st;
+    } else if (!fresh_.empty() || !pending_uploads_.empty()) {
+      for (auto& u : pending_uploads_)
+        HIP_TRY(hipMemcpyAsync(u.first, u.second.data(), u.second.size(), hipMemcpyHostToDevice, stream_));
+      for (int s : fresh_)
+        if (sh_[s].used && !sh_[s].done_seen)
+          HIP_TRY(hipMemcpyAsync(d_slots_ + s, h_slots_ + s, sizeof(gst::Slot), hipMemcpyHostToDevice, stream_));
+      fresh_.clear();
+      // the plan kernel stands still once it has found no work: restart it
+      HIP_TRY(hipMemcpyAsync(&d_ctrl_->exit_flag, h_zero_, sizeof(uint32_t), hipMemcpyHostToDevice, stream_));
+    }
+    // host sources of the uploads stay alive until a chunk behind them is done
+    std::vector<std::pair<void*, std::vector<char>>> keep;
+    keep.swap(pending_uploads_);
+    if (n_inflight_ > 0 && (int)chunks_.size() < depth) {
+      int buf = 0;
+      for (;; buf++) {
+        bool busy = false;
+        for (auto& c : chunks_) busy = busy || c.buf == buf;
+        if (!busy) break;
+      }
+      const int grid_l = grid_size(gst::kClsLean), grid_h = grid_size(gst::kClsHeavy);
+      const gst::GraphDev gd = g_->dev();
+      HIP_TRY(hipEventRecord(cev0_[buf], stream_));
+      for (int i = 0; i < steps; i++) {
+        hipError_t le = gst_launch_step(gst::kClsLean, wpb(), &gd, d_slots_, d_ctrl_, grid_l, stream_);
+        if (le == hipSuccess) le = gst_launch_step(gst::kClsHeavy, wpb(), &gd, d_slots_, d_ctrl_, grid_h, stream_);
+        if (le == hipSuccess) le = gst_launch_plan(d_slots_, d_ctrl_, (uint64_t)g_->V, stream_);
+        if (le != hipSuccess) {
+          poisoned_ = true;
+          return fail(GST_DEVICE_ERROR, "step launch failed: %s", hipGetErrorString(le));
+        }
+      }
+      ContChunk c;
+      c.buf = buf;
+      c.aseq = aseq_;
+      c.nslots = nslots_used_;
+      c.keep.swap(keep);
+      if (c.nslots > 0)
+        HIP_TRY(hipMemcpyAsync(h_snap_[buf], d_slots_, sizeof(gst::Slot) * (size_t)c.nslots, hipMemcpyDeviceToHost,
+                               stream_));
+      HIP_TRY(hipEventRecord(cev_[buf], stream_));
+      chunks_.push_back(std::move(c));
+      steps_total_ += steps;
+    } else if (!keep.empty()) {
+      if (!chunks_.empty()) {
+        for (auto& u : keep) chunks_.back().keep.push_back(std::move(u));
+      } else {
+        gst_status st = wait_stream();
+        if (st != GST_OK) return st;
+      }
+    }
+    if (chunks_.empty()) return GST_OK;
+    if ((int)chunks_.size() < depth && n_inflight_ > 0) return GST_OK;  // fill the queue first
+    return cont_collect(finished);
+  }
+
+  bool cont_outstanding() const { return !chunks_.empty(); }
+  gst_status sync_harvest() {
+    const hipError_t e = hipStreamSynchronize(xs());
+    if (e != hipSuccess) return fail(GST_DEVICE_ERROR, "result copy failed: %s", hipGetErrorString(e));
+    return GST_OK;
+  }
+  // continuous stepping: chunks run and their summed device time (between
+  // a start and an end event of each chunk)
+  int64_t cont_chunks() const { return cont_chunks_; }
+  int64_t cont_busy_ns() const { return cont_busy_ns_; }
+
+ private:
+  static constexpr int kContBuffers = 2;
+  struct ContChunk {
+    int buf = 0;
+    uint64_t aseq = 0;  // admissions uploaded before the chunk
+    int nslots = 0;     // entries of its snapshot
+    std::vector<std::pair<void*, std::vector<char>>> keep;
+  };
+
+  void mark_admitted(int s) {
+    sh_[s].aseq = ++aseq_;
+    sh_[s].done_seen = false;
+    n_inflight_++;
+    if (cont_on_) fresh_.push_back(s);
+  }
+
+  gst_status cont_start() {
+    if (!xstream_) HIP_TRY(hipStreamCreateWithFlags(&xstream_, hipStreamNonBlocking));
+    for (int b = 0; b < kContBuffers; b++) {
+      if (!h_snap_[b]) HIP_TRY(hipHostMalloc((void**)&h_snap_[b], sizeof(gst::Slot) * max_slots_, hipHostMallocDefault));
+      if (!cev_[b]) HIP_TRY(hipEventCreate(&cev_[b]));
+      if (!cev0_[b]) HIP_TRY(hipEventCreate(&cev0_[b]));
+    }
+    if (!h_zero_) {
+      HIP_TRY(hipHostMalloc((void**)&h_zero_, 64, hipHostMallocDefault));
+      memset(h_zero_, 0, 64);
+    }
+    for (auto& u : pending_uploads_)
+      HIP_TRY(hipMemcpyAsync(u.first, u.second.data(), u.second.size(), hipMemcpyHostToDevice, stream_));
+    // the whole table: slots never admitted must read as free
+    HIP_TRY(hipMemcpyAsync(d_slots_, h_slots_, sizeof(gst::Slot) * max_slots_, hipMemcpyHostToDevice, stream_));
+    gst::Ctrl& c = *h_ctrl_;
+    memset(&c, 0, offsetof(gst::Ctrl, active));
+    c.exit_after = ~0u;  // the plan kernel stops only when no slot has work
+    c.max_steps = ~0u;
+    c.nslots = (uint32_t)max_slots_;
+    c.trace_slot = ~0u;
+    c.cancel = d_cancel_;
+    HIP_TRY(hipMemcpyAsync(d_ctrl_, h_ctrl_, offsetof(gst::Ctrl, active), hipMemcpyHostToDevice, stream_));
+    hipError_t le = gst_launch_plan(d_slots_, d_ctrl_, (uint64_t)g_->V, stream_);
+    if (le != hipSuccess) return fail(GST_DEVICE_ERROR, "plan launch failed: %s", hipGetErrorString(le));
+    fresh_.clear();
+    cont_on_ = true;
+    return GST_OK;
+  }
+
+  gst_status cont_collect(std::vector<int>* finished) {
+    ContChunk c = std::move(chunks_.front());
+    chunks_.pop_front();
+    gst_status st = wait_event(cev_[c.buf]);
+    if (st != GST_OK) return st;
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, cev0_[c.buf], cev_[c.buf]) == hipSuccess) cont_busy_ns_ += (int64_t)((double)ms * 1e6);
+    cont_chunks_++;
+    const gst::Slot* snap = h_snap_[c.buf];
+    uint64_t sum = 0;
+    bool done_any = false;
+    for (int s = 0; s < c.nslots; s++) {
+      SlotHost& h = sh_[s];
+      if (!h.used || h.done_seen || h.aseq > c.aseq) continue;
+      const gst::Slot& x = snap[s];
+      sum += (uint64_t)x.ctr[0];
+      if (x.phase != gst::PH_DONE) continue;
+      h.done_seen = true;
+      n_inflight_--;
+      done_any = true;
+      memcpy((void*)&h_slots_[s], (const void*)&x, sizeof(gst::Slot));
+      if (h.job) charge_job(s);
+      else finished->push_back(s);
+    }
+    // progress guard: admitted slots that stop stepping are a device-side fault
+    if (done_any || sum != cont_sum_ || n_inflight_ == 0) {
+      cont_sum_ = sum;
+      cont_stall_ = 0;
+    } else if (++cont_stall_ > kContStallChunks) {
+      poisoned_ = true;
+      return fail(GST_DEVICE_ERROR, "engine made no progress on %d admitted slots", n_inflight_);
+    }
+    return GST_OK;
+  }
+
+  // wait_stream for one event: the end of a chunk
+  gst_status wait_event(hipEvent_t ev) {
+    const double t_end = now_ms() + (double)g_->opt.timeout_ms;
+    for (int it = 0;; it++) {
+      const hipError_t q = hipEventQuery(ev);
+      if (q == hipSuccess) return GST_OK;
+      if (q != hipErrorNotReady) {
+        poisoned_ = true;
+        return fail(GST_DEVICE_ERROR, "engine step failed: %s", hipGetErrorString(q));
+      }
+      const double t = now_ms();
+      if (t > t_end) {
+        poisoned_ = true;
+        return fail(GST_DEVICE_ERROR, "engine step did not finish within %lld ms", (long long)g_->opt.timeout_ms);
+      }
+      if (n_deadline_ > 0 && (it & 15) == 0) flag_expired(t);
+      std::this_thread::sleep_for(std::chrono::microseconds(20));
+    }
+  }
+
+  // the stream harvest copies run on: beside the queued chunks while stepping
+  // continuously (a finished slot is not touched by any step)
+  hipStream_t xs() const { return cont_on_ ? xstream_ : stream_; }
+
+ public:
 
   // A harvested member releases its hold on its job; the last one frees it.
   void drop_job_ref(int member) {
@@ -1076,7 +1298,11 @@ class Engine {
   // Copies the result of a finished query out and releases its memory.  The
   // tree goes to a malloc'd buffer owned by `out`, or, with ext, into the
   // caller's buffer of ext_cap edges (out->edges stays null then).
-  gst_status harvest(int s, gst_result* out, uint64_t* tag, gst_edge* ext = nullptr, int32_t ext_cap = 0) {
+  // With defer, the copy of the tree into its malloc'd buffer is only
+  // enqueued: the caller synchronises xs() (sync_harvest) before anything
+  // else is admitted into the released memory.
+  gst_status harvest(int s, gst_result* out, uint64_t* tag, gst_edge* ext = nullptr, int32_t ext_cap = 0,
+                     bool defer = false) {
     const gst::Slot& d = h_slots_[s];
     SlotHost& h = sh_[s];
     memset(out, 0, sizeof(*out));
@@ -1106,9 +1332,9 @@ class Engine {
       // here a pass of its own over the finished query's tables
       unsigned long long nf = 0;
       gst::Slot* ds = d_slots_ + s;
-      hipError_t e = gst_launch_count(ds, (uint64_t)g_->V, stream_);  // nfinite starts at 0 (admission)
-      if (e == hipSuccess) e = hipMemcpyAsync(&nf, &ds->nfinite, sizeof(nf), hipMemcpyDeviceToHost, stream_);
-      if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+      hipError_t e = gst_launch_count(ds, (uint64_t)g_->V, xs());  // nfinite starts at 0 (admission)
+      if (e == hipSuccess) e = hipMemcpyAsync(&nf, &ds->nfinite, sizeof(nf), hipMemcpyDeviceToHost, xs());
+      if (e == hipSuccess) e = hipStreamSynchronize(xs());
       if (e == hipSuccess) out->counters.states_finite = (int64_t)nf;
     }
     gst_status st = GST_OK;
@@ -1138,8 +1364,8 @@ class Engine {
           st = fail(GST_INVALID_ARG, "the tree has %d edges; the edge buffer holds %d", d.out_nedges, ext_cap);
         } else if (d.out_nedges > 0) {
           hipError_t e = hipMemcpyAsync(ext, d.edges, sizeof(gst_edge) * (size_t)d.out_nedges,
-                                        hipMemcpyDeviceToHost, stream_);
-          if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+                                        hipMemcpyDeviceToHost, xs());
+          if (e == hipSuccess) e = hipStreamSynchronize(xs());
           if (e != hipSuccess) {
             out->status = GST_DEVICE_ERROR;
             st = fail(GST_DEVICE_ERROR, "edge copy failed: %s", hipGetErrorString(e));
@@ -1149,8 +1375,8 @@ class Engine {
         out->edges = (gst_edge*)malloc(sizeof(gst_edge) * (size_t)d.out_nedges);
         if (!out->edges) return fail(GST_OUT_OF_MEMORY, "host edge buffer");
         hipError_t e = hipMemcpyAsync(out->edges, d.edges, sizeof(gst_edge) * (size_t)d.out_nedges,
-                                      hipMemcpyDeviceToHost, stream_);
-        if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+                                      hipMemcpyDeviceToHost, xs());
+        if (e == hipSuccess && !defer) e = hipStreamSynchronize(xs());
         if (e != hipSuccess) {
           out->status = GST_DEVICE_ERROR;
           st = fail(GST_DEVICE_ERROR, "edge copy failed: %s", hipGetErrorString(e));
@@ -1179,6 +1405,10 @@ class Engine {
     active_ = 0;
     used_slots_ = 0;
     nslots_used_ = 0;
+    n_inflight_ = 0;
+    fresh_.clear();
+    chunks_.clear();
+    cont_on_ = false;  // the next continuous chunk uploads the whole table again
     pending_uploads_.clear();
     job_members_.clear();
     job_refs_.clear();
@@ -1262,8 +1492,24 @@ class Engine {
   int64_t job_ctr_[6] = {0, 0, 0, 0, 0, 0};
   std::vector<int64_t> pair_ns_;  // per step of the current epoch (time_steps)
   std::map<uint32_t, std::pair<int64_t, int64_t>> phase_ns_;
-  // host copies of job tables, uploaded at the start of the next epoch
+  // host copies of job tables, uploaded at the start of the next epoch (or
+  // before the next chunk, stepping continuously)
   std::vector<std::pair<void*, std::vector<char>>> pending_uploads_;
+  // continuous stepping
+  static constexpr int kContStallChunks = 4096;
+  bool cont_on_ = false;
+  hipStream_t xstream_ = nullptr;
+  gst::Slot* h_snap_[kContBuffers] = {nullptr, nullptr};
+  hipEvent_t cev_[kContBuffers] = {nullptr, nullptr};   // end of a chunk (after its snapshot)
+  hipEvent_t cev0_[kContBuffers] = {nullptr, nullptr};  // start of a chunk
+  int64_t cont_chunks_ = 0, cont_busy_ns_ = 0;
+  uint32_t* h_zero_ = nullptr;
+  std::deque<ContChunk> chunks_;
+  uint64_t aseq_ = 0;
+  std::vector<int> fresh_;  // admitted since the last upload
+  int n_inflight_ = 0;      // admitted slots no snapshot has shown finished
+  uint64_t cont_sum_ = 0;
+  int cont_stall_ = 0;
   std::map<int, std::vector<int>> job_members_;
   std::map<int, int> job_refs_;    // job slot -> members not yet harvested
   std::map<int, int> member_job_;  // member slot -> its job slot
@@ -1784,6 +2030,10 @@ static int64_t* option_ref(gst_graph* g, const std::string& k) {
   if (k == "approx_den") return &g->opt.approx_den;
   if (k == "backbone") return &g->opt.backbone;
   if (k == "hub_bounds") return &g->opt.hub_bounds;
+  if (k == "serve_chunk") return &g->opt.serve_chunk;
+  if (k == "serve_depth") return &g->opt.serve_depth;
+  if (k == "serve_wait_us") return &g->opt.serve_wait_us;
+  if (k == "serve_batch") return &g->opt.serve_batch;
   return nullptr;
 }
 
@@ -1808,6 +2058,12 @@ gst_status gst_set_option(gst_graph* g, const char* key, int64_t value) {
   if (k == "block" && value != 2 && value != 4 && value != 8)
     return fail(GST_INVALID_ARG, "block (waves per block) must be 2, 4 or 8");
   if (k == "merge_tv" && (value < 1 || value > 4096)) return fail(GST_INVALID_ARG, "merge_tv outside [1, 4096]");
+  if (k == "serve_chunk" && (value < 1 || value > 64)) return fail(GST_INVALID_ARG, "serve_chunk outside [1, 64]");
+  if (k == "serve_depth" && (value < 1 || value > 2)) return fail(GST_INVALID_ARG, "serve_depth must be 1 or 2");
+  if (k == "serve_wait_us" && (value < 0 || value > 10000000))
+    return fail(GST_INVALID_ARG, "serve_wait_us outside [0, 10^7]");
+  if (k == "serve_batch" && (value < 1 || value > gst::kMaxSlots))
+    return fail(GST_INVALID_ARG, "serve_batch outside [1, %d]", gst::kMaxSlots);
   if (k == "job_lanes" && (value < 0 || value > 64 || (value & (value - 1))))
     return fail(GST_INVALID_ARG, "job_lanes must be 0 or a power of two up to 64");
   if ((k == "approx_num" || k == "approx_den") && (value < 1 || value > (1 << 20)))
@@ -2001,8 +2257,9 @@ struct gst_server {
   gst_status error = GST_OK;
   std::string error_msg;
   size_t max_pending = 1 << 20;
-  uint32_t epoch_steps = 256;
   int waiters = 0;  // threads inside gst_wait
+  // device statistics (gst_server_stat), written by the server thread
+  std::atomic<int64_t> steps{0}, chunks{0}, busy_ns{0};
 };
 
 namespace {
@@ -2038,13 +2295,31 @@ void drop_expired(gst_server* s) {
 
 void server_loop(gst_server* s) {
   std::vector<int> fin;
+  Engine* e = s->eng.get();
   for (;;) {
+    int64_t chunk, depth, wait_us, batch;
+    {
+      std::lock_guard<std::mutex> dk(s->g->dev_mu);
+      chunk = s->g->opt.serve_chunk;
+      depth = s->g->opt.serve_depth;
+      wait_us = s->g->opt.serve_wait_us;
+      batch = s->g->opt.serve_batch;
+    }
     {
       std::unique_lock<std::mutex> lk(s->mu);
-      s->cv_new.wait(lk, [&] { return s->stop || !s->pending.empty() || s->eng->active() > 0; });
-      if (s->stop && s->pending.empty() && s->eng->active() == 0) break;
+      // sleep only while the device holds nothing of this server's
+      s->cv_new.wait(lk, [&] {
+        return s->stop || !s->pending.empty() || e->active() > 0 || e->cont_outstanding();
+      });
+      if (s->stop && s->pending.empty() && e->active() == 0 && !e->cont_outstanding()) break;
       drop_expired(s);
-      while (!s->pending.empty()) {
+      // While the device is busy, admission may hold requests back a little
+      // so that more of them share a group-distance job; an idle device
+      // takes a request at once.
+      bool admit = !s->pending.empty();
+      if (admit && e->active() > 0 && (int64_t)s->pending.size() < batch)
+        admit = now_ms() - s->pending.front().second.t_submit >= (double)wait_us * 1e-3;
+      while (admit && !s->pending.empty()) {
         std::deque<QueryReq*> pend;
         for (auto& pr : s->pending) {
           if (pend.size() >= (size_t)gst::kMaxSlots) break;
@@ -2052,7 +2327,11 @@ void server_loop(gst_server* s) {
           pend.push_back(&pr.second);
         }
         const std::vector<QueryReq*> offered(pend.begin(), pend.end());
-        const int n = s->eng->admit_front(pend);
+        int n;
+        {
+          std::lock_guard<std::mutex> dk(s->g->dev_mu);
+          n = e->admit_front(pend);
+        }
         if (n > 0) {
           // the admitted requests are the offered ones no longer pending (not
           // always a prefix: a job gathers requests of one bound from further
@@ -2068,7 +2347,7 @@ void server_loop(gst_server* s) {
           }
           s->pending.swap(rest);
         }
-        if (n > 0 || s->eng->active() > 0) break;
+        if (n > 0 || e->active() > 0) break;
         // the first pending query can never fit: answer it with an error result
         gst_result r;
         memset(&r, 0, sizeof(r));
@@ -2081,17 +2360,29 @@ void server_loop(gst_server* s) {
         s->cv_done.notify_all();
       }
     }
-    if (s->eng->active() == 0) continue;
-    uint32_t exit_after;
-    {
-      std::lock_guard<std::mutex> lk(s->mu);
-      exit_after = s->pending.empty() ? std::max(1, s->eng->active() / 8) : 1u;
-    }
+    if (e->active() == 0 && !e->cont_outstanding()) continue;
+    // One chunk of steps goes on the device; the snapshot of the chunk before
+    // it (or of this one, with nothing queued behind) is read back.
     gst_status st;
+    std::vector<std::pair<uint64_t, gst_result>> out;
     {
       std::lock_guard<std::mutex> dk(s->g->dev_mu);
-      st = s->eng->run_epoch(exit_after, s->epoch_steps, &fin);
-      if (st != GST_OK) s->eng->poison();  // the server stops; nothing runs on this engine again
+      st = e->cont_iter((int)chunk, (int)depth, &fin);
+      if (st != GST_OK) e->poison();  // the server stops; nothing runs on this engine again
+      for (int slot : fin) {
+        if (st != GST_OK) break;
+        uint64_t tag = 0;
+        gst_result r;
+        e->harvest(slot, &r, &tag, nullptr, 0, true);
+        out.push_back({tag, r});
+      }
+      if (st == GST_OK && !out.empty()) {
+        st = e->sync_harvest();  // the trees are in their buffers
+        if (st != GST_OK) e->poison();
+      }
+      s->steps = e->steps_total();
+      s->chunks = e->cont_chunks();
+      s->busy_ns = e->cont_busy_ns();
     }
     std::lock_guard<std::mutex> lk(s->mu);
     if (st != GST_OK) {
@@ -2101,14 +2392,11 @@ void server_loop(gst_server* s) {
       s->cv_done.notify_all();
       break;
     }
-    for (int slot : fin) {
-      uint64_t tag = 0;
-      gst_result r;
-      s->eng->harvest(slot, &r, &tag);
-      s->done[tag] = r;
-      s->known[tag] = 2;
+    for (auto& pr : out) {
+      s->done[pr.first] = pr.second;
+      s->known[pr.first] = 2;
     }
-    if (!fin.empty()) s->cv_done.notify_all();
+    if (!out.empty()) s->cv_done.notify_all();
   }
 }
 
@@ -2170,6 +2458,16 @@ gst_status gst_submit(gst_server* s, const gst_query* q, uint64_t* ticket) {
   return GST_OK;
 }
 
+gst_status gst_server_stat(gst_server* s, const char* key, int64_t* value) {
+  if (!s || !key || !value) return fail(GST_INVALID_ARG, "null argument");
+  const std::string k(key);
+  if (k == "steps") *value = s->steps.load();
+  else if (k == "chunks") *value = s->chunks.load();
+  else if (k == "busy_us") *value = s->busy_ns.load() / 1000;
+  else return fail(GST_INVALID_ARG, "unknown server statistic '%s'", key);
+  return GST_OK;
+}
+
 gst_status gst_poll(gst_server* s, uint64_t ticket, gst_result* out) {
   if (!s || !out) return fail(GST_INVALID_ARG, "null argument");
   std::lock_guard<std::mutex> lk(s->mu);
@@ -2220,6 +2518,7 @@ gst_status gst_cancel(gst_server* s, uint64_t ticket) {
     s->cv_done.notify_all();
   } else if (kn->second == 1) {
     // on the device: the plan kernel stops it at its next step
+    std::lock_guard<std::mutex> dk(s->g->dev_mu);
     s->eng->cancel_tag(ticket);
   }
   return GST_OK;
@@ -2237,6 +2536,7 @@ gst_status gst_server_destroy(gst_server* s) {
       s->known[pr.first] = 2;
     }
     s->pending.clear();
+    std::lock_guard<std::mutex> dk(s->g->dev_mu);
     s->eng->cancel_all();
     s->cv_done.notify_all();
   }

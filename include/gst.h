@@ -223,7 +223,14 @@ gst_status gst_get_option(gst_graph *g, const char *key, int64_t *value);
 gst_status gst_query_bytes(gst_graph *g, int32_t k, int32_t D, int64_t *bytes);
 
 /* Server: a background thread keeps the graph resident and admits submitted
- * queries into the running engine as memory and slots free up. */
+ * queries into the running engine as memory and slots free up.  The engine
+ * steps continuously: between chunks of "serve_chunk" engine steps (option,
+ * default 2; "serve_depth" chunks, default 2, stay queued on the device) the
+ * thread uploads newly admitted queries, which the running step plan takes up
+ * at its next step, and copies finished answers out.  While the device is
+ * busy, admission may wait up to "serve_wait_us" microseconds for
+ * "serve_batch" pending requests, so that they share a group-distance job
+ * (defaults 10000 and 32; 0 and 1 admit every request at once). */
 gst_status gst_server_create(gst_graph *g, int64_t budget_bytes,
                              int32_t max_slots, gst_server **out);
 gst_status gst_submit(gst_server *s, const gst_query *q, uint64_t *ticket);
@@ -236,6 +243,10 @@ gst_status gst_wait(gst_server *s, uint64_t ticket, gst_result *out);
  * GST_CANCELLED unless it finished first.  No reference counterpart (the
  * reference has no request loop). */
 gst_status gst_cancel(gst_server *s, uint64_t ticket);
+/* Device statistics of the server: "steps" (engine steps run), "chunks"
+ * (chunks collected), "busy_us" (their summed device time).  No reference
+ * counterpart (monitoring of the request loop). */
+gst_status gst_server_stat(gst_server *s, const char *key, int64_t *value);
 /* Answers every queued request with GST_CANCELLED, cancels the running ones,
  * wakes every gst_wait and returns once they have returned; no call on s may
  * start after it. */

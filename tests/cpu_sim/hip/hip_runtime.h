@@ -184,6 +184,7 @@ hipError_t hipStreamQuery(hipStream_t);
 hipError_t hipEventCreate(hipEvent_t*);
 hipError_t hipEventDestroy(hipEvent_t);
 hipError_t hipEventRecord(hipEvent_t, hipStream_t);
+hipError_t hipEventQuery(hipEvent_t);
 hipError_t hipEventElapsedTime(float*, hipEvent_t, hipEvent_t);
 hipError_t hipStreamBeginCapture(hipStream_t, int);
 hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t*);

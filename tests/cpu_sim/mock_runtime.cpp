@@ -47,6 +47,11 @@ hipError_t hipStreamQuery(hipStream_t) {
 hipError_t hipEventCreate(hipEvent_t* e) { *e = new ihipEvent{0}; return hipSuccess; }
 hipError_t hipEventDestroy(hipEvent_t e) { delete e; return hipSuccess; }
 hipError_t hipEventRecord(hipEvent_t e, hipStream_t) { e->t = now_ms(); return hipSuccess; }
+// events complete when recorded (work completes when issued), unless the
+// test hook holds every stream busy
+hipError_t hipEventQuery(hipEvent_t) {
+  return __atomic_load_n(&g_stream_busy, __ATOMIC_SEQ_CST) ? hipErrorNotReady : hipSuccess;
+}
 hipError_t hipEventElapsedTime(float* ms, hipEvent_t a, hipEvent_t b) { *ms = (float)(b->t - a->t); return hipSuccess; }
 // Stream capture: launches made while capturing are recorded (with their
 // arguments copied) instead of run; a graph launch runs them in order.
