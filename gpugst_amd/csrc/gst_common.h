@@ -86,6 +86,7 @@ enum Phase : int32_t {
   PH_WAITJ = 10, // query waiting for its group-distance job (no work items)
   PH_STAR = 11,    // job: star bound of every member query
   PH_COPY = 12,    // hop job: height level starts as a copy of height level - 1
+  PH_JLIVE = 13,   // job: live bits of every member query (one pass over its rows)
   PH_CACHEIN = 14, // job: copy cached group-distance columns into its rows
   PH_CACHEOUT = 15 // job: copy its new labels' distances out into cache columns
 };
@@ -189,7 +190,8 @@ struct alignas(64) Slot {
   int32_t phase, level;
   uint32_t step, in_step;
   uint32_t n_work, n_clr;   // this step's phase items and CLEAR items
-  int32_t cls, pad2;        // work class of the phase items
+  int32_t cls;              // work class of the phase items
+  uint32_t n_hv;            // job STAR: heavy items beside the lean ones (a route table per member)
   uint64_t app[3];      // (count << 32) | arcs of the three frontier buffers
   uint32_t best;        // upper bound on the optimum (cost of a real walk-tree)
   uint32_t status;

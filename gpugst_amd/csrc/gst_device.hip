@@ -1182,6 +1182,146 @@ __device__ void item_live(Slot* sp, uint32_t tile, uint64_t V, WaveLDS* L) {
   }
 }
 
+// lower_bound(Crow, k, W - 1, R) with the group distances c[0..k) and the full
+// set's route entries Rf[i * kMaxK + j] already in registers.
+__device__ __forceinline__ uint32_t lb_full_regs(const uint32_t* c, int k, const uint32_t* Rf) {
+  uint32_t lb1 = 0;
+#pragma unroll
+  for (int i = 0; i < kMaxK; i++)
+    if (i < k) lb1 = c[i] > lb1 ? c[i] : lb1;
+  if (lb1 >= kInf) return kInf;
+  if (k >= 2) {
+    uint32_t tour = 3u * kInf;
+#pragma unroll
+    for (int i = 0; i < kMaxK; i++) {
+#pragma unroll
+      for (int j = 0; j < kMaxK; j++) {
+        if (i >= k || j >= k || i == j) continue;
+        const uint32_t t = c[i] + Rf[i * kMaxK + j] + c[j];
+        tour = t < tour ? t : tour;
+      }
+    }
+    const uint32_t lb2 = (tour + 1u) >> 1;
+    lb1 = lb2 > lb1 ? lb2 : lb1;
+  }
+  return sat(lb1);
+}
+
+// Job LIVE: the live test of item_live for every member query of a job in one
+// pass over the job's rows.  A tile of kJLiveTV vertices; one lane per member
+// (64 members at a time) holds its query's full-set route entries, bound and
+// row offset, and the wave walks the tile's vertices, so the members read
+// their labels side by side from the same row (each row is fetched once for
+// the whole job instead of a line per member and vertex).  A lane gathers the
+// live bits of 32 vertices into a word and stores it into its query's live
+// bits (the tile owns those words; the query's FILL cleared them).  The
+// query's LIVE then lists its live vertices and initialises their T segments
+// (item_live2).  Progressive members also get the minima prog_bounds keeps.
+constexpr uint32_t kJLiveTV = 256;
+constexpr int kJLiveBatch = 2;  // vertices whose labels are read before the tests
+
+__device__ void item_jlive(Slot* slots, Slot* sp, uint32_t tile, uint64_t V) {
+  const int lane = lane_id();
+  const uint64_t v0 = (uint64_t)tile * kJLiveTV;
+  const uint64_t v1 = v0 + kJLiveTV < V ? v0 + kJLiveTV : V;
+  const int nmem = sp->nmem;
+  for (int mg = 0; mg < nmem; mg += 64) {
+    const int m = mg + lane;
+    const bool has = m < nmem;
+    Slot* q = has ? &slots[sp->jmem[m]] : nullptr;
+    const int k = has ? q->k : 0;
+    uint32_t Rf[kMaxK * kMaxK];
+    g_cu32* Rq = has ? (g_cu32*)q->R + (uint64_t)((1u << k) - 1u) * (uint64_t)(k * k) : nullptr;
+#pragma unroll
+    for (int i = 0; i < kMaxK; i++)
+#pragma unroll
+      for (int j = 0; j < kMaxK; j++) Rf[i * kMaxK + j] = (i < k && j < k) ? Rq[i * k + j] : kInf;
+    const uint32_t bst = has ? __hip_atomic_load(&q->best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : kInf;
+    const uint32_t lim = bst < kInf ? bst : kInf - 1;
+    const bool prog = has && q->anum != 0;
+    // A member's labels sit at offset lab in every row of the job (rows of
+    // rs words per vertex, heights Lp words apart): the row base is the same
+    // for the whole wave, the lane adds its offset for height D (bound) and
+    // floor(D / 2) (star).
+    const uint64_t Lp = (uint64_t)sp->Lp, rs = (uint64_t)sp->ND * Lp;
+    const uint32_t co = has ? (uint32_t)q->lab + (uint32_t)(sp->D >= 0 ? (uint64_t)sp->D * Lp : 0) : 0u;
+    const uint32_t so = has ? (uint32_t)q->lab + (uint32_t)(sp->D >= 0 ? (uint64_t)(sp->D / 2) * Lp : 0) : 0u;
+    uint32_t lbmn = kInf;
+    uint64_t pkmn = ~0ull;
+    for (uint64_t w0 = v0; w0 < v1; w0 += 32) {
+      const uint64_t w1 = w0 + 32 < v1 ? w0 + 32 : v1;
+      uint32_t word = 0;
+      for (uint64_t vb = w0; vb < w1; vb += kJLiveBatch) {
+        uint32_t c[kJLiveBatch][kMaxK];
+#pragma unroll
+        for (int b = 0; b < kJLiveBatch; b++) {
+          const bool ok = has && vb + b < w1;
+          g_cu32* row = (g_cu32*)sp->dist + (vb + b) * rs;
+#pragma unroll
+          for (int i = 0; i < kMaxK; i++) c[b][i] = (ok && i < k) ? row[co + (uint32_t)i] : kInf;
+        }
+#pragma unroll
+        for (int b = 0; b < kJLiveBatch; b++) {
+          if (!has || vb + b >= w1) continue;
+          const uint32_t lb = lb_full_regs(c[b], k, Rf);
+          if (lb <= lim) word |= 1u << (uint32_t)(vb + b - w0);
+          if (prog) {
+            lbmn = lb < lbmn ? lb : lbmn;
+            g_cu32* row = (g_cu32*)sp->dist + (vb + b) * rs;
+            uint32_t st = 0;
+            for (int i = 0; i < k; i++) st = sat(st + row[so + (uint32_t)i]);
+            const uint64_t pk = ((uint64_t)st << 32) | (vb + b);
+            if (st < kInf && pk < pkmn) pkmn = pk;
+          }
+        }
+      }
+      if (word) q->lvb[w0 >> 5] = word;
+    }
+    if (prog) {
+      if (lbmn < kInf) atomicMin(&q->lbmin, lbmn);
+      if (pkmn != ~0ull) atomicMin((unsigned long long*)&q->star_pk, (unsigned long long)pkmn);
+    }
+  }
+}
+
+// LIVE of a query fed by a group-distance job, whose job has set its live
+// bits (item_jlive): a tile of kLive2TV vertices, one 32-bit word per lane,
+// lists the live vertices (one atomic per tile) and initialises their T
+// segments.
+constexpr uint32_t kLive2TV = 2048;
+
+__device__ void item_live2(Slot* sp, uint32_t tile, uint64_t V, WaveLDS* L) {
+  const int lane = lane_id();
+  const uint64_t nw = (V + 31) / 32;
+  const uint64_t wi = (uint64_t)tile * (kLive2TV / 32) + (uint64_t)lane;
+  uint32_t word = wi < nw ? sp->lvb[wi] : 0u;
+  const uint32_t cnt = (uint32_t)__popc(word);
+  const uint32_t excl = wave_excl_scan(cnt);
+  const uint32_t total = __shfl(excl + cnt, 63, 64);
+  if (total == 0) return;
+  uint32_t base = 0;
+  if (lane == 0) base = atomicAdd(&sp->nlive, total);
+  base = __shfl(base, 0, 64);
+  uint32_t* ids = (uint32_t*)L->stage;  // kLive2TV ids at most
+  uint32_t o = excl;
+  while (word) {
+    const uint32_t b = (uint32_t)(__ffs(word) - 1);
+    word &= word - 1u;
+    const uint32_t v = (uint32_t)(wi * 32u + b);
+    ids[o] = v;
+    sp->live[base + o] = v;
+    o++;
+  }
+  wave_sync();
+  const uint32_t rowlen = (uint32_t)(sp->W * sp->NS);
+  for (uint32_t t = (uint32_t)lane; t < total * rowlen; t += 64) {
+    const uint32_t jj = t / rowlen;
+    // not read again before the merges above level 2
+    store_stream(&sp->T[(uint64_t)ids[jj] * rowlen + (t - jj * rowlen)], kInfState);
+  }
+  wave_sync();
+}
+
 // Level-L merge: T(v,p,nd) for every |p| = L subset p and height nd of every
 // live vertex, from the final lower levels.  The W * NS segments of up to VPW
 // live vertices are staged in LDS; the wave's lanes then take the (height,
@@ -1737,6 +1877,10 @@ __device__ void complete_step(Slot* slots, Slot* s, Ctrl* ctrl) {
         }
         break;
       case PH_STAR:
+        // the members' route tables were built beside the STAR tiles
+        s->phase = PH_JLIVE;
+        break;
+      case PH_JLIVE:
         for (int m = 0; m < s->nmem; m++) {
           Slot* q = &slots[s->jmem[m]];
           q->vsrc = s->vsrc;
@@ -1821,6 +1965,7 @@ __device__ uint32_t plan_step(Slot* s, uint64_t V) {
   uint32_t work = 0;
   int cls = kClsLean;
   const bool job = s->kind == KIND_JOB;
+  s->n_hv = 0;
   switch (s->phase) {
     case PH_FILL: {
       uint64_t nA, nM, nF;
@@ -1840,13 +1985,23 @@ __device__ uint32_t plan_step(Slot* s, uint64_t V) {
       if (!job) cls = kClsHeavy;  // the per-query grow carries the lower-bound code
       break;
     case PH_STAR:
+      work = ceil_div(V, kStarTV);
+      s->n_hv = (uint32_t)s->nmem;  // heavy: the route table of each member
+      break;
     case PH_COPY: work = ceil_div(V, kStarTV); break;
+    // the route entries of 64 members stay in registers: the heavy kernel
+    // has them, the job rounds' kernel does not
+    case PH_JLIVE: work = ceil_div(V, kJLiveTV); cls = kClsHeavy; break;
     // cache copies run in the heavy kernel: they would push the lean one
     // (job rounds) over its register budget
     case PH_CACHEIN:
     case PH_CACHEOUT: work = ceil_div(V, kStarTV); cls = kClsHeavy; break;
     case PH_DIST: work = (s->S1 ? ceil_div(V, kDistTV) : 0u) + 1u; cls = kClsHeavy; break;
-    case PH_LIVE: work = ceil_div(s->vsrc ? (uint64_t)s->nvsrc : V, kDistTV); cls = kClsHeavy; break;
+    case PH_LIVE:
+      // a job-fed query lists the live bits its job set
+      work = s->job >= 0 ? ceil_div(V, kLive2TV) : ceil_div(s->vsrc ? (uint64_t)s->nvsrc : V, kDistTV);
+      cls = kClsHeavy;
+      break;
     case PH_MERGE: work = ceil_div(s->level >= 3 ? s->nact : s->nlive, s->mtv); cls = kClsHeavy; break;
     case PH_FINAL: work = ceil_div(s->k >= 3 ? s->nact : s->nlive, s->mtv); cls = kClsHeavy; break;
     case PH_RECON: work = 1; cls = kClsHeavy; break;
@@ -1857,7 +2012,7 @@ __device__ uint32_t plan_step(Slot* s, uint64_t V) {
   s->n_work = work;
   s->cls = cls;
   s->n_clr = ceil_div((uint32_t)(s->app[(r + 2) % 3] >> 32), kClearPer);
-  return work + s->n_clr;
+  return work + s->n_clr + s->n_hv;
 }
 
 // Run by the one-block plan kernel between steps.
@@ -1904,14 +2059,15 @@ __device__ void phase_b(Slot* slots, Ctrl* ctrl, uint64_t V, uint32_t (*sh_items
     Slot* s = &slots[i];
     // a query whose job has finished leaves the wait (also on the last step
     // of an epoch, so that the job can be released by the host)
-    if (s->phase == PH_WAITJ && s->jdone) s->phase = PH_DIST;
+    // (its route table and live bits come from the job: straight to LIVE)
+    if (s->phase == PH_WAITJ && s->jdone) s->phase = PH_LIVE;
     uint32_t lean = 0, heavy = 0;
     if (!leave && s->phase != PH_FREE && s->phase != PH_DONE) {
       const uint32_t items = plan_step(s, V);
       s->in_step = items ? 1u : 0u;
       if (items) {
         lean = s->n_clr + (s->cls == kClsLean ? s->n_work : 0u);
-        heavy = s->cls == kClsHeavy ? s->n_work : 0u;
+        heavy = (s->cls == kClsHeavy ? s->n_work : 0u) + s->n_hv;
       }
     }
     sh_items[kClsLean][i] = lean;
@@ -2041,7 +2197,12 @@ __device__ void step_body(const GraphDev& g, Slot* slots, Ctrl* ctrl, LDS* L) {
             else item_dist(g, sp, local);
             break;
           case PH_ROUND: item_round(g, sp, local, &((WaveLDS*)L)->f); break;
-          case PH_LIVE: item_live(sp, local, V, (WaveLDS*)L); break;
+          case PH_LIVE:
+            if (sp->job >= 0) item_live2(sp, local, V, (WaveLDS*)L);
+            else item_live(sp, local, V, (WaveLDS*)L);
+            break;
+          case PH_STAR: item_vvr(g, &slots[sp->jmem[local]], (WaveLDS*)L); break;  // job: a member's tables
+          case PH_JLIVE: item_jlive(slots, sp, local, V); break;
           case PH_MERGE: item_merge(g, sp, local, (WaveLDS*)L); break;
           case PH_FINAL: item_final(g, sp, local, (WaveLDS*)L); break;
           case PH_RECON: item_recon(g, sp); break;
