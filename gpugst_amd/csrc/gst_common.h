@@ -167,9 +167,8 @@ struct alignas(64) Slot {
   uint32_t* dist;       // job: V * ND * Lp distances (per vertex, one row of Lp labels per height)
   int32_t* lgrp;        // job: group id of each label
   uint32_t* jmem;       // job: member slot indices
-  uint32_t* vsrc;       // vertices LIVE examines (null: all V); job: the vertices it reached
-  uint32_t nvsrc;       // entries of vsrc
-  uint32_t pad3;
+  uint64_t spare0;      // unused (keeps the fields below at their offsets)
+  uint32_t spare1, pad3;
   uint32_t* live;       // V: vertices that can lie on a tree cheaper than the bound
   uint32_t* lvb;        // V bits: live flag per vertex
   uint32_t* hold;       // V bits: the vertex holds a finite state with |p| >= 2

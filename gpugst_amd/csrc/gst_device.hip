@@ -422,7 +422,11 @@ __device__ __forceinline__ void fill_regions(const Slot* sp, uint64_t V, uint64_
   }
   const bool own = sp->S1 != nullptr;
   *nA = own ? V * (uint64_t)sp->ND * (uint64_t)sp->k : 0;
-  *nM = 3 * V;
+  // A query fed by a job starts at level 2, where only live vertices are ever
+  // marked dirty (merges run over live and active vertices, grows skip
+  // targets that are not live): its LIVE clears the masks of its live
+  // vertices instead.
+  *nM = sp->job >= 0 ? 0 : 3 * V;
   *nF = 2 * ((V + 63) / 64);  // live bits, then hold bits
 }
 
@@ -913,60 +917,51 @@ __device__ void item_jcopy(Slot* sp, uint32_t tile, uint64_t V) {
 }
 
 // Job STAR: the star bound min_v sum_i c_i(v) of every member query, from
-// its k labels of each distance row (one lane per member, at most two members
-// per lane since a job has at most 128; the wave reads each row once,
-// coalesced), and the list of vertices where some member has every group at
-// a finite distance.  A vertex off that list cannot be live for any member,
-// so the members' LIVE passes examine only the list.  A hop job's height d
-// holds the distance within d hops, so height floor(D/2) gives the star arms
-// and height D the reach.
+// its k labels of each distance row.  One lane per member (64 members at a
+// time): the row base is the same for the whole wave and each lane adds its
+// labels' offset, so a row is fetched once for the job; the labels of
+// kStarBatch vertices are read before they are summed.  A hop job's height d
+// holds the distance within d hops, so height floor(D/2) gives the star arms.
+constexpr int kStarBatch = 2;
+
 __device__ void item_jstar(Slot* slots, Slot* sp, uint32_t tile, uint64_t V) {
   const int lane = lane_id();
   const uint64_t v0 = (uint64_t)tile * kStarTV;
   const uint64_t v1 = v0 + kStarTV < V ? v0 + kStarTV : V;
-  const uint64_t Lp = (uint64_t)sp->Lp;
-  const int ND = sp->ND;
-  const uint64_t rs = (uint64_t)ND * Lp;
-  // hop: arms of at most floor(D/2) hops for the star, every height for reach
-  const int hs = sp->D >= 0 ? sp->D / 2 : 0;
+  const uint64_t Lp = (uint64_t)sp->Lp, rs = (uint64_t)sp->ND * Lp;
+  const uint64_t hs = sp->D >= 0 ? (uint64_t)(sp->D / 2) : 0u;
   const int nmem = sp->nmem;
-  Slot* q0 = lane < nmem ? &slots[sp->jmem[lane]] : nullptr;
-  Slot* q1 = lane + 64 < nmem ? &slots[sp->jmem[lane + 64]] : nullptr;
-  const int k0 = q0 ? q0->k : 0, k1 = q1 ? q1->k : 0;
-  const uint32_t lab0 = q0 ? (uint32_t)q0->lab : 0u, lab1 = q1 ? (uint32_t)q1->lab : 0u;
-  uint32_t star0 = kInf, star1 = kInf;
-  uint64_t reached = 0;
-  for (uint64_t v = v0; v < v1; v++) {
-    const uint32_t* row = sp->dist + v * rs;
-    bool fin = false;
-    for (int m = 0; m < 2; m++) {
-      const int km = m ? k1 : k0;
-      const uint32_t lab = m ? lab1 : lab0;
-      if (km == 0) continue;
-      uint32_t s = 0, sr = 0;
-      for (int i = 0; i < km; i++) {
-        s = sat(s + row[(uint64_t)hs * Lp + lab + i]);
-        sr = sat(sr + row[(uint64_t)(ND - 1) * Lp + lab + i]);
+  for (int mg = 0; mg < nmem; mg += 64) {
+    const int m = mg + lane;
+    const bool has = m < nmem;
+    Slot* q = has ? &slots[sp->jmem[m]] : nullptr;
+    const int k = has ? q->k : 0;
+    const uint32_t off = has ? (uint32_t)q->lab + (uint32_t)(hs * Lp) : 0u;
+    uint32_t star = kInf;
+    for (uint64_t vb = v0; vb < v1; vb += kStarBatch) {
+      uint32_t c[kStarBatch][kMaxK];
+#pragma unroll
+      for (int b = 0; b < kStarBatch; b++) {
+        const bool ok = has && vb + b < v1;
+        g_cu32* row = (g_cu32*)sp->dist + (vb + b) * rs;
+#pragma unroll
+        for (int i = 0; i < kMaxK; i++) c[b][i] = (ok && i < k) ? row[off + (uint32_t)i] : 0u;
       }
-      if (m) star1 = s < star1 ? s : star1;
-      else star0 = s < star0 ? s : star0;
-      fin = fin || sr < kInf;
+#pragma unroll
+      for (int b = 0; b < kStarBatch; b++) {
+        if (!has || vb + b >= v1) continue;
+        uint32_t s = 0;
+#pragma unroll
+        for (int i = 0; i < kMaxK; i++) s = sat(s + c[b][i]);
+        star = s < star ? s : star;
+      }
     }
-    if (__ballot(fin)) reached |= 1ull << (v - v0);
+    // shortest paths from one vertex to every group form a connected cover,
+    // so their sum bounds the optimum from above (for a hop query, paths of
+    // at most floor(D/2) hops: every two of them are within D hops of each
+    // other)
+    if (has && star < kInf) atomicMin(&q->best, star);
   }
-  // shortest paths from one vertex to every group form a connected cover, so
-  // their sum bounds the optimum from above (for a hop query, paths of at most
-  // floor(D/2) hops: every two of them are within D hops of each other)
-  if (q0 && star0 < kInf) atomicMin(&q0->best, star0);
-  if (q1 && star1 < kInf) atomicMin(&q1->best, star1);
-  const bool r = (reached >> lane) & 1ull;
-  const uint64_t ball = __ballot(r);
-  if (ball == 0) return;
-  const int leader = __ffsll((long long)ball) - 1;
-  uint32_t base = 0;
-  if (lane == leader) base = atomicAdd(&sp->nvsrc, (uint32_t)__popcll(ball));
-  base = __shfl(base, leader, 64);
-  if (r) sp->vsrc[base + __popcll(ball & __lanemask_lt())] = (uint32_t)(v0 + (uint64_t)lane);
 }
 
 // ------------------------------------------------------ per-query DIST ----
@@ -1144,15 +1139,14 @@ __device__ void item_live(Slot* sp, uint32_t tile, uint64_t V, WaveLDS* L) {
   const int k = sp->k, W = sp->W;
   const int rowlen = W * sp->NS;
   const uint32_t bst = __hip_atomic_load(&sp->best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  const uint64_t n = sp->vsrc ? (uint64_t)sp->nvsrc : V;  // candidate vertices
   const uint64_t v0 = (uint64_t)tile * kDistTV;
-  const uint64_t v1 = v0 + kDistTV < n ? v0 + kDistTV : n;
+  const uint64_t v1 = v0 + kDistTV < V ? v0 + kDistTV : V;
   const bool prog = sp->anum != 0;
   for (uint64_t vb = v0; vb < v1; vb += 64) {
     const uint64_t idx = vb + lane;
     uint64_t v = 0;
     bool lv = false;
-    if (idx < v1) v = sp->vsrc ? (uint64_t)sp->vsrc[idx] : idx;
+    if (idx < v1) v = idx;
     if (prog) {
       lv = prog_bounds(sp, idx < v1, v, bst < kInf ? bst : kInf - 1);
     } else if (idx < v1) {
@@ -1286,8 +1280,8 @@ __device__ void item_jlive(Slot* slots, Slot* sp, uint32_t tile, uint64_t V) {
 
 // LIVE of a query fed by a group-distance job, whose job has set its live
 // bits (item_jlive): a tile of kLive2TV vertices, one 32-bit word per lane,
-// lists the live vertices (one atomic per tile) and initialises their T
-// segments.
+// lists the live vertices (one atomic per tile), initialises their T
+// segments and clears their dirty masks.
 constexpr uint32_t kLive2TV = 2048;
 
 __device__ void item_live2(Slot* sp, uint32_t tile, uint64_t V, WaveLDS* L) {
@@ -1318,6 +1312,11 @@ __device__ void item_live2(Slot* sp, uint32_t tile, uint64_t V, WaveLDS* L) {
     const uint32_t jj = t / rowlen;
     // not read again before the merges above level 2
     store_stream(&sp->T[(uint64_t)ids[jj] * rowlen + (t - jj * rowlen)], kInfState);
+  }
+  // the three dirty masks of each live vertex (FILL left them alone)
+  for (uint32_t t = (uint32_t)lane; t < total * 3u; t += 64) {
+    const uint32_t jj = t / 3u;
+    sp->mask[(uint64_t)(t - jj * 3u) * V + ids[jj]] = 0ull;
   }
   wave_sync();
 }
@@ -1881,12 +1880,7 @@ __device__ void complete_step(Slot* slots, Slot* s, Ctrl* ctrl) {
         s->phase = PH_JLIVE;
         break;
       case PH_JLIVE:
-        for (int m = 0; m < s->nmem; m++) {
-          Slot* q = &slots[s->jmem[m]];
-          q->vsrc = s->vsrc;
-          q->nvsrc = s->nvsrc;
-          q->jdone = 1u;
-        }
+        for (int m = 0; m < s->nmem; m++) slots[s->jmem[m]].jdone = 1u;
         s->phase = PH_DONE;
         break;
       default:
@@ -1999,7 +1993,7 @@ __device__ uint32_t plan_step(Slot* s, uint64_t V) {
     case PH_DIST: work = (s->S1 ? ceil_div(V, kDistTV) : 0u) + 1u; cls = kClsHeavy; break;
     case PH_LIVE:
       // a job-fed query lists the live bits its job set
-      work = s->job >= 0 ? ceil_div(V, kLive2TV) : ceil_div(s->vsrc ? (uint64_t)s->nvsrc : V, kDistTV);
+      work = s->job >= 0 ? ceil_div(V, kLive2TV) : ceil_div(V, kDistTV);
       cls = kClsHeavy;
       break;
     case PH_MERGE: work = ceil_div(s->level >= 3 ? s->nact : s->nlive, s->mtv); cls = kClsHeavy; break;

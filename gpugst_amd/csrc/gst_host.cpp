@@ -268,7 +268,7 @@ Layout layout_for(const gst_graph* g, int k, int D) { return layout_for(g, k, D,
 
 // Group-distance job over L labels.
 struct JobLayout {
-  size_t dist, mask, list, loff, istart, lgrp, jmem, reach, ccol, hA, total;
+  size_t dist, mask, list, loff, istart, lgrp, jmem, ccol, hA, total;
   int Lp, G, chunk;
   uint32_t istride;
 };
@@ -297,7 +297,6 @@ JobLayout job_layout(const gst_graph* g, int L, int nmem, int D) {
   J.istart = off; off += align256(3 * (size_t)J.istride * 4);
   J.lgrp = off;   off += align256((size_t)L * 4);
   J.jmem = off;   off += align256((size_t)nmem * 4);
-  J.reach = off;  off += align256(V * 4);
   J.ccol = off;   off += align256((size_t)L * 2 * sizeof(void*));
   J.hA = off;     off += align256((size_t)J.Lp * gst::kBbHubs * 4);
   J.total = off;
@@ -692,7 +691,6 @@ class Engine {
     d.istart = (uint32_t*)(base + J.istart);
     d.lgrp = (int32_t*)(base + J.lgrp);
     d.jmem = (uint32_t*)(base + J.jmem);
-    d.vsrc = (uint32_t*)(base + J.reach);
     d.phase = gst::PH_FILL;
     // label -> group and member lists go up with the next epoch's uploads
     std::vector<int32_t> lg;
