@@ -83,9 +83,10 @@ def lean_bytes(g, res_list, V, ks, ND=1):
                       improved chunk 16 B (write back)
       job COPY        a hop job's heights 1..D: 4 B read + 4 B written per
                       (vertex, label, height)
-      job STAR        4 B per (vertex, label) read, from one row (unbounded)
-                      or two (hop: heights floor(D/2) and D)
-      query FILL      24 B per vertex (3 masks) + 1/4 B live and hold bits"""
+      job STAR        4 B per (vertex, label) read, from the row of height
+                      floor(D/2) (0 unbounded)
+      query FILL      1/4 B per vertex (live and hold bits); a query fed by a
+                      job clears only its live vertices' masks, in LIVE"""
     labels = int(sum(ks))
     jobs = max(1, g.get_option("batch_jobs"))
     lp = labels + 3 * jobs
@@ -96,24 +97,29 @@ def lean_bytes(g, res_list, V, ks, ND=1):
     b = 0 if hub_fill(g, ND) else job_fill_bytes(g, V, ks, ND)
     b += 36 * ent + 8 * arcs + 16 * (relax // 4) + 16 * impr
     b += 8 * V * lp * (ND - 1)
-    b += 4 * V * lp * (1 if ND == 1 else 2)
-    b += (24 * V + V // 4) * len(res_list)
+    b += 4 * V * lp
+    b += (V // 4) * len(res_list)
     return b
 
 
 def heavy_bytes(g, res_list, V, ks, ND=1):
-    """Algorithmic bytes of the heavy kernel's items, per query: the live test
-    (C row, 4 B per vertex and group), the T segments of live vertices written
-    once (8 B per state; NS = ceil(D/2) + 1 = ND // 2 + 1 state heights for a
-    hop query, 1 otherwise), the level-2 merge reading the
-    singletons (8 B per vertex, group and state height) and C rows (4 B per
-    vertex, group and row) of every live vertex, the later merges and FINAL
-    reading the T segments of the active vertices (8 B per state and pass),
-    and the level >= 2 grow rounds (24 B per frontier entry, 8 B per arc,
-    16 B per relaxation candidate, 8 B per improvement) -- the counters of a
-    query fed by a job hold only these rounds; plus the FILL of unbounded jobs
-    with hub bounds (job_fill_bytes)."""
-    total = job_fill_bytes(g, V, ks, ND) if hub_fill(g, ND) else 0
+    """Algorithmic bytes of the heavy kernel's items for a batch of job-fed
+    queries: the jobs' live test (each job's rows of height D read once, 4 B
+    per vertex and label) and, per query, its LIVE (its live bits, V / 8 B,
+    then per live vertex 4 B of list, 24 B of dirty masks cleared and its T
+    segment written once, 8 B per state; NS = ceil(D/2) + 1 = ND // 2 + 1
+    state heights for a hop query, 1 otherwise), the level-2 merge reading
+    the singletons (8 B per vertex, group and state height) and C rows (4 B
+    per vertex, group and row) of every live vertex, the later merges and
+    FINAL reading the T segments of the active vertices (8 B per state and
+    pass), and the level >= 2 grow rounds (24 B per frontier entry, 8 B per
+    arc, 16 B per relaxation candidate, 8 B per improvement) -- the counters
+    of a query fed by a job hold only these rounds; plus the FILL of
+    unbounded jobs with hub bounds (job_fill_bytes)."""
+    jobs = max(1, g.get_option("batch_jobs"))
+    total = 4 * V * (int(sum(ks)) + 3 * jobs)
+    if hub_fill(g, ND):
+        total += job_fill_bytes(g, V, ks, ND)
     for r, k in zip(res_list, ks):
         W = 1 << k
         c = r.counters
@@ -121,7 +127,7 @@ def heavy_bytes(g, res_list, V, ks, ND=1):
         na = c.get("active_vertices", 0)
         passes = max(k - 3, 0) + 1  # merges of levels 3..k-1, then FINAL
         NS = ND // 2 + 1
-        total += 4 * V * k + nl * W * NS * 8 + nl * k * (8 * NS + 4 * ND) + na * W * NS * 8 * passes
+        total += V // 8 + nl * (W * NS * 8 + 28) + nl * k * (8 * NS + 4 * ND) + na * W * NS * 8 * passes
         total += 24 * c["frontier_entries"] + 8 * c["arcs"] + 16 * c["relaxations"] + 8 * c["improvements"]
     return total
 

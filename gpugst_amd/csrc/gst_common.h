@@ -64,6 +64,7 @@ constexpr uint32_t kStarTV = 64;        // vertices per job STAR item
 // hub vertices (one 256-byte row per vertex) prove arcs redundant.
 constexpr int kBbHubs = 64;
 constexpr uint32_t kBbArcs = 256;       // arcs per backbone work item
+constexpr uint32_t kHub16Inf = 0x7FFF;  // "no bound" in the 16-bit hub distances
 
 // work classes: each has its own kernel, so the arc-parallel items do not run
 // at the register budget (and occupancy) of the merge items
@@ -118,6 +119,10 @@ struct GraphDev {
   // exact distances d(h, v) from the kBbHubs hub vertices of the backbone
   // build, V rows of kBbHubs (null: none)
   const uint32_t* hubd;
+  // the same distances as 16-bit words, kHub16Inf for unreached, when every
+  // finite one is below kHub16Inf / 2 (so two of them add without overflow);
+  // null otherwise
+  const uint16_t* hubd16;
   // vertices by hub reach: the n_hubr vertices some hub reaches, then the rest
   // (jobs compute hub bounds for the first ones only; the others' are
   // infinite)
@@ -225,6 +230,9 @@ struct alignas(64) Slot {
   // from hub h to the nearest member of label l's group (null: its rows start
   // at infinity)
   const uint32_t* hA;
+  // 1: the job's hub FILL uses GraphDev::hubd16 (two labels per lane in
+  // packed 16-bit arithmetic)
+  uint32_t hub16;
 };
 
 struct alignas(64) Ctrl {

@@ -475,6 +475,67 @@ __device__ void item_ubfill(const GraphDev& g, Slot* sp, uint32_t tile) {
   }
 }
 
+// item_ubfill in packed 16-bit arithmetic (graphs whose hub distances are all
+// below kHub16Inf / 2, GraphDev::hubd16): a lane takes labels l and l + 64 of
+// a group of 128, one per half of a packed word, so each packed add and min
+// serves two labels.  Bounds and distances are clamped to kHub16Inf; two
+// finite values add below it and a sum at or above it means "no bound" (the
+// row starts at infinity, which is always a valid start).
+typedef unsigned short us2 __attribute__((vector_size(4)));
+
+__device__ __forceinline__ us2 us2_min(us2 a, us2 b) {
+#ifdef GST_CPU_SIM
+  us2 r;
+  r[0] = a[0] < b[0] ? a[0] : b[0];
+  r[1] = a[1] < b[1] ? a[1] : b[1];
+  return r;
+#else
+  return __builtin_elementwise_min(a, b);
+#endif
+}
+
+__device__ __forceinline__ uint32_t ub16_groups(const Slot* sp) { return ((uint32_t)sp->Lp + 127u) / 128u; }
+
+__device__ void item_ubfill16(const GraphDev& g, Slot* sp, uint32_t tile) {
+  const int lane = lane_id();
+  const uint32_t ngrp = ub16_groups(sp);
+  const uint32_t l0 = (tile % ngrp) * 128u + (uint32_t)lane, l1 = l0 + 64u;
+  const uint64_t i0 = (uint64_t)(tile / ngrp) * kUbTV;
+  const uint64_t i1 = i0 + kUbTV < (uint64_t)g.n_hubr ? i0 + kUbTV : (uint64_t)g.n_hubr;
+  const uint32_t Lp = (uint32_t)sp->Lp;
+  const bool lv0 = l0 < Lp, lv1 = l1 < Lp;
+  us2 a[kBbHubs];
+  const bchunk* r0 = (const bchunk*)(sp->hA + (uint64_t)(lv0 ? l0 : 0u) * kBbHubs);
+  const bchunk* r1 = (const bchunk*)(sp->hA + (uint64_t)(lv1 ? l1 : 0u) * kBbHubs);
+#pragma unroll
+  for (int j = 0; j < kBbHubs / 4; j++) {
+    const bchunk x0 = r0[j], x1 = r1[j];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const uint32_t b0 = lv0 && x0[q] < kHub16Inf ? x0[q] : kHub16Inf;
+      const uint32_t b1 = lv1 && x1[q] < kHub16Inf ? x1[q] : kHub16Inf;
+      a[4 * j + q] = us2{(unsigned short)b0, (unsigned short)b1};
+    }
+  }
+  const GST_AS_CONST uint32_t* hubd = (const GST_AS_CONST uint32_t*)g.hubd16;  // hub pairs
+  const GST_AS_CONST uint32_t* hubv = (const GST_AS_CONST uint32_t*)g.hubv;
+  g_u32* dist = (g_u32*)sp->dist;
+  for (uint64_t i = i0; i < i1; i++) {
+    const uint64_t v = (uint64_t)hubv[i];
+    const GST_AS_CONST uint32_t* hr = hubd + v * (kBbHubs / 2);
+    us2 u = us2{(unsigned short)kHub16Inf, (unsigned short)kHub16Inf};
+#pragma unroll
+    for (int h = 0; h < kBbHubs / 2; h++) {
+      const uint32_t pr = hr[h];
+      const unsigned short d0 = (unsigned short)(pr & 0xFFFFu), d1 = (unsigned short)(pr >> 16);
+      u = us2_min(u, a[2 * h] + us2{d0, d0});
+      u = us2_min(u, a[2 * h + 1] + us2{d1, d1});
+    }
+    if (lv0) store_stream(&dist[v * Lp + l0], u[0] < kHub16Inf ? (uint32_t)u[0] : kInf);
+    if (lv1) store_stream(&dist[v * Lp + l1], u[1] < kHub16Inf ? (uint32_t)u[1] : kInf);
+  }
+}
+
 // The rows of 64 vertices no hub reaches: infinite for every label (one lane
 // per chunk, a row at a time).
 __device__ void item_ubfill_inf(const GraphDev& g, Slot* sp, uint32_t tile) {
@@ -519,9 +580,11 @@ __device__ void item_fill(Slot* sp, uint32_t idx, uint64_t V) {
 // FILL of an unbounded job with hub bounds (heavy kernel): the rows' items
 // first, then the masks'.
 __device__ void item_fill_hub(const GraphDev& g, Slot* sp, uint32_t idx, uint64_t V) {
-  const uint32_t nr = (uint32_t)(((uint64_t)g.n_hubr + kUbTV - 1u) / kUbTV) * ub_groups(sp);
+  const uint32_t nr =
+      (uint32_t)(((uint64_t)g.n_hubr + kUbTV - 1u) / kUbTV) * (sp->hub16 ? ub16_groups(sp) : ub_groups(sp));
   const uint32_t nu = (uint32_t)((V - (uint64_t)g.n_hubr + 63u) / 64u);
-  if (idx < nr) item_ubfill(g, sp, idx);
+  if (idx < nr && sp->hub16) item_ubfill16(g, sp, idx);
+  else if (idx < nr) item_ubfill(g, sp, idx);
   else if (idx < nr + nu) item_ubfill_inf(g, sp, idx - nr);
   else item_fill(sp, idx - nr - nu, V);
 }
@@ -1968,7 +2031,7 @@ __device__ uint32_t plan_step(Slot* s, uint64_t V) {
       // hub bounds keep 64 distances per lane in registers: the heavy kernel
       // has them, the job rounds' kernel does not
       if (job && s->hA) {
-        work += ceil_div(s->nhubr, kUbTV) * ub_groups(s) + ceil_div(V - s->nhubr, 64u);
+        work += ceil_div(s->nhubr, kUbTV) * (s->hub16 ? ub16_groups(s) : ub_groups(s)) + ceil_div(V - s->nhubr, 64u);
         cls = kClsHeavy;
       }
       break;

@@ -139,6 +139,7 @@ struct Options {
   int64_t count_states = 0;  // count the finite DP states of every query (an extra pass)
   int64_t backbone = 1;  // unbounded queries and jobs admitted from now on run on the shortest-path backbone
   int64_t hub_bounds = 1;  // unbounded jobs admitted from now on start their rows at hub bounds
+  int64_t hub16 = 1;  // ... computed in packed 16-bit arithmetic where the graph's hub distances allow
   // server (continuous stepping): engine steps per chunk between two host
   // visits, chunks kept queued on the device, and how long / for how many
   // pending requests admission waits while the device is busy, so that more
@@ -176,6 +177,7 @@ struct gst_graph {
   // and on the host (the host forms each job's hub bounds); null / empty when
   // the build did not run
   uint32_t* d_hubd = nullptr;
+  uint16_t* d_hubd16 = nullptr;  // the 16-bit copy (GraphDev::hubd16), or null
   std::vector<uint32_t> h_hubd;
   // vertices some hub reaches, then the others; n_hubr of the first kind
   uint32_t* d_hubv = nullptr;
@@ -196,6 +198,7 @@ struct gst_graph {
     g.gptr = d_gptr;
     g.gmem = d_gmem;
     g.hubd = d_hubd;
+    g.hubd16 = d_hubd16;
     g.hubv = d_hubv;
     g.n_hubr = n_hubr;
     g.V = V;
@@ -713,6 +716,7 @@ class Engine {
       }
       d.hA = (const uint32_t*)(base + J.hA);
       d.nhubr = (uint32_t)g_->n_hubr;
+      d.hub16 = g_->d_hubd16 != nullptr && g_->opt.hub16 != 0 ? 1u : 0u;
       pending_uploads_.push_back({(void*)d.hA, std::vector<char>((char*)hA.data(), (char*)(hA.data() + hA.size()))});
     }
     // group-distance cache (unbounded jobs): a label whose group has a valid
@@ -1738,6 +1742,31 @@ gst_status build_backbone(gst_graph* g, const int64_t* beg_pos, const std::vecto
         hipMemcpy(g->d_hubv, hv.data(), sizeof(uint32_t) * V, hipMemcpyHostToDevice) == hipSuccess) {
       g->d_hubd = dist;
       dist = nullptr;
+      // a 16-bit copy when every finite hub distance is below half of
+      // kHub16Inf: then a bound plus a distance never reaches it, and the
+      // jobs' hub FILL works on two labels per packed instruction
+      std::atomic<uint32_t> dmax{0};
+      const size_t nh = g->h_hubd.size();
+      parallel_for((int64_t)nh, [&](int64_t b, int64_t e) {
+        uint32_t m = 0;
+        for (int64_t i = b; i < e; i++)
+          if (g->h_hubd[i] < gst::kInf) m = std::max(m, g->h_hubd[i]);
+        uint32_t cur = dmax.load();
+        while (m > cur && !dmax.compare_exchange_weak(cur, m)) {}
+      });
+      if (dmax.load() < gst::kHub16Inf / 2) {
+        std::vector<uint16_t> h16(nh);
+        parallel_for((int64_t)nh, [&](int64_t b, int64_t e) {
+          for (int64_t i = b; i < e; i++)
+            h16[i] = (uint16_t)(g->h_hubd[i] < gst::kInf ? g->h_hubd[i] : gst::kHub16Inf);
+        });
+        if (hipMalloc(&g->d_hubd16, nh * 2) != hipSuccess ||
+            hipMemcpy(g->d_hubd16, h16.data(), nh * 2, hipMemcpyHostToDevice) != hipSuccess) {
+          (void)hipFree(g->d_hubd16);
+          (void)hipGetLastError();
+          g->d_hubd16 = nullptr;
+        }
+      }
     } else {
       (void)hipFree(g->d_hubv);
       (void)hipGetLastError();
@@ -1839,7 +1868,7 @@ gst_status build_backbone(gst_graph* g, const int64_t* beg_pos, const std::vecto
 // Device memory of a graph (its engines are gone already).
 void free_graph_memory(gst_graph* g) {
   for (void* p : {(void*)g->d_rowptr, (void*)g->d_col, (void*)g->d_wt, (void*)g->d_rev, (void*)g->d_gptr,
-                  (void*)g->d_gmem, (void*)g->d_hubd, (void*)g->d_hubv})
+                  (void*)g->d_gmem, (void*)g->d_hubd, (void*)g->d_hubd16, (void*)g->d_hubv})
     if (p) (void)hipFree(p);
 }
 
@@ -2028,6 +2057,7 @@ static int64_t* option_ref(gst_graph* g, const std::string& k) {
   if (k == "approx_den") return &g->opt.approx_den;
   if (k == "backbone") return &g->opt.backbone;
   if (k == "hub_bounds") return &g->opt.hub_bounds;
+  if (k == "hub16") return &g->opt.hub16;
   if (k == "serve_chunk") return &g->opt.serve_chunk;
   if (k == "serve_depth") return &g->opt.serve_depth;
   if (k == "serve_wait_us") return &g->opt.serve_wait_us;
@@ -2051,7 +2081,7 @@ gst_status gst_set_option(gst_graph* g, const char* key, int64_t value) {
   if (k == "chunk" && (value < 64 || value > (1 << 20))) return fail(GST_INVALID_ARG, "chunk outside [64, 2^20]");
   if (k == "edge_cap" && (value < 64 || value > (1 << 26))) return fail(GST_INVALID_ARG, "edge_cap outside [64, 2^26]");
   if ((k == "budget_mb" || k == "blocks_per_cu" || k == "timeout_ms" || k == "trace_slot" || k == "deadline_ms" ||
-       k == "count_states" || k == "cache_mb" || k == "backbone" || k == "hub_bounds") && value < 0)
+       k == "count_states" || k == "cache_mb" || k == "backbone" || k == "hub_bounds" || k == "hub16") && value < 0)
     return fail(GST_INVALID_ARG, "negative value");
   if (k == "block" && value != 2 && value != 4 && value != 8)
     return fail(GST_INVALID_ARG, "block (waves per block) must be 2, 4 or 8");
