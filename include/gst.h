@@ -205,7 +205,12 @@ void gst_result_free(gst_result *r);
  * "trace_slot" (with time_steps: the engine slot whose phase each step's
  * time is binned by; slot 0 is the query of a one-query call),
  * "backbone" (default 1: queries without a hop bound admitted from then on
- * run on the graph's shortest-path backbone; 0: on every arc).
+ * run on the graph's shortest-path backbone; 0: on every arc), "hub_bounds"
+ * (default 1: group-distance jobs without a hop bound start their rows at
+ * bounds through the backbone's hub vertices; 0: at infinity), "hub16"
+ * (default 1: those bounds in packed 16-bit arithmetic when the graph's hub
+ * distances allow; 0: 32-bit), "serve_chunk", "serve_depth",
+ * "serve_wait_us", "serve_batch" (the server, see gst_server_create).
  * Read-only: "num_cu", "backbone_arcs", "backbone_rounds" (hub rounds of
  * its build), "backbone_us" (build time), "occupancy", "occupancy_heavy", "grid",
  * "grid_heavy", "{plain,batch}_{kernel_ns,launches,steps,lean_ns,heavy_ns,
