@@ -348,6 +348,7 @@ def check_robust():
     srv = Server(g)
     ts = [srv.submit(x, D=d) for x, d in zip(mq, mD)]
     mixed = [srv.wait(t).cost for t in ts]
+    out["server_stats"] = [srv.stat("steps"), srv.stat("chunks")]
     srv.close()
     out["mix"] = mixed
     out["mix_ok"] = mixed == [O.dpbf(rp, col, wt, [groups[y] for y in x], D=d)[0] for x, d in zip(mq, mD)]

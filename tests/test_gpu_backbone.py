@@ -82,6 +82,26 @@ def test_rmat_costs_with_and_without_backbone(rmat):
         assert v["ok"], v
 
 
+def test_rmat_hub_bound_fills_agree(rmat):
+    """Unbounded jobs start their rows at hub bounds, filled in packed 16-bit
+    arithmetic (this graph's hub distances allow it), in 32-bit arithmetic,
+    or not at all (rows at infinity): the same costs, equal to the oracle."""
+    inst, g = rmat
+    qs = data.make_queries(32, 5, inst.meta["eligible"], 93)
+    costs = {}
+    for hb, h16 in ((1, 1), (1, 0), (0, 0)):
+        g.set_option("hub_bounds", hb)
+        g.set_option("hub16", h16)
+        costs[(hb, h16)] = [r.cost for r in g.solve_batch(list(qs))]
+        _say("hub_bounds", hb, "hub16", h16, "costs", costs[(hb, h16)][:6])
+    g.set_option("hub_bounds", 1)
+    g.set_option("hub16", 1)
+    assert costs[(1, 1)] == costs[(1, 0)] == costs[(0, 0)]
+    for i in range(2):
+        groups = [inst.group(x) for x in qs[i]]
+        assert costs[(1, 1)][i] == O.dpbf_pruned(inst.rowptr, inst.col, inst.wt, groups)[0]
+
+
 def test_c1_every_arc_vs_golden(golden_dir):
     ref = json.load(open(os.path.join(golden_dir, "c1_costs.json")))
     inst = data.make_config(1)

@@ -63,6 +63,29 @@ def test_mix_equals_alone():
     g.close()
 
 
+def test_admission_settings_and_stats():
+    """The server steps continuously; whether admission takes every request at
+    once or holds requests back to share a job, each answer equals the
+    one-query call, and the server reports its engine steps and device time."""
+    inst, qs, Ds = _instance()
+    g = Graph(inst.rowptr, inst.col, inst.wt)
+    g.set_groups(inst.gptr, inst.gmem)
+    alone = [g.solve(q, D=d).cost for q, d in zip(qs, Ds)]
+    for wait_us, batch, chunk in ((0, 1, 1), (20000, 8, 4)):
+        g.set_option("serve_wait_us", wait_us)
+        g.set_option("serve_batch", batch)
+        g.set_option("serve_chunk", chunk)
+        srv = Server(g)
+        tickets = [srv.submit(q, D=d) for q, d in zip(qs, Ds)]
+        mixed = [srv.wait(t).cost for t in tickets]
+        steps, chunks, busy = srv.stat("steps"), srv.stat("chunks"), srv.stat("busy_us")
+        srv.close()
+        print("wait_us", wait_us, "batch", batch, "steps", steps, "chunks", chunks, "busy_us", busy, flush=True)
+        assert mixed == alone
+        assert steps >= chunks > 0 and busy > 0
+    g.close()
+
+
 def test_poll_and_unknown_ticket():
     inst, qs, Ds = _instance()
     g = Graph(inst.rowptr, inst.col, inst.wt)
