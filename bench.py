@@ -55,7 +55,7 @@ def job_fill_bytes(g, V, ks, ND=1):
     bounds also reads each vertex's row of hub distances once (4 B per hub)."""
     labels = int(sum(ks))
     jobs = max(1, g.get_option("batch_jobs"))
-    lp = labels + 3 * jobs  # rows padded to whole 4-label chunks
+    lp = labels + (JOB_CHUNK_LABELS - 1) * jobs // 2  # rows padded to whole chunks (half a chunk on average)
     b = 4 * V * lp + 24 * V * jobs
     if hub_fill(g, ND):
         b += 4 * HUBS * V * jobs
@@ -63,6 +63,7 @@ def job_fill_bytes(g, V, ks, ND=1):
 
 
 HUBS = 64  # hub vertices of the shortest-path backbone build (kBbHubs)
+JOB_CHUNK_LABELS = 16  # labels per dirty chunk of a job row (kJobChunkLabels)
 
 
 def hub_fill(g, ND):
@@ -79,8 +80,8 @@ def lean_bytes(g, res_list, V, ks, ND=1):
                       with hub bounds)
       job ROUND       per frontier entry 24 B (list id, arc offset, mask, row
                       pointer) + 12 B clear; per arc 8 B (column + weight);
-                      per 4-label chunk relaxed 16 B (target chunk read); per
-                      improved chunk 16 B (write back)
+                      per label relaxed 4 B (target chunk read, 64 B per
+                      16-label chunk); per improved chunk 64 B (write back)
       job COPY        a hop job's heights 1..D: 4 B read + 4 B written per
                       (vertex, label, height)
       job STAR        4 B per (vertex, label) read, from the row of height
@@ -89,13 +90,13 @@ def lean_bytes(g, res_list, V, ks, ND=1):
                       job clears only its live vertices' masks, in LIVE"""
     labels = int(sum(ks))
     jobs = max(1, g.get_option("batch_jobs"))
-    lp = labels + 3 * jobs
+    lp = labels + (JOB_CHUNK_LABELS - 1) * jobs // 2
     ent = g.get_option("batch_job_entries")
     arcs = g.get_option("batch_job_arcs")
     relax = g.get_option("batch_job_relaxations")
     impr = g.get_option("batch_job_improvements")
     b = 0 if hub_fill(g, ND) else job_fill_bytes(g, V, ks, ND)
-    b += 36 * ent + 8 * arcs + 16 * (relax // 4) + 16 * impr
+    b += 36 * ent + 8 * arcs + 4 * relax + 4 * JOB_CHUNK_LABELS * impr
     b += 8 * V * lp * (ND - 1)
     b += 4 * V * lp
     b += (V // 4) * len(res_list)
@@ -117,7 +118,7 @@ def heavy_bytes(g, res_list, V, ks, ND=1):
     of a query fed by a job hold only these rounds; plus the FILL of
     unbounded jobs with hub bounds (job_fill_bytes)."""
     jobs = max(1, g.get_option("batch_jobs"))
-    total = 4 * V * (int(sum(ks)) + 3 * jobs)
+    total = 4 * V * (int(sum(ks)) + (JOB_CHUNK_LABELS - 1) * jobs // 2)
     if hub_fill(g, ND):
         total += job_fill_bytes(g, V, ks, ND)
     for r, k in zip(res_list, ks):

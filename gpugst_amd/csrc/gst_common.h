@@ -52,10 +52,12 @@ constexpr int kStageWords = 1152;       // per-wave LDS staging (u64): 64*ND row
 
 // Group-distance jobs: one multi-label SSSP per batch of unbounded queries.
 // Labels are (query, group) pairs; the distance row of a vertex holds all of
-// them (u32, padded to whole 4-label chunks), so an arc relaxation reads and
-// writes coalesced 16-byte chunks shared by every query of the job.
-constexpr int kJobChunkLabels = 4;     // labels per chunk of a job row (one dirty bit)
-constexpr int kJobLabels = 64 * kJobChunkLabels;  // labels per job: 64 chunks
+// them (u32, padded to whole 16-label chunks), so an arc relaxation reads and
+// writes coalesced 64-byte chunks (one HBM line each) shared by every query of
+// the job.  Swept 4 / 8 / 16 labels: coarser dirty bits relax more labels, but
+// each relaxation fetches a whole line instead of a quarter of one.
+constexpr int kJobChunkLabels = 16;    // labels per chunk of a job row (one dirty bit)
+constexpr int kJobLabels = 256;        // labels per job (at most 64 chunks)
 constexpr uint32_t kJobArcs = 512;      // arcs per job ROUND item (swept: 128 / 256 / 512 / 1024)
 constexpr int kJobChunksPerLane = 8;    // chunks one lane relaxes per arc (swept: 2 / 4 / 8)
 constexpr uint32_t kStarTV = 64;        // vertices per job STAR item

@@ -224,15 +224,13 @@ __device__ __forceinline__ g_cu32* crow(const Slot* sp, uint64_t v, int h) {
 //   ceil( min_{i != j in x} (c_i(u) + R[x][i][j] + c_j(u)) / 2 ) ).
 // The second term halves a closed walk from u through every group of x; R is
 // the cheapest group-to-group route visiting all of x from i to j.
-__device__ __forceinline__ uint32_t lower_bound(g_cu32* __restrict__ Crow, int k, uint32_t x,
-                                                g_cu32* __restrict__ R) {
-  uint32_t c[kMaxK];
+// lower_bound_c takes the vertex's group distances c[0..k) already read,
+// lower_bound reads them from its C row.
+__device__ __forceinline__ uint32_t lower_bound_c(const uint32_t* c, int k, uint32_t x, g_cu32* __restrict__ R) {
   uint32_t lb1 = 0;
 #pragma unroll
-  for (int i = 0; i < kMaxK; i++) {
-    c[i] = (i < k) ? Crow[i] : kInf;
+  for (int i = 0; i < kMaxK; i++)
     if (i < k && ((x >> i) & 1u)) lb1 = c[i] > lb1 ? c[i] : lb1;
-  }
   if (lb1 >= kInf) return kInf;
   if (__popc(x) >= 2) {
     g_cu32* Rx = R + (size_t)x * (uint32_t)(k * k);
@@ -251,6 +249,14 @@ __device__ __forceinline__ uint32_t lower_bound(g_cu32* __restrict__ Crow, int k
     lb1 = lb2 > lb1 ? lb2 : lb1;
   }
   return sat(lb1);
+}
+
+__device__ __forceinline__ uint32_t lower_bound(g_cu32* __restrict__ Crow, int k, uint32_t x,
+                                                g_cu32* __restrict__ R) {
+  uint32_t c[kMaxK];
+#pragma unroll
+  for (int i = 0; i < kMaxK; i++) c[i] = (i < k) ? Crow[i] : kInf;
+  return lower_bound_c(c, k, x, R);
 }
 
 // lower_bound(Crow, k, x, R) <= lim, decided with early exits: false as soon
@@ -2399,7 +2405,7 @@ __global__ __launch_bounds__(256) void gst_bb_test_kernel(BbArgs a, uint64_t n_i
 // Blocks of WPB waves (a schedule option); the claim loop is per wave, so the
 // block shape sets only how waves share a CU's LDS and how the grid fills it.
 template <int WPB>
-__global__ __launch_bounds__(WPB * 64) __attribute__((amdgpu_waves_per_eu(5))) void gst_lean_kernel(GraphDev g,
+__global__ __launch_bounds__(WPB * 64) __attribute__((amdgpu_waves_per_eu(4))) void gst_lean_kernel(GraphDev g,
                                                                                                 Slot* slots,
                                                                                                 Ctrl* ctrl) {
   __shared__ FrontLDS lds[WPB];
