@@ -1302,6 +1302,9 @@ __device__ void item_jlive(Slot* slots, Slot* sp, uint32_t tile, uint64_t V) {
     const uint32_t bst = has ? __hip_atomic_load(&q->best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : kInf;
     const uint32_t lim = bst < kInf ? bst : kInf - 1;
     const bool prog = has && q->anum != 0;
+    // an unbounded job's live test also takes each member's star (its labels
+    // are the bound row itself); a hop job's STAR pass has done that
+    const bool flat = sp->D < 0;
     // A member's labels sit at offset lab in every row of the job (rows of
     // rs words per vertex, heights Lp words apart): the row base is the same
     // for the whole wave, the lane adds its offset for height D (bound) and
@@ -1309,7 +1312,7 @@ __device__ void item_jlive(Slot* slots, Slot* sp, uint32_t tile, uint64_t V) {
     const uint64_t Lp = (uint64_t)sp->Lp, rs = (uint64_t)sp->ND * Lp;
     const uint32_t co = has ? (uint32_t)q->lab + (uint32_t)(sp->D >= 0 ? (uint64_t)sp->D * Lp : 0) : 0u;
     const uint32_t so = has ? (uint32_t)q->lab + (uint32_t)(sp->D >= 0 ? (uint64_t)(sp->D / 2) * Lp : 0) : 0u;
-    uint32_t lbmn = kInf;
+    uint32_t lbmn = kInf, smin = kInf;
     uint64_t pkmn = ~0ull;
     for (uint64_t w0 = v0; w0 < v1; w0 += 32) {
       const uint64_t w1 = w0 + 32 < v1 ? w0 + 32 : v1;
@@ -1328,11 +1331,19 @@ __device__ void item_jlive(Slot* slots, Slot* sp, uint32_t tile, uint64_t V) {
           if (!has || vb + b >= w1) continue;
           const uint32_t lb = lb_full_regs(c[b], k, Rf);
           if (lb <= lim) word |= 1u << (uint32_t)(vb + b - w0);
+          uint32_t st = 0;
+          if (flat) {
+#pragma unroll
+            for (int i = 0; i < kMaxK; i++)
+              if (i < k) st = sat(st + c[b][i]);
+            smin = st < smin ? st : smin;
+          }
           if (prog) {
             lbmn = lb < lbmn ? lb : lbmn;
-            g_cu32* row = (g_cu32*)sp->dist + (vb + b) * rs;
-            uint32_t st = 0;
-            for (int i = 0; i < k; i++) st = sat(st + row[so + (uint32_t)i]);
+            if (!flat) {
+              g_cu32* row = (g_cu32*)sp->dist + (vb + b) * rs;
+              for (int i = 0; i < k; i++) st = sat(st + row[so + (uint32_t)i]);
+            }
             const uint64_t pk = ((uint64_t)st << 32) | (vb + b);
             if (st < kInf && pk < pkmn) pkmn = pk;
           }
@@ -1340,6 +1351,9 @@ __device__ void item_jlive(Slot* slots, Slot* sp, uint32_t tile, uint64_t V) {
       }
       if (word) q->lvb[w0 >> 5] = word;
     }
+    // the star is the cost of a real walk-tree: shortest paths from one
+    // vertex to every group
+    if (has && smin < kInf) atomicMin(&q->best, smin);
     if (prog) {
       if (lbmn < kInf) atomicMin(&q->lbmin, lbmn);
       if (pkmn != ~0ull) atomicMin((unsigned long long*)&q->star_pk, (unsigned long long)pkmn);
@@ -1516,14 +1530,21 @@ __device__ void item_merge(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* 
       const uint32_t c = (uint32_t)(bestpk >> 32);
       if (c >= kInf) continue;
       // a state of height nd reaches the rest of its tree within D - nd hops
-      if (c + lower_bound(crow(sp, v, hop ? D - nd : 0), k, x, (g_cu32*)sp->R) > bst0) continue;
+      // (the staged C rows hold every height of the vertex)
+      {
+        const uint32_t* cb = cst + jj * ND * k + (hop ? (D - nd) * k : 0);
+        uint32_t cv[kMaxK];
+#pragma unroll
+        for (int i = 0; i < kMaxK; i++) cv[i] = i < k ? cb[i] : kInf;
+        if (c + lower_bound_c(cv, k, x, (g_cu32*)sp->R) > bst0) continue;
+      }
       sp->T[((uint64_t)v * NS + nd) * W + p] = bestpk;
       atomicOr((unsigned long long*)&L->f.wm[jj], 1ull << p);
       // upper bounds from complete walk-trees at v: this state plus one
       // shortest path per missing group (hop: paths of at most h hops, with
       // nd + h <= D and, for two or more paths, 2h <= D) ...
       const int h = hop ? ((__popc(x) >= 2 && D / 2 < D - nd) ? D / 2 : D - nd) : 0;
-      g_cu32* cr = crow(sp, v, h);
+      const uint32_t* cr = cst + jj * ND * k + h * k;
       uint32_t psum = 0;
       for (int i = 0; i < k; i++)
         if ((x >> i) & 1u) psum = sat(psum + cr[i]);
@@ -2048,8 +2069,15 @@ __device__ uint32_t plan_step(Slot* s, uint64_t V) {
       if (!job) cls = kClsHeavy;  // the per-query grow carries the lower-bound code
       break;
     case PH_STAR:
-      work = ceil_div(V, kStarTV);
-      s->n_hv = (uint32_t)s->nmem;  // heavy: the route table of each member
+      if (s->hA) {
+        // hub bounds: the members start from their hub stars and the job live
+        // test takes the true stars, so only the route tables are built here
+        work = (uint32_t)s->nmem;
+        cls = kClsHeavy;
+      } else {
+        work = ceil_div(V, kStarTV);
+        s->n_hv = (uint32_t)s->nmem;  // heavy: the route table of each member
+      }
       break;
     case PH_COPY: work = ceil_div(V, kStarTV); break;
     // the route entries of 64 members stay in registers: the heavy kernel

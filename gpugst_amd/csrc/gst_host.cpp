@@ -714,6 +714,21 @@ class Engine {
           for (int hh = 0; hh < gst::kBbHubs; hh++) x[hh] = std::min(x[hh], row[hh]);
         }
       }
+      // each member starts from its hub star, min over hubs h of the sum of
+      // its groups' distances to h: shortest paths from h to every group form
+      // a real walk-tree, and on power-law graphs a hub is often the best
+      // star centre.  The job live test then runs before the true stars are
+      // known (they are taken in the same pass).
+      for (size_t m = 0; m < grp.size(); m++) {
+        gst::Slot& q = h_slots_[mem[m]];
+        uint32_t hs = gst::kInf;
+        for (int hh = 0; hh < gst::kBbHubs; hh++) {
+          uint32_t sum = 0;
+          for (int i = 0; i < q.k; i++) sum = std::min(gst::kInf, sum + hA[(size_t)(q.lab + i) * gst::kBbHubs + hh]);
+          hs = std::min(hs, sum);
+        }
+        q.best = hs;
+      }
       d.hA = (const uint32_t*)(base + J.hA);
       d.nhubr = (uint32_t)g_->n_hubr;
       d.hub16 = g_->d_hubd16 != nullptr && g_->opt.hub16 != 0 ? 1u : 0u;
@@ -1615,12 +1630,21 @@ gst_status run_queries(gst_graph* g, Engine* eng, std::vector<QueryReq>& qs, gst
       eng->poison();  // reset once the stream has drained (next call)
       return st;
     }
+    // the trees of the finished queries are copied out together, with one
+    // wait, before anything is admitted into their released memory
     for (int s : fin) {
       uint64_t tag = 0;
       gst_result r;
-      gst_status hs = eng->harvest(s, &r, &tag, ext, ext_cap);
+      gst_status hs = eng->harvest(s, &r, &tag, ext, ext_cap, ext == nullptr);
       out[tag] = r;
       if (hs != GST_OK && first_err == GST_OK) first_err = hs;
+    }
+    if (!fin.empty() && ext == nullptr) {
+      gst_status hs = eng->sync_harvest();
+      if (hs != GST_OK) {
+        eng->poison();
+        return hs;
+      }
     }
   }
   return first_err;
