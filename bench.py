@@ -63,7 +63,7 @@ def job_fill_bytes(g, V, ks, ND=1):
 
 
 HUBS = 64  # hub vertices of the shortest-path backbone build (kBbHubs)
-JOB_CHUNK_LABELS = 16  # labels per dirty chunk of a job row (kJobChunkLabels)
+JOB_CHUNK_LABELS = 16  # labels per dirty chunk of an unbounded job's row (kJobChunkFlat)
 
 
 def hub_fill(g, ND):

@@ -52,11 +52,15 @@ constexpr int kStageWords = 1152;       // per-wave LDS staging (u64): 64*ND row
 
 // Group-distance jobs: one multi-label SSSP per batch of unbounded queries.
 // Labels are (query, group) pairs; the distance row of a vertex holds all of
-// them (u32, padded to whole 16-label chunks), so an arc relaxation reads and
-// writes coalesced 64-byte chunks (one HBM line each) shared by every query of
-// the job.  Swept 4 / 8 / 16 labels: coarser dirty bits relax more labels, but
-// each relaxation fetches a whole line instead of a quarter of one.
-constexpr int kJobChunkLabels = 16;    // labels per chunk of a job row (one dirty bit)
+// them (u32, padded to whole chunks), so an arc relaxation reads and writes
+// coalesced chunks shared by every query of the job.  A chunk is the unit of
+// the dirty bits: unbounded jobs use 16 labels (64 bytes, one HBM line per
+// relaxation), hop jobs 4 (their rounds relax only the chunks the height below
+// improved, and coarser bits relax more labels there).  Swept 4 / 8 / 16 for
+// both kinds: 16 / 4 were fastest (batch of 256 at k = 5: 53.0 -> 48.6 ms;
+// D = 4, batch of 128: 58.1 ms at 4 labels, 62 at 8, 74 at 16).
+constexpr int kJobChunkFlat = 16;      // labels per chunk of an unbounded job's row
+constexpr int kJobChunkHop = 4;        // ... of a hop job's row
 constexpr int kJobLabels = 256;        // labels per job (at most 64 chunks)
 constexpr uint32_t kJobArcs = 512;      // arcs per job ROUND item (swept: 128 / 256 / 512 / 1024)
 constexpr int kJobChunksPerLane = 8;    // chunks one lane relaxes per arc (swept: 2 / 4 / 8)
@@ -175,7 +179,7 @@ struct alignas(64) Slot {
   int32_t* lgrp;        // job: group id of each label
   uint32_t* jmem;       // job: member slot indices
   uint64_t spare0;      // unused (keeps the fields below at their offsets)
-  uint32_t spare1, pad3;
+  uint32_t jcl, pad3;    // job: labels per dirty chunk (kJobChunkFlat or kJobChunkHop)
   uint32_t* live;       // V: vertices that can lie on a tree cheaper than the bound
   uint32_t* lvb;        // V bits: live flag per vertex
   uint32_t* hold;       // V bits: the vertex holds a finite state with |p| >= 2

@@ -118,10 +118,20 @@ __device__ __forceinline__ uint32_t sat(uint32_t a) { return a < kInf ? a : kInf
 // memory no kernel writes while it runs: wave-uniform loads become scalar ones
 #define GST_AS_CONST __attribute__((address_space(4)))
 #endif
-typedef uint32_t jchunk __attribute__((vector_size(4 * kJobChunkLabels)));
-typedef uint32_t bchunk __attribute__((vector_size(16)));  // four hub labels
-typedef GST_AS_GLOBAL const jchunk g_cchunk;
-typedef GST_AS_GLOBAL jchunk g_chunk;
+// a job chunk of CL labels, and the 16-byte granule of copies and fills
+template <int CL>
+struct JChunk;
+template <>
+struct JChunk<4> {
+  typedef uint32_t type __attribute__((vector_size(16)));
+};
+template <>
+struct JChunk<16> {
+  typedef uint32_t type __attribute__((vector_size(64)));
+};
+typedef uint32_t bchunk __attribute__((vector_size(16)));  // four labels (hub or job)
+typedef GST_AS_GLOBAL const bchunk g_cgran;
+typedef GST_AS_GLOBAL bchunk g_gran;
 typedef GST_AS_GLOBAL uint32_t g_u32;
 typedef GST_AS_GLOBAL const uint32_t g_cu32;
 typedef GST_AS_GLOBAL unsigned long long g_u64;
@@ -141,7 +151,7 @@ __device__ __forceinline__ void store_stream(g_u32* p, uint32_t v) {
   __builtin_nontemporal_store(v, p);
 #endif
 }
-__device__ __forceinline__ void store_stream(g_chunk* p, jchunk v) {
+__device__ __forceinline__ void store_stream(g_gran* p, bchunk v) {
 #ifdef GST_CPU_SIM
   *p = v;
 #else
@@ -548,13 +558,13 @@ __device__ void item_ubfill_inf(const GraphDev& g, Slot* sp, uint32_t tile) {
   const uint64_t n_un = (uint64_t)g.V - (uint64_t)g.n_hubr;
   const uint64_t i0 = (uint64_t)tile * 64u;
   const uint64_t i1 = i0 + 64u < n_un ? i0 + 64u : n_un;
-  const int nch = sp->Lp / kJobChunkLabels;
-  jchunk inf;
-  for (int q = 0; q < kJobChunkLabels; q++) inf[q] = kInf;
+  const int ng = sp->Lp / 4;  // 16-byte granules per row
+  bchunk inf;
+  for (int q = 0; q < 4; q++) inf[q] = kInf;
   for (uint64_t i = i0; i < i1; i++) {
     const uint64_t v = (uint64_t)g.hubv[(uint64_t)g.n_hubr + i];
-    g_chunk* row = (g_chunk*)(sp->dist + v * (uint64_t)sp->Lp);
-    for (int c = lane_id(); c < nch; c += 64) store_stream(&row[c], inf);
+    g_gran* row = (g_gran*)(sp->dist + v * (uint64_t)sp->Lp);
+    for (int c = lane_id(); c < ng; c += 64) store_stream(&row[c], inf);
   }
 }
 
@@ -816,7 +826,7 @@ __device__ void item_jseed(const GraphDev& g, Slot* sp, uint32_t l) {
     if (t < end) {
       v = (uint32_t)g.gmem[t];
       sp->dist[(uint64_t)v * rs + l] = 0u;  // height 0
-      const unsigned long long old = atomicOr((unsigned long long*)&mn[v], 1ull << (l / kJobChunkLabels));
+      const unsigned long long old = atomicOr((unsigned long long*)&mn[v], 1ull << (l / sp->jcl));
       app = (old == 0);
     }
     wave_append(app, v, sp, nxt, A.rowptr, V);
@@ -842,14 +852,17 @@ __device__ __forceinline__ bool jrelax(uint32_t s, uint32_t t, uint32_t w, g_u32
 
 // Job ROUND: one chunk of arcs of the job's frontier.  G lanes share an arc
 // (64 / G arcs per wave iteration); lane r of the group relaxes the dirty
-// kJobChunkLabels-label chunks r, r + G, ... of the source row into the target
+// CL-label chunks r, r + G, ... of the source row into the target
 // row (one vector load per chunk, coalesced over the group).  Unbounded jobs
 // relax within the one row per vertex until no chunk is dirty; hop jobs relax
 // the rows of height level - 1 into those of height level, one round per
 // height, after COPY has set height level to height level - 1: the rows hold
 // the distances within level hops, and a round's frontier is only the chunks
 // that the previous height improved.
-__device__ void item_jround(const GraphDev& g, Slot* sp, uint32_t item, FrontLDS* L) {
+template <int CL>
+__device__ void item_jround_t(const GraphDev& g, Slot* sp, uint32_t item, FrontLDS* L) {
+  typedef typename JChunk<CL>::type jchunk;
+  typedef GST_AS_GLOBAL const jchunk g_cchunk;
   const Arcs A = arcs_of(g, sp);
   const int lane = lane_id();
   const uint64_t V = (uint64_t)g.V;
@@ -870,7 +883,7 @@ __device__ void item_jround(const GraphDev& g, Slot* sp, uint32_t item, FrontLDS
   const bool hop = sp->D >= 0;
   const uint32_t* sdist = sp->dist + (hop ? (uint64_t)(sp->level - 1) * Lp : 0);
   uint32_t* tdist = sp->dist + (hop ? (uint64_t)sp->level * Lp : 0);
-  const int nch = sp->Lp / kJobChunkLabels;
+  const int nch = sp->Lp / CL;
   const int G = sp->G;
   const int APW = 64 / G;
   const int sub = lane / G, rr = lane - sub * G;
@@ -915,9 +928,8 @@ __device__ void item_jround(const GraphDev& g, Slot* sp, uint32_t item, FrontLDS
           const jchunk tv = trow[c];
           bool im = false;
 #pragma unroll
-          for (int q = 0; q < kJobChunkLabels; q++)
-            im |= jrelax(sv[q], tv[q], w, tw + kJobChunkLabels * c + q, n_at);
-          n_relax += kJobChunkLabels;
+          for (int q = 0; q < CL; q++) im |= jrelax(sv[q], tv[q], w, tw + CL * c + q, n_at);
+          n_relax += CL;
           if (im) {
             imp |= 1ull << c;
             n_impr++;
@@ -941,6 +953,11 @@ __device__ void item_jround(const GraphDev& g, Slot* sp, uint32_t item, FrontLDS
     atomicAdd(&sp->ctr[4], (unsigned long long)n_impr);
     atomicAdd(&sp->ctr[5], (unsigned long long)n_atom);  // job: atomicMin issued
   }
+}
+
+__device__ __forceinline__ void item_jround(const GraphDev& g, Slot* sp, uint32_t item, FrontLDS* L) {
+  if (sp->jcl == (uint32_t)kJobChunkFlat) item_jround_t<kJobChunkFlat>(g, sp, item, L);
+  else item_jround_t<kJobChunkHop>(g, sp, item, L);
 }
 
 // Job CACHEIN / CACHEOUT over a tile of kStarTV vertices: one lane per label,
@@ -974,14 +991,12 @@ __device__ void item_jcopy(Slot* sp, uint32_t tile, uint64_t V) {
   const uint64_t v1 = v0 + kStarTV < V ? v0 + kStarTV : V;
   const uint64_t Lp = (uint64_t)sp->Lp;
   const uint64_t rs = (uint64_t)sp->ND * Lp;
-  const int nch = (int)(Lp / kJobChunkLabels);
+  const int ng = (int)(Lp / 4);  // 16-byte granules per row
   const uint64_t from = (uint64_t)(sp->level - 1) * Lp, to = (uint64_t)sp->level * Lp;
   for (uint64_t v = v0; v < v1; v++) {
     g_u32* row = (g_u32*)(sp->dist + v * rs);
-    for (int c = lane_id(); c < nch; c += 64) {
-      store_stream((g_chunk*)(row + to + (uint64_t)c * kJobChunkLabels),
-                   *(g_cchunk*)(row + from + (uint64_t)c * kJobChunkLabels));
-    }
+    for (int c = lane_id(); c < ng; c += 64)
+      store_stream((g_gran*)(row + to + (uint64_t)c * 4u), *(g_cgran*)(row + from + (uint64_t)c * 4u));
   }
 }
 

@@ -187,6 +187,12 @@ struct gst_graph {
   int32_t* d_gmem = nullptr;
   Options opt;
   std::mutex dev_mu;  // one engine on the device at a time
+  // Servers step continuously, so their chunks stay queued on the device
+  // after they release dev_mu: cont_out counts those chunks (under dev_mu).
+  // gst_groups_set raises quiesce, under which servers queue no new chunk,
+  // and changes the device group arrays only once cont_out is 0.
+  int cont_out = 0;
+  std::atomic<int> quiesce{0};
   std::unique_ptr<class Engine> plain;
   std::unique_ptr<class Engine> batch;
   gst::GraphDev dev() const {
@@ -272,7 +278,7 @@ Layout layout_for(const gst_graph* g, int k, int D) { return layout_for(g, k, D,
 // Group-distance job over L labels.
 struct JobLayout {
   size_t dist, mask, list, loff, istart, lgrp, jmem, ccol, hA, total;
-  int Lp, G, chunk;
+  int Lp, G, chunk, cl;
   uint32_t istride;
 };
 
@@ -280,12 +286,13 @@ JobLayout job_layout(const gst_graph* g, int L, int nmem, int D) {
   const size_t V = (size_t)g->V;
   JobLayout J;
   // rows of whole chunks, and of whole u64 words (FILL writes pairs of labels)
-  const int unit = gst::kJobChunkLabels > 2 ? gst::kJobChunkLabels : 2;
+  const int unit = D < 0 ? gst::kJobChunkFlat : gst::kJobChunkHop;  // labels per dirty chunk
   J.Lp = (L + unit - 1) / unit * unit;
   // lanes per arc: the smallest power of two that gives each lane at most
   // kJobChunksPerLane chunks (64 chunks: 8 lanes, 8 arcs per wave iteration;
   // one or two chunks: 1 lane, 64 arcs)
-  const int nch = J.Lp / gst::kJobChunkLabels;
+  J.cl = unit;
+  const int nch = J.Lp / unit;
   J.G = 1;
   while (J.G * gst::kJobChunksPerLane < nch) J.G <<= 1;
   if (g->opt.job_lanes > 0) J.G = (int)g->opt.job_lanes;
@@ -682,6 +689,7 @@ class Engine {
     d.bb = uses_backbone(g_, D) ? 1u : 0u;
     d.L = L;
     d.Lp = J.Lp;
+    d.jcl = (uint32_t)J.cl;
     d.G = J.G;
     d.chunk = J.chunk;
     d.istride = J.istride;
@@ -1016,7 +1024,7 @@ class Engine {
   // stay queued, so the device steps on while the host handles the oldest
   // one's snapshot.  Returns the queries the collected snapshot shows finished
   // (their final entries copied into the host mirror for harvest).
-  gst_status cont_iter(int steps, int depth, std::vector<int>* finished) {
+  gst_status cont_iter(int steps, int depth, std::vector<int>* finished, bool may_step = true) {
     finished->clear();
     if (poisoned_) return fail(GST_DEVICE_ERROR, "engine stopped by an earlier failure");
     depth = std::max(1, std::min(depth, kContBuffers));
@@ -1037,7 +1045,7 @@ class Engine {
     // host sources of the uploads stay alive until a chunk behind them is done
     std::vector<std::pair<void*, std::vector<char>>> keep;
     keep.swap(pending_uploads_);
-    if (n_inflight_ > 0 && (int)chunks_.size() < depth) {
+    if (may_step && n_inflight_ > 0 && (int)chunks_.size() < depth) {
       int buf = 0;
       for (;; buf++) {
         bool busy = false;
@@ -1076,11 +1084,12 @@ class Engine {
       }
     }
     if (chunks_.empty()) return GST_OK;
-    if ((int)chunks_.size() < depth && n_inflight_ > 0) return GST_OK;  // fill the queue first
+    if (may_step && (int)chunks_.size() < depth && n_inflight_ > 0) return GST_OK;  // fill the queue first
     return cont_collect(finished);
   }
 
   bool cont_outstanding() const { return !chunks_.empty(); }
+  int cont_queued() const { return (int)chunks_.size(); }
   gst_status sync_harvest() {
     const hipError_t e = hipStreamSynchronize(xs());
     if (e != hipSuccess) return fail(GST_DEVICE_ERROR, "result copy failed: %s", hipGetErrorString(e));
@@ -2043,7 +2052,19 @@ gst_status gst_groups_set(gst_graph* g, const int64_t* grp_ptr, const int32_t* m
   for (int64_t i = 0; i < nm; i++)
     if (members[i] < 0 || members[i] >= g->V)
       return fail(GST_INVALID_ARG, "member %lld = %d out of range", (long long)i, members[i]);
-  std::lock_guard<std::mutex> lk(g->dev_mu);
+  // servers of this graph queue no new chunk from now on; the arrays change
+  // once the chunks they queued have finished
+  struct Quiesce {
+    std::atomic<int>& q;
+    explicit Quiesce(std::atomic<int>& x) : q(x) { q++; }
+    ~Quiesce() { q--; }
+  } quiesce(g->quiesce);
+  std::unique_lock<std::mutex> lk(g->dev_mu);
+  while (g->cont_out > 0) {
+    lk.unlock();
+    std::this_thread::sleep_for(std::chrono::microseconds(100));
+    lk.lock();
+  }
   if (g->d_gptr) (void)hipFree(g->d_gptr);
   if (g->d_gmem) (void)hipFree(g->d_gmem);
   g->d_gptr = nullptr;
@@ -2348,7 +2369,12 @@ void drop_expired(gst_server* s) {
 void server_loop(gst_server* s) {
   std::vector<int> fin;
   Engine* e = s->eng.get();
+  bool quiesced = false;
   for (;;) {
+    // while the group arrays are being replaced the device gets no new
+    // chunk: wait a little instead of spinning
+    if (quiesced) std::this_thread::sleep_for(std::chrono::microseconds(200));
+    quiesced = false;
     int64_t chunk, depth, wait_us, batch;
     {
       std::lock_guard<std::mutex> dk(s->g->dev_mu);
@@ -2417,10 +2443,18 @@ void server_loop(gst_server* s) {
     // it (or of this one, with nothing queued behind) is read back.
     gst_status st;
     std::vector<std::pair<uint64_t, gst_result>> out;
+    bool held = false;
     {
       std::lock_guard<std::mutex> dk(s->g->dev_mu);
-      st = e->cont_iter((int)chunk, (int)depth, &fin);
-      if (st != GST_OK) e->poison();  // the server stops; nothing runs on this engine again
+      // while the group arrays are being replaced, queue no new chunk
+      held = s->g->quiesce.load() > 0;
+      const int before = e->cont_queued();
+      st = e->cont_iter((int)chunk, (int)depth, &fin, !held);
+      s->g->cont_out += e->cont_queued() - before;
+      if (st != GST_OK) {
+        s->g->cont_out -= e->cont_queued();  // poisoned: its chunks are never collected
+        e->poison();  // the server stops; nothing runs on this engine again
+      }
       for (int slot : fin) {
         if (st != GST_OK) break;
         uint64_t tag = 0;
@@ -2449,6 +2483,7 @@ void server_loop(gst_server* s) {
       s->known[pr.first] = 2;
     }
     if (!out.empty()) s->cv_done.notify_all();
+    if (held) quiesced = true;
   }
 }
 

@@ -349,6 +349,13 @@ def check_robust():
     ts = [srv.submit(x, D=d) for x, d in zip(mq, mD)]
     mixed = [srv.wait(t).cost for t in ts]
     out["server_stats"] = [srv.stat("steps"), srv.stat("chunks")]
+    # group lists replaced while the server has requests in flight: the
+    # change waits for the server's queued chunks, and nothing hangs
+    t1 = [srv.submit(q) for _ in range(4)]
+    g.set_groups(*groups_csr(groups))
+    t2 = [srv.submit(q) for _ in range(4)]
+    out["regroup_server"] = [srv.wait(t).cost for t in t1 + t2]
+    out["regroup_server_ok"] = out["regroup_server"] == [exp] * 8
     srv.close()
     out["mix"] = mixed
     out["mix_ok"] = mixed == [O.dpbf(rp, col, wt, [groups[y] for y in x], D=d)[0] for x, d in zip(mq, mD)]

@@ -78,6 +78,7 @@ def test_failure_handling_on_cpu_build(cpu_results):
     assert r["poison_ok"], r
     assert r["server_ok"], r
     assert r["server_stats"][0] >= r["server_stats"][1] > 0, r
+    assert r["regroup_server_ok"], r
 
 
 def test_group_distance_cache_on_cpu_build(cpu_results):
