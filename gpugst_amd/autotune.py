@@ -9,15 +9,16 @@ Tunables (all runtime options of the C ABI):
 * ``merge_tv``       -- live vertices per MERGE / FINAL work item (how many
                         vertices' subset tuples one wave takes per item);
 * ``job_lanes``      -- lanes sharing an arc in group-distance job rounds, i.e.
-                        how many of a row's 4-label subsets each lane relaxes
-                        (0: the smallest power of two giving each lane <= 8);
+                        how many of a row's dirty chunks (16 labels unbounded,
+                        4 hop-bounded) each lane relaxes (0: the smallest power
+                        of two giving each lane <= 8);
 * ``graph``          -- replay chunks of engine steps from a captured hipGraph;
 * ``max_slots``      -- queries and group-distance jobs admitted at once.
 
 ``tune()`` times a sample batch through ``solve_batch`` and searches the space
 by coordinate descent: starting from the engine defaults it sweeps one knob at
 a time over its values, keeps the fastest, and repeats the sweep once more
-(about 2 x 12 timed batches instead of the ~1,300 of the full grid).  Every
+(about 2 x 14 timed batches instead of the ~2,200 of the full grid).  Every
 schedule must return the same costs.  Results are kept per (graph size, k or
 "mix", D) bucket in a JSON file (``gpugst_amd/tuned.json`` by default) that
 ``apply()`` reads back.
@@ -37,7 +38,7 @@ GRID = {
     "blocks_per_cu": (0, 2, 3),
     "chunk": (256, 512, 1024),
     "merge_tv": (32, 64, 128),
-    "job_lanes": (0, 4, 16),
+    "job_lanes": (0, 1, 4, 8, 16),
     "graph": (0, 1),
     "max_slots": (256, 512),
 }
