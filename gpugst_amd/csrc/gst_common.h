@@ -155,6 +155,14 @@ struct EdgeDev {
   int32_t u, v, w, parent;
 };
 
+// One finished query's tree for the gather kernel: n edges from src to
+// position off of the staging buffer.
+struct GatherEnt {
+  const EdgeDev* src;
+  uint64_t off;
+  uint32_t n, pad;
+};
+
 struct alignas(64) Slot {
   // ---- query or job (set at admission) ----
   int32_t kind;         // KIND_QUERY or KIND_JOB

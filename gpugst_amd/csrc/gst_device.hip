@@ -2475,6 +2475,13 @@ __global__ __launch_bounds__(64) void gst_count_kernel(Slot* slot, uint64_t V) {
   item_count(slot, blockIdx.x, V);
 }
 
+// Trees of finished queries gathered into one staging buffer (one block per
+// tree), so the host copies them out with one transfer.
+__global__ __launch_bounds__(256) void gst_gather_kernel(const GatherEnt* t, EdgeDev* dst) {
+  const GatherEnt e = t[blockIdx.x];
+  for (uint32_t i = threadIdx.x; i < e.n; i += blockDim.x) dst[e.off + i] = e.src[i];
+}
+
 __global__ __launch_bounds__(kBlock) void gst_plan_kernel(Slot* slots, Ctrl* ctrl, uint64_t V) {
   __shared__ uint32_t sh_items[kNumCls][kMaxSlots];
   __shared__ uint32_t sh_scan[kNumCls][2][kBlock];
@@ -2511,6 +2518,12 @@ extern "C" hipError_t gst_launch_step(int cls, int wpb, const gst::GraphDev* g, 
 
 extern "C" hipError_t gst_launch_plan(gst::Slot* slots, gst::Ctrl* ctrl, uint64_t V, hipStream_t stream) {
   hipLaunchKernelGGL(gst::gst_plan_kernel, dim3(1), dim3(gst::kBlock), 0, stream, slots, ctrl, V);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t gst_launch_gather(const gst::GatherEnt* t, int n, gst::EdgeDev* dst, hipStream_t stream) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(gst::gst_gather_kernel, dim3((unsigned)n), dim3(256), 0, stream, t, dst);
   return hipGetLastError();
 }
 

@@ -30,6 +30,7 @@ extern "C" hipError_t gst_launch_plan(gst::Slot* slots, gst::Ctrl* ctrl, uint64_
 extern "C" hipError_t gst_step_occupancy(int cls, int wpb, int* blocks_per_cu);
 extern "C" hipError_t gst_scratch_setup(int device, void** ptr);
 extern "C" hipError_t gst_launch_count(gst::Slot* slot, uint64_t V, hipStream_t stream);
+extern "C" hipError_t gst_launch_gather(const gst::GatherEnt* t, int n, gst::EdgeDev* dst, hipStream_t stream);
 extern "C" hipError_t gst_bb_fill(uint32_t* p, uint64_t n, uint32_t val, hipStream_t stream);
 extern "C" hipError_t gst_bb_seed(uint32_t* dist, uint8_t* flags, const int32_t* hubs, int H, hipStream_t stream);
 extern "C" hipError_t gst_bb_pass(int what, const gst::BbArgs* a, hipStream_t stream);
@@ -387,6 +388,12 @@ class Engine {
       if (h_snap_[b]) (void)hipHostFree(h_snap_[b]);
     }
     if (h_zero_) (void)hipHostFree(h_zero_);
+    if (!borrowed_) {
+      if (d_gtab_) (void)hipFree(d_gtab_);
+      if (h_gtab_) (void)hipHostFree(h_gtab_);
+      if (d_gstage_) (void)hipFree(d_gstage_);
+      if (h_gstage_) (void)hipHostFree(h_gstage_);
+    }
     if (xstream_) {
       (void)hipStreamSynchronize(xstream_);
       (void)hipStreamDestroy(xstream_);
@@ -1090,11 +1097,6 @@ class Engine {
 
   bool cont_outstanding() const { return !chunks_.empty(); }
   int cont_queued() const { return (int)chunks_.size(); }
-  gst_status sync_harvest() {
-    const hipError_t e = hipStreamSynchronize(xs());
-    if (e != hipSuccess) return fail(GST_DEVICE_ERROR, "result copy failed: %s", hipGetErrorString(e));
-    return GST_OK;
-  }
   // continuous stepping: chunks run and their summed device time (between
   // a start and an end event of each chunk)
   int64_t cont_chunks() const { return cont_chunks_; }
@@ -1321,14 +1323,66 @@ class Engine {
     gexec_ = nullptr;
   }
 
+  // Copies the trees of the finished queries `slots` to the host: one gather
+  // kernel puts them side by side in a device staging buffer, one transfer
+  // brings that to pinned host memory.  harvest() then takes each tree from
+  // there (instead of a device-to-host copy of its own, each of which waits
+  // for its transfer into unpinned memory).
+  gst_status gather_trees(const std::vector<int>& slots) {
+    gents_.clear();
+    uint64_t tot = 0;
+    for (int s : slots) {
+      const gst::Slot& d = h_slots_[s];
+      staged_[s] = -1;
+      if (d.status != 0 || d.out_cost >= (int32_t)gst::kInf || d.out_nedges <= 0) continue;
+      gst::GatherEnt e;
+      e.src = d.edges;
+      e.off = tot;
+      e.n = (uint32_t)d.out_nedges;
+      e.pad = 0;
+      gents_.push_back(e);
+      staged_[s] = (int64_t)tot;
+      tot += (uint64_t)d.out_nedges;
+    }
+    if (gents_.empty()) return GST_OK;
+    if (gents_.size() > gtab_cap_) {
+      if (d_gtab_) (void)hipFree(d_gtab_);
+      if (h_gtab_) (void)hipHostFree(h_gtab_);
+      d_gtab_ = nullptr;
+      h_gtab_ = nullptr;
+      gtab_cap_ = 0;
+      const size_t cap = std::max<size_t>(gents_.size(), 1024);
+      HIP_TRY(hipMalloc((void**)&d_gtab_, cap * sizeof(gst::GatherEnt)));
+      HIP_TRY(hipHostMalloc((void**)&h_gtab_, cap * sizeof(gst::GatherEnt), hipHostMallocDefault));
+      gtab_cap_ = cap;
+    }
+    if (tot > gstage_cap_) {
+      if (d_gstage_) (void)hipFree(d_gstage_);
+      if (h_gstage_) (void)hipHostFree(h_gstage_);
+      d_gstage_ = nullptr;
+      h_gstage_ = nullptr;
+      gstage_cap_ = 0;
+      const size_t cap = std::max<size_t>((size_t)tot * 2, 65536);
+      HIP_TRY(hipMalloc((void**)&d_gstage_, cap * sizeof(gst::EdgeDev)));
+      HIP_TRY(hipHostMalloc((void**)&h_gstage_, cap * sizeof(gst::EdgeDev), hipHostMallocDefault));
+      gstage_cap_ = cap;
+    }
+    memcpy(h_gtab_, gents_.data(), gents_.size() * sizeof(gst::GatherEnt));
+    HIP_TRY(hipMemcpyAsync(d_gtab_, h_gtab_, gents_.size() * sizeof(gst::GatherEnt), hipMemcpyHostToDevice, xs()));
+    const hipError_t le = gst_launch_gather(d_gtab_, (int)gents_.size(), d_gstage_, xs());
+    if (le != hipSuccess) return fail(GST_DEVICE_ERROR, "tree gather launch failed: %s", hipGetErrorString(le));
+    HIP_TRY(hipMemcpyAsync(h_gstage_, d_gstage_, tot * sizeof(gst::EdgeDev), hipMemcpyDeviceToHost, xs()));
+    const hipError_t e = hipStreamSynchronize(xs());
+    if (e != hipSuccess) return fail(GST_DEVICE_ERROR, "tree copy failed: %s", hipGetErrorString(e));
+    return GST_OK;
+  }
+
   // Copies the result of a finished query out and releases its memory.  The
   // tree goes to a malloc'd buffer owned by `out`, or, with ext, into the
-  // caller's buffer of ext_cap edges (out->edges stays null then).
-  // With defer, the copy of the tree into its malloc'd buffer is only
-  // enqueued: the caller synchronises xs() (sync_harvest) before anything
-  // else is admitted into the released memory.
-  gst_status harvest(int s, gst_result* out, uint64_t* tag, gst_edge* ext = nullptr, int32_t ext_cap = 0,
-                     bool defer = false) {
+  // caller's buffer of ext_cap edges (out->edges stays null then).  A tree
+  // that gather_trees staged is taken from the staging buffer; any other is
+  // copied from the device on its own.
+  gst_status harvest(int s, gst_result* out, uint64_t* tag, gst_edge* ext = nullptr, int32_t ext_cap = 0) {
     const gst::Slot& d = h_slots_[s];
     SlotHost& h = sh_[s];
     memset(out, 0, sizeof(*out));
@@ -1400,15 +1454,22 @@ class Engine {
       } else if (d.out_nedges > 0) {
         out->edges = (gst_edge*)malloc(sizeof(gst_edge) * (size_t)d.out_nedges);
         if (!out->edges) return fail(GST_OUT_OF_MEMORY, "host edge buffer");
-        hipError_t e = hipMemcpyAsync(out->edges, d.edges, sizeof(gst_edge) * (size_t)d.out_nedges,
-                                      hipMemcpyDeviceToHost, xs());
-        if (e == hipSuccess && !defer) e = hipStreamSynchronize(xs());
-        if (e != hipSuccess) {
-          out->status = GST_DEVICE_ERROR;
-          st = fail(GST_DEVICE_ERROR, "edge copy failed: %s", hipGetErrorString(e));
+        auto sg = staged_.find(s);
+        if (sg != staged_.end() && sg->second >= 0) {
+          static_assert(sizeof(gst_edge) == sizeof(gst::EdgeDev), "edge layouts differ");
+          memcpy(out->edges, h_gstage_ + sg->second, sizeof(gst_edge) * (size_t)d.out_nedges);
+        } else {
+          hipError_t e = hipMemcpyAsync(out->edges, d.edges, sizeof(gst_edge) * (size_t)d.out_nedges,
+                                        hipMemcpyDeviceToHost, xs());
+          if (e == hipSuccess) e = hipStreamSynchronize(xs());
+          if (e != hipSuccess) {
+            out->status = GST_DEVICE_ERROR;
+            st = fail(GST_DEVICE_ERROR, "edge copy failed: %s", hipGetErrorString(e));
+          }
         }
       }
     }
+    staged_.erase(s);
     *tag = h.tag;
     for (auto& x : h.extra) x = 0;
     release_slot(s);
@@ -1529,6 +1590,16 @@ class Engine {
   hipEvent_t cev_[kContBuffers] = {nullptr, nullptr};   // end of a chunk (after its snapshot)
   hipEvent_t cev0_[kContBuffers] = {nullptr, nullptr};  // start of a chunk
   int64_t cont_chunks_ = 0, cont_busy_ns_ = 0;
+  // tree gather (gather_trees): entry table and staging buffers, and the
+  // staging offset of each harvested slot's tree
+  std::vector<gst::GatherEnt> gents_;
+  gst::GatherEnt* d_gtab_ = nullptr;
+  gst::GatherEnt* h_gtab_ = nullptr;
+  size_t gtab_cap_ = 0;
+  gst::EdgeDev* d_gstage_ = nullptr;
+  gst::EdgeDev* h_gstage_ = nullptr;
+  size_t gstage_cap_ = 0;
+  std::unordered_map<int, int64_t> staged_;
   uint32_t* h_zero_ = nullptr;
   std::deque<ContChunk> chunks_;
   uint64_t aseq_ = 0;
@@ -1639,21 +1710,21 @@ gst_status run_queries(gst_graph* g, Engine* eng, std::vector<QueryReq>& qs, gst
       eng->poison();  // reset once the stream has drained (next call)
       return st;
     }
-    // the trees of the finished queries are copied out together, with one
-    // wait, before anything is admitted into their released memory
+    // the trees of the finished queries come out together (one gather, one
+    // transfer), before anything is admitted into their released memory
+    if (!fin.empty() && ext == nullptr) {
+      gst_status gs = eng->gather_trees(fin);
+      if (gs != GST_OK) {
+        eng->poison();
+        return gs;
+      }
+    }
     for (int s : fin) {
       uint64_t tag = 0;
       gst_result r;
-      gst_status hs = eng->harvest(s, &r, &tag, ext, ext_cap, ext == nullptr);
+      gst_status hs = eng->harvest(s, &r, &tag, ext, ext_cap);
       out[tag] = r;
       if (hs != GST_OK && first_err == GST_OK) first_err = hs;
-    }
-    if (!fin.empty() && ext == nullptr) {
-      gst_status hs = eng->sync_harvest();
-      if (hs != GST_OK) {
-        eng->poison();
-        return hs;
-      }
     }
   }
   return first_err;
@@ -2455,16 +2526,16 @@ void server_loop(gst_server* s) {
         s->g->cont_out -= e->cont_queued();  // poisoned: its chunks are never collected
         e->poison();  // the server stops; nothing runs on this engine again
       }
+      if (st == GST_OK && !fin.empty()) {
+        st = e->gather_trees(fin);  // the finished trees, in one transfer
+        if (st != GST_OK) e->poison();
+      }
       for (int slot : fin) {
         if (st != GST_OK) break;
         uint64_t tag = 0;
         gst_result r;
-        e->harvest(slot, &r, &tag, nullptr, 0, true);
+        e->harvest(slot, &r, &tag);
         out.push_back({tag, r});
-      }
-      if (st == GST_OK && !out.empty()) {
-        st = e->sync_harvest();  // the trees are in their buffers
-        if (st != GST_OK) e->poison();
       }
       s->steps = e->steps_total();
       s->chunks = e->cont_chunks();

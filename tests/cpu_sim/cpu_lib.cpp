@@ -94,6 +94,12 @@ extern "C" hipError_t gst_launch_step(int cls, int wpb, const gst::GraphDev* g, 
   return hipSuccess;
 }
 
+extern "C" hipError_t gst_launch_gather(const gst::GatherEnt* t, int n, gst::EdgeDev* dst, hipStream_t) {
+  for (int b = 0; b < n; b++)
+    for (uint32_t i = 0; i < t[b].n; i++) dst[t[b].off + i] = t[b].src[i];
+  return hipSuccess;
+}
+
 extern "C" hipError_t gst_launch_count(gst::Slot* slot, uint64_t V, hipStream_t) {
   const uint32_t tiles = (uint32_t)((V + gst::kDistTV - 1) / gst::kDistTV);
   for (uint32_t t = 0; t < tiles; t++) {
