@@ -91,7 +91,7 @@ enum Phase : int32_t {
   PH_DONE = 8,
   PH_LIVE = 9,   // live-vertex list + initialisation of their T segments
   PH_WAITJ = 10, // query waiting for its group-distance job (no work items)
-  PH_STAR = 11,    // job: star bound of every member query
+  PH_STAR = 11,    // job: every member's route table (and, for a hop job, its star bound)
   PH_COPY = 12,    // hop job: height level starts as a copy of height level - 1
   PH_JLIVE = 13,   // job: live bits of every member query (one pass over its rows)
   PH_CACHEIN = 14, // job: copy cached group-distance columns into its rows

@@ -15,11 +15,12 @@
 //              reference's dis0_init_1 SSSP, code/GPUGST/src/DPQ.cu:241-271)
 //              or, for unbounded queries, from a group-distance job: one
 //              multi-label SSSP shared by a batch of queries, whose distance
-//              rows the queries then read as their C (STAR gives each its
-//              star bound)
+//              rows the queries then read as their C; the job then builds
+//              every member's vv and R, and one pass over its rows gives each
+//              member its star bound and live vertices (JLIVE)
 //   DIST     : compact distance table C[v][i], inter-group distances vv,
 //              route table R (host loops of code/GPUGST/src/DPQ.cu:409-483
-//              moved on the device)
+//              moved on the device) -- queries with their own level 1
 //   level L  : MERGE pass (every vertex segment staged in LDS; all sub-mask
 //              splits of every |p| = L subset, hop pairs d1 + d2 <= D, new
 //              depth max(d1,d2) -- code/D-GPUGST/src/DPQ.cu:100-124) followed by
@@ -1318,7 +1319,7 @@ __device__ void item_jlive(Slot* slots, Slot* sp, uint32_t tile, uint64_t V) {
     const uint32_t lim = bst < kInf ? bst : kInf - 1;
     const bool prog = has && q->anum != 0;
     // an unbounded job's live test also takes each member's star (its labels
-    // are the bound row itself); a hop job's STAR pass has done that
+    // are the bound row itself); a hop job's STAR step has done that
     const bool flat = sp->D < 0;
     // A member's labels sit at offset lab in every row of the job (rows of
     // rs words per vertex, heights Lp words apart): the row base is the same
@@ -1981,7 +1982,8 @@ __device__ void complete_step(Slot* slots, Slot* s, Ctrl* ctrl) {
         }
         break;
       case PH_STAR:
-        // the members' route tables were built beside the STAR tiles
+        // the members' route tables were built in this step (beside the star
+        // tiles of a hop job)
         s->phase = PH_JLIVE;
         break;
       case PH_JLIVE:
