@@ -28,11 +28,18 @@ def cpu_results():
         pytest.skip(f"CPU build unavailable: {e}")
     env = dict(os.environ, GST_LIB=lib)
     # every 30th tiny case (all D; the GPU tests take all of them), 54 k=5
-    # queries = 270 labels = two jobs, and 16 hop-bounded queries
-    r = subprocess.run([sys.executable, os.path.join(ROOT, "tests", "cpu_sim", "run_checks.py"), "30", "54", "16"],
-                       env=env, capture_output=True, text=True, timeout=900)
-    assert r.returncode == 0, r.stderr[-2000:]
-    return json.loads(r.stdout.strip().splitlines()[-1])
+    # queries = 270 labels = two jobs, and 16 hop-bounded queries; the checks
+    # run in three processes side by side (each emulated wave is 64 threads)
+    groups = ["tiny,hop,branching,static", "jobs,robust,graph", "cache,progressive,backbone"]
+    procs = [subprocess.Popen([sys.executable, os.path.join(ROOT, "tests", "cpu_sim", "run_checks.py"),
+                               "30", "54", "16", grp], env=env, stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                              text=True) for grp in groups]
+    out = {}
+    for p in procs:
+        so, se = p.communicate(timeout=900)
+        assert p.returncode == 0, se[-2000:]
+        out.update(json.loads(so.strip().splitlines()[-1]))
+    return out
 
 
 def test_tiny_cases_on_cpu_build(cpu_results):
