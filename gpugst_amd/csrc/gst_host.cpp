@@ -145,10 +145,10 @@ struct Options {
   // visits, chunks kept queued on the device, and how long / for how many
   // pending requests admission waits while the device is busy, so that more
   // requests share a group-distance job
-  int64_t serve_chunk = 2;
+  int64_t serve_chunk = 4;
   int64_t serve_depth = 2;
-  int64_t serve_wait_us = 10000;  // measured: 1.1M-vertex graph, 1000-2000 q/s offered
-  int64_t serve_batch = 32;
+  int64_t serve_wait_us = 10000;  // measured: 1.1M-vertex graph, 1000-3300 q/s offered
+  int64_t serve_batch = 64;
 };
 
 }  // namespace

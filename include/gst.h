@@ -230,12 +230,12 @@ gst_status gst_query_bytes(gst_graph *g, int32_t k, int32_t D, int64_t *bytes);
 /* Server: a background thread keeps the graph resident and admits submitted
  * queries into the running engine as memory and slots free up.  The engine
  * steps continuously: between chunks of "serve_chunk" engine steps (option,
- * default 2; "serve_depth" chunks, default 2, stay queued on the device) the
+ * default 4; "serve_depth" chunks, default 2, stay queued on the device) the
  * thread uploads newly admitted queries, which the running step plan takes up
  * at its next step, and copies finished answers out.  While the device is
  * busy, admission may wait up to "serve_wait_us" microseconds for
  * "serve_batch" pending requests, so that they share a group-distance job
- * (defaults 10000 and 32; 0 and 1 admit every request at once). */
+ * (defaults 10000 and 64; 0 and 1 admit every request at once). */
 gst_status gst_server_create(gst_graph *g, int64_t budget_bytes,
                              int32_t max_slots, gst_server **out);
 gst_status gst_submit(gst_server *s, const gst_query *q, uint64_t *ticket);
