@@ -1495,6 +1495,7 @@ class Engine {
     n_inflight_ = 0;
     fresh_.clear();
     chunks_.clear();
+    staged_.clear();
     cont_on_ = false;  // the next continuous chunk uploads the whole table again
     pending_uploads_.clear();
     job_members_.clear();
