@@ -55,7 +55,7 @@ def job_fill_bytes(g, V, ks, ND=1):
     bounds also reads each vertex's row of hub distances once (4 B per hub)."""
     labels = int(sum(ks))
     jobs = max(1, g.get_option("batch_jobs"))
-    lp = labels + (JOB_CHUNK_LABELS - 1) * jobs // 2  # rows padded to whole chunks (half a chunk on average)
+    lp = labels + (chunk_labels(ND) - 1) * jobs // 2  # rows padded to whole chunks (half a chunk on average)
     b = 4 * V * lp + 24 * V * jobs
     if hub_fill(g, ND):
         b += 4 * HUBS * V * jobs
@@ -64,6 +64,11 @@ def job_fill_bytes(g, V, ks, ND=1):
 
 HUBS = 64  # hub vertices of the shortest-path backbone build (kBbHubs)
 JOB_CHUNK_LABELS = 16  # labels per dirty chunk of an unbounded job's row (kJobChunkFlat)
+JOB_CHUNK_LABELS_HOP = 4  # labels per dirty chunk of a hop job's rows (kJobChunkHop)
+
+
+def chunk_labels(ND):
+    return JOB_CHUNK_LABELS if ND == 1 else JOB_CHUNK_LABELS_HOP
 
 
 def hub_fill(g, ND):
@@ -79,9 +84,12 @@ def lean_bytes(g, res_list, V, ks, ND=1):
       job FILL        (job_fill_bytes; in the heavy kernel for unbounded jobs
                       with hub bounds)
       job ROUND       per frontier entry 24 B (list id, arc offset, mask, row
-                      pointer) + 12 B clear; per arc 8 B (column + weight);
-                      per label relaxed 4 B (target chunk read, 64 B per
-                      16-label chunk); per improved chunk 64 B (write back)
+                      pointer) + 12 B clear, and its dirty chunks of the source
+                      row (CL labels of 4 B each, read once per entry); per arc
+                      8 B (column + weight); per label relaxed 4 B (target
+                      chunk read); per improved chunk 4 * CL B (write back);
+                      CL = 16 labels per chunk for an unbounded job, 4 for a
+                      hop job
       job COPY        a hop job's heights 1..D: 4 B read + 4 B written per
                       (vertex, label, height)
       job STAR        4 B per (vertex, label) read, from the row of height
@@ -90,13 +98,15 @@ def lean_bytes(g, res_list, V, ks, ND=1):
                       job clears only its live vertices' masks, in LIVE"""
     labels = int(sum(ks))
     jobs = max(1, g.get_option("batch_jobs"))
-    lp = labels + (JOB_CHUNK_LABELS - 1) * jobs // 2
+    lp = labels + (chunk_labels(ND) - 1) * jobs // 2
     ent = g.get_option("batch_job_entries")
     arcs = g.get_option("batch_job_arcs")
     relax = g.get_option("batch_job_relaxations")
     impr = g.get_option("batch_job_improvements")
+    src = g.get_option("batch_job_src_chunks")
+    cl = chunk_labels(ND)
     b = 0 if hub_fill(g, ND) else job_fill_bytes(g, V, ks, ND)
-    b += 36 * ent + 8 * arcs + 4 * relax + 4 * JOB_CHUNK_LABELS * impr
+    b += 36 * ent + 8 * arcs + 4 * relax + 4 * cl * (impr + src)
     b += 8 * V * lp * (ND - 1)
     b += 4 * V * lp
     b += (V // 4) * len(res_list)
@@ -118,7 +128,7 @@ def heavy_bytes(g, res_list, V, ks, ND=1):
     of a query fed by a job hold only these rounds; plus the FILL of
     unbounded jobs with hub bounds (job_fill_bytes)."""
     jobs = max(1, g.get_option("batch_jobs"))
-    total = 4 * V * (int(sum(ks)) + (JOB_CHUNK_LABELS - 1) * jobs // 2)
+    total = 4 * V * (int(sum(ks)) + (chunk_labels(ND) - 1) * jobs // 2)
     if hub_fill(g, ND):
         total += job_fill_bytes(g, V, ks, ND)
     for r, k in zip(res_list, ks):

@@ -186,7 +186,7 @@ struct alignas(64) Slot {
   uint32_t* dist;       // job: V * ND * Lp distances (per vertex, one row of Lp labels per height)
   int32_t* lgrp;        // job: group id of each label
   uint32_t* jmem;       // job: member slot indices
-  uint64_t spare0;      // unused (keeps the fields below at their offsets)
+  unsigned long long nsrc;  // job ROUND: source chunks read (the dirty chunks of its frontier entries)
   uint32_t jcl, pad3;    // job: labels per dirty chunk (kJobChunkFlat or kJobChunkHop)
   uint32_t* live;       // V: vertices that can lie on a tree cheaper than the bound
   uint32_t* lvb;        // V bits: live flag per vertex

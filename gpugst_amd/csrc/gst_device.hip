@@ -889,7 +889,7 @@ __device__ void item_jround_t(const GraphDev& g, Slot* sp, uint32_t item, FrontL
   const int APW = 64 / G;
   const int sub = lane / G, rr = lane - sub * G;
   uint64_t n_relax = 0, n_impr = 0;
-  uint32_t n_at = 0, apn = 0;
+  uint32_t n_at = 0, apn = 0, n_src = 0;
   uint32_t a = a0;
   for (uint32_t i = sp->istart[(uint64_t)cur * sp->istride + item]; a < a1; i += 64) {
     stage_front(A.rowptr, L, i, n, arcs_total, lst, lof, mcur);
@@ -923,10 +923,13 @@ __device__ void item_jround_t(const GraphDev& g, Slot* sp, uint32_t item, FrontL
         g_cchunk* srow = (g_cchunk*)(sdist + (uint64_t)vs * rs);
         g_cchunk* trow = (g_cchunk*)(tdist + (uint64_t)u * rs);
         g_u32* tw = (g_u32*)(tdist + (uint64_t)u * rs);
+        // the source's chunks are counted once per entry, at its first arc
+        const uint32_t first = (pos == L->woff[o]) ? 1u : 0u;
         for (int c = rr; c < nch; c += G) {
           if (!((m >> c) & 1ull)) continue;
           const jchunk sv = srow[c];
           const jchunk tv = trow[c];
+          n_src += first;
           bool im = false;
 #pragma unroll
           for (int q = 0; q < CL; q++) im |= jrelax(sv[q], tv[q], w, tw + CL * c + q, n_at);
@@ -949,10 +952,12 @@ __device__ void item_jround_t(const GraphDev& g, Slot* sp, uint32_t item, FrontL
   n_relax = wave_sum_u64(n_relax);
   n_impr = wave_sum_u64(n_impr);
   const uint64_t n_atom = wave_sum_u64(n_at);
+  const uint64_t n_srcw = wave_sum_u64(n_src);
   if (lane == 0) {
     atomicAdd(&sp->ctr[3], (unsigned long long)n_relax);
     atomicAdd(&sp->ctr[4], (unsigned long long)n_impr);
     atomicAdd(&sp->ctr[5], (unsigned long long)n_atom);  // job: atomicMin issued
+    atomicAdd(&sp->nsrc, (unsigned long long)n_srcw);
   }
 }
 

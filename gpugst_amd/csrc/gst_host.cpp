@@ -1015,6 +1015,7 @@ class Engine {
     }
     jobs_done_++;
     for (int t = 0; t < 6; t++) job_ctr_[t] += (int64_t)j.ctr[t];
+    job_ctr_[6] += (int64_t)j.nsrc;
     sh_[i].charged = true;
     release_job_cache(i);
   }
@@ -1577,7 +1578,7 @@ class Engine {
   int graph_wpb_ = 0;
   int64_t kernel_ns_ = 0, launches_ = 0, steps_total_ = 0;
   int64_t cls_ns_[2] = {0, 0}, cls_launches_ = 0, jobs_done_ = 0;
-  int64_t job_ctr_[6] = {0, 0, 0, 0, 0, 0};
+  int64_t job_ctr_[7] = {0, 0, 0, 0, 0, 0, 0};  // the job slots' ctr[0..5], then their nsrc
   std::vector<int64_t> pair_ns_;  // per step of the current epoch (time_steps)
   std::map<uint32_t, std::pair<int64_t, int64_t>> phase_ns_;
   // host copies of job tables, uploaded at the start of the next epoch (or
@@ -2243,6 +2244,7 @@ gst_status gst_get_option(gst_graph* g, const char* key, int64_t* value) {
     if (f == "job_relaxations") { *value = e ? e->job_ctr(3) : 0; return GST_OK; }
     if (f == "job_improvements") { *value = e ? e->job_ctr(4) : 0; return GST_OK; }
     if (f == "job_atomics") { *value = e ? e->job_ctr(5) : 0; return GST_OK; }
+    if (f == "job_src_chunks") { *value = e ? e->job_ctr(6) : 0; return GST_OK; }
     if (f == "cache_hits") { *value = e ? e->cache_hits_ : 0; return GST_OK; }
     if (f == "cache_misses") { *value = e ? e->cache_misses_ : 0; return GST_OK; }
     if (f == "cache_inserts") { *value = e ? e->cache_inserts_ : 0; return GST_OK; }

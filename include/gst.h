@@ -215,7 +215,7 @@ void gst_result_free(gst_result *r);
  * its build), "backbone_us" (build time), "occupancy", "occupancy_heavy", "grid",
  * "grid_heavy", "{plain,batch}_{kernel_ns,launches,steps,lean_ns,heavy_ns,
  * timed_steps,jobs,job_entries,job_arcs,job_relaxations,job_improvements,
- * job_atomics,cache_hits,cache_misses,cache_inserts,cache_evictions,
+ * job_atomics,job_src_chunks,cache_hits,cache_misses,cache_inserts,cache_evictions,
  * cache_entries}", "{plain,batch}_phase_ns:<phase>:<level>" and
  * "{plain,batch}_phase_steps:<phase>:<level>" (step time and step count of
  * the traced slot per engine phase).

@@ -103,6 +103,7 @@ def check_jobs(V, nq, seed):
     g = Graph(rp, col, wt)
     g.set_groups(*groups_csr(groups))
     rs = g.solve_batch(qs)
+    ctr = {x: g.get_option("batch_job_" + x) for x in ("entries", "arcs", "relaxations", "src_chunks")}
     g.set_option("backbone", 0)  # the same batch on every arc
     rf = g.solve_batch(qs)
     g.set_option("backbone", 1)
@@ -117,7 +118,7 @@ def check_jobs(V, nq, seed):
             ok = validate_tree(rp, col, wt, gl, rs[i].cost, rs[i].root, rs[i].edges)["ok"]
         if not ok:
             bad.append([i, rs[i].cost, r0[i].cost, exp])
-    return {"checked": nq, "bad": bad, "labels": 5 * nq}
+    return {"checked": nq, "bad": bad, "labels": 5 * nq, "counters": ctr}
 
 
 def check_hop(V, nq, seed):

@@ -52,6 +52,12 @@ def test_group_distance_jobs_on_cpu_build(cpu_results):
     j = cpu_results["jobs"]
     assert j["labels"] > 256  # more than one job
     assert j["bad"] == [], j["bad"]
+    # source chunks: the dirty 16-label chunks of the frontier entries, each
+    # relaxed along every arc of its entry (16 labels per relaxation) and at
+    # most 16 chunks (256 labels) per entry
+    c = j["counters"]
+    assert 0 < 16 * c["src_chunks"] <= c["relaxations"], c
+    assert c["src_chunks"] <= 16 * c["entries"], c
 
 
 def test_hop_queries_on_cpu_build(cpu_results):
