@@ -52,13 +52,14 @@ def pct(x, q):
 def job_fill_bytes(g, V, ks, ND=1):
     """Algorithmic bytes of the group-distance jobs' FILL: 4 B per (vertex,
     label) of height 0 + 24 B per vertex (3 masks).  An unbounded job with hub
-    bounds also reads each vertex's row of hub distances once (4 B per hub)."""
+    bounds also reads each vertex's row of hub distances once (2 B per hub in
+    the packed 16-bit fill, else 4 B)."""
     labels = int(sum(ks))
     jobs = max(1, g.get_option("batch_jobs"))
     lp = labels + (chunk_labels(ND) - 1) * jobs // 2  # rows padded to whole chunks (half a chunk on average)
     b = 4 * V * lp + 24 * V * jobs
     if hub_fill(g, ND):
-        b += 4 * HUBS * V * jobs
+        b += g.get_option("hub_bytes") * HUBS * V * jobs
     return b
 
 

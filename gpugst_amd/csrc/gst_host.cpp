@@ -2229,6 +2229,7 @@ gst_status gst_get_option(gst_graph* g, const char* key, int64_t* value) {
   if (k == "backbone_arcs") { *value = g->has_bb ? g->bb_arcs : g->n_arcs; return GST_OK; }
   if (k == "backbone_rounds") { *value = g->bb_rounds; return GST_OK; }
   if (k == "backbone_us") { *value = (int64_t)(g->bb_ms * 1e3); return GST_OK; }
+  if (k == "hub_bytes") { *value = g->d_hubd16 && g->opt.hub16 ? 2 : 4; return GST_OK; }
   if (k.rfind("plain_", 0) == 0 || k.rfind("batch_", 0) == 0) {
     Engine* e = (k[0] == 'p') ? g->plain.get() : g->batch.get();
     const std::string f = k.substr(6);

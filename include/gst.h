@@ -212,7 +212,8 @@ void gst_result_free(gst_result *r);
  * distances allow; 0: 32-bit), "serve_chunk", "serve_depth",
  * "serve_wait_us", "serve_batch" (the server, see gst_server_create).
  * Read-only: "num_cu", "backbone_arcs", "backbone_rounds" (hub rounds of
- * its build), "backbone_us" (build time), "occupancy", "occupancy_heavy", "grid",
+ * its build), "backbone_us" (build time), "hub_bytes" (bytes per hub distance
+ * a job's hub FILL reads: 2 in packed 16-bit arithmetic, else 4), "occupancy", "occupancy_heavy", "grid",
  * "grid_heavy", "{plain,batch}_{kernel_ns,launches,steps,lean_ns,heavy_ns,
  * timed_steps,jobs,job_entries,job_arcs,job_relaxations,job_improvements,
  * job_atomics,job_src_chunks,cache_hits,cache_misses,cache_inserts,cache_evictions,
