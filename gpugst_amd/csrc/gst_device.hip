@@ -2191,42 +2191,50 @@ __device__ void phase_b(Slot* slots, Ctrl* ctrl, uint64_t V, uint32_t (*sh_items
   }
   __syncthreads();
   // per class: exclusive scan over slots with work, compacting them into the
-  // active list
+  // active list.  Thread t owns slots [b0, b1); a scan within each wave, then
+  // the waves' totals (kBlock / 64 of them) give every thread its offsets.
   const int per = (nslots + kBlock - 1) / kBlock;
   const int b0 = tid * per;
   const int b1 = (b0 + per) < nslots ? (b0 + per) : nslots;
+  const int wid = tid >> 6;
+  uint32_t xcnt[kNumCls], xact[kNumCls];
   for (int c = 0; c < kNumCls; c++) {
     uint32_t cnt = 0, act = 0;
     for (int i = b0; i < b1; i++) {
       cnt += sh_items[c][i];
       act += sh_items[c][i] ? 1u : 0u;
     }
-    sh_scan[c][0][tid] = cnt;
-    sh_scan[c][1][tid] = act;
-  }
-  __syncthreads();
-  if (tid == 0) {
-    uint32_t any = 0;
-    for (int c = 0; c < kNumCls; c++) {
-      uint32_t cs = 0, as = 0;
-      for (int t = 0; t < kBlock; t++) {
-        const uint32_t x = sh_scan[c][0][t], y = sh_scan[c][1][t];
-        sh_scan[c][0][t] = cs;
-        sh_scan[c][1][t] = as;
-        cs += x;
-        as += y;
-      }
-      ctrl->item_total[c] = cs;
-      ctrl->n_active[c] = as;
-      ctrl->prefix[c][as] = cs;
-      ctrl->item_next[c] = 0;
-      any += as;
+    xcnt[c] = wave_excl_scan(cnt);
+    xact[c] = wave_excl_scan(act);
+    if ((tid & 63) == 63) {
+      sh_scan[c][0][wid] = xcnt[c] + cnt;
+      sh_scan[c][1][wid] = xact[c] + act;
     }
-    ctrl->exit_flag = (leave || any == 0) ? 1u : 0u;
   }
   __syncthreads();
+  uint32_t any = 0;
   for (int c = 0; c < kNumCls; c++) {
-    uint32_t cs = sh_scan[c][0][tid], as = sh_scan[c][1][tid];
+    uint32_t tc = 0, ta = 0;
+    for (int w = 0; w < kBlock / 64; w++) {
+      const uint32_t x = sh_scan[c][0][w], y = sh_scan[c][1][w];
+      if (w < wid) {
+        xcnt[c] += x;
+        xact[c] += y;
+      }
+      tc += x;
+      ta += y;
+    }
+    if (tid == 0) {
+      ctrl->item_total[c] = tc;
+      ctrl->n_active[c] = ta;
+      ctrl->prefix[c][ta] = tc;
+      ctrl->item_next[c] = 0;
+    }
+    any += ta;
+  }
+  if (tid == 0) ctrl->exit_flag = (leave || any == 0) ? 1u : 0u;
+  for (int c = 0; c < kNumCls; c++) {
+    uint32_t cs = xcnt[c], as = xact[c];
     for (int i = b0; i < b1; i++) {
       if (sh_items[c][i]) {
         ctrl->active[c][as] = (uint32_t)i;
