@@ -513,13 +513,39 @@ __device__ __forceinline__ us2 us2_min(us2 a, us2 b) {
 
 __device__ __forceinline__ uint32_t ub16_groups(const Slot* sp) { return ((uint32_t)sp->Lp + 127u) / 128u; }
 
+// One vertex's packed bounds: the least of a[h] + d(h, v) over the hubs, for
+// the lane's two labels (hr: the vertex's kBbHubs / 2 hub-distance pairs).
+template <class HR>
+__device__ __forceinline__ us2 ub16_vertex(const us2* a, HR hr) {
+  us2 u = us2{(unsigned short)kHub16Inf, (unsigned short)kHub16Inf};
+#pragma unroll
+  for (int h = 0; h < kBbHubs / 2; h++) {
+    const uint32_t pr = hr(h);
+    const unsigned short d0 = (unsigned short)(pr & 0xFFFFu), d1 = (unsigned short)(pr >> 16);
+    u = us2_min(u, a[2 * h] + us2{d0, d0});
+    u = us2_min(u, a[2 * h + 1] + us2{d1, d1});
+  }
+  return u;
+}
+
+// A group of 128 labels takes 64 lanes per vertex, one vertex at a time (its
+// hub distances arrive as scalar loads).  The last group of a row whose
+// length is not a multiple of 128 has rem labels: it takes the smallest P of
+// 8, 16, 32, 64 lanes with 2P >= rem per vertex and 64 / P vertices at a
+// time, so that a job of, say, 160 labels does not pay a second full group.
 __device__ void item_ubfill16(const GraphDev& g, Slot* sp, uint32_t tile) {
   const int lane = lane_id();
   const uint32_t ngrp = ub16_groups(sp);
-  const uint32_t l0 = (tile % ngrp) * 128u + (uint32_t)lane, l1 = l0 + 64u;
+  const uint32_t Lp = (uint32_t)sp->Lp;
+  const uint32_t base = (tile % ngrp) * 128u;
+  const uint32_t rem = Lp - base < 128u ? Lp - base : 128u;
+  uint32_t sh = 6;  // log2 of the lanes per vertex
+  while (sh > 3 && (1u << sh) >= rem) sh--;
+  const uint32_t P = 1u << sh;
+  const uint32_t sub = (uint32_t)lane >> sh, r = (uint32_t)lane & (P - 1u);
+  const uint32_t l0 = base + r, l1 = base + P + r;
   const uint64_t i0 = (uint64_t)(tile / ngrp) * kUbTV;
   const uint64_t i1 = i0 + kUbTV < (uint64_t)g.n_hubr ? i0 + kUbTV : (uint64_t)g.n_hubr;
-  const uint32_t Lp = (uint32_t)sp->Lp;
   const bool lv0 = l0 < Lp, lv1 = l1 < Lp;
   us2 a[kBbHubs];
   const bchunk* r0 = (const bchunk*)(sp->hA + (uint64_t)(lv0 ? l0 : 0u) * kBbHubs);
@@ -534,22 +560,29 @@ __device__ void item_ubfill16(const GraphDev& g, Slot* sp, uint32_t tile) {
       a[4 * j + q] = us2{(unsigned short)b0, (unsigned short)b1};
     }
   }
-  const GST_AS_CONST uint32_t* hubd = (const GST_AS_CONST uint32_t*)g.hubd16;  // hub pairs
-  const GST_AS_CONST uint32_t* hubv = (const GST_AS_CONST uint32_t*)g.hubv;
   g_u32* dist = (g_u32*)sp->dist;
-  for (uint64_t i = i0; i < i1; i++) {
-    const uint64_t v = (uint64_t)hubv[i];
-    const GST_AS_CONST uint32_t* hr = hubd + v * (kBbHubs / 2);
-    us2 u = us2{(unsigned short)kHub16Inf, (unsigned short)kHub16Inf};
-#pragma unroll
-    for (int h = 0; h < kBbHubs / 2; h++) {
-      const uint32_t pr = hr[h];
-      const unsigned short d0 = (unsigned short)(pr & 0xFFFFu), d1 = (unsigned short)(pr >> 16);
-      u = us2_min(u, a[2 * h] + us2{d0, d0});
-      u = us2_min(u, a[2 * h + 1] + us2{d1, d1});
+  if (P == 64u) {
+    const GST_AS_CONST uint32_t* hubd = (const GST_AS_CONST uint32_t*)g.hubd16;  // hub pairs
+    const GST_AS_CONST uint32_t* hubv = (const GST_AS_CONST uint32_t*)g.hubv;
+    for (uint64_t i = i0; i < i1; i++) {
+      const uint64_t v = (uint64_t)hubv[i];
+      const GST_AS_CONST uint32_t* hr = hubd + v * (kBbHubs / 2);
+      const us2 u = ub16_vertex(a, [hr](int h) { return hr[h]; });
+      if (lv0) store_stream(&dist[v * Lp + l0], u[0] < kHub16Inf ? (uint32_t)u[0] : kInf);
+      if (lv1) store_stream(&dist[v * Lp + l1], u[1] < kHub16Inf ? (uint32_t)u[1] : kInf);
     }
-    if (lv0) store_stream(&dist[v * Lp + l0], u[0] < kHub16Inf ? (uint32_t)u[0] : kInf);
-    if (lv1) store_stream(&dist[v * Lp + l1], u[1] < kHub16Inf ? (uint32_t)u[1] : kInf);
+  } else {
+    const uint32_t vpi = 64u >> sh;
+    for (uint64_t i = i0 + sub; i < i1; i += vpi) {
+      const uint64_t v = (uint64_t)g.hubv[i];
+      g_cgran* hq = (g_cgran*)(g.hubd16 + v * kBbHubs);  // kBbHubs / 8 granules of four pairs
+      bchunk q[kBbHubs / 8];
+#pragma unroll
+      for (int j = 0; j < kBbHubs / 8; j++) q[j] = hq[j];
+      const us2 u = ub16_vertex(a, [&q](int h) { return q[h >> 2][h & 3]; });
+      if (lv0) store_stream(&dist[v * Lp + l0], u[0] < kHub16Inf ? (uint32_t)u[0] : kInf);
+      if (lv1) store_stream(&dist[v * Lp + l1], u[1] < kHub16Inf ? (uint32_t)u[1] : kInf);
+    }
   }
 }
 
