@@ -1041,9 +1041,12 @@ __device__ void item_jcopy(Slot* sp, uint32_t tile, uint64_t V) {
 
 // Job STAR: the star bound min_v sum_i c_i(v) of every member query, from
 // its k labels of each distance row.  One lane per member (64 members at a
-// time): the row base is the same for the whole wave and each lane adds its
-// labels' offset, so a row is fetched once for the job; the labels of
-// kStarBatch vertices are read before they are summed.  A hop job's height d
+// time): the row base is the same for the lanes of a member group and each
+// lane adds its labels' offset, so a row is fetched once for the job; the
+// labels of kStarBatch vertices are read before they are summed.  A last
+// group of fewer than 64 members takes the smallest power of two P of lanes
+// that holds it, and the 64 / P groups of P lanes take turns over the
+// vertices.  A hop job's height d
 // holds the distance within d hops, so height floor(D/2) gives the star arms.
 constexpr int kStarBatch = 2;
 
@@ -1055,13 +1058,17 @@ __device__ void item_jstar(Slot* slots, Slot* sp, uint32_t tile, uint64_t V) {
   const uint64_t hs = sp->D >= 0 ? (uint64_t)(sp->D / 2) : 0u;
   const int nmem = sp->nmem;
   for (int mg = 0; mg < nmem; mg += 64) {
-    const int m = mg + lane;
+    const int rem = nmem - mg < 64 ? nmem - mg : 64;
+    int sh = 6;  // log2 of the lanes per member group
+    while (sh > 0 && (1 << (sh - 1)) >= rem) sh--;
+    const int m = mg + (lane & ((1 << sh) - 1));
+    const uint64_t sub = (uint64_t)(lane >> sh), ngr = (uint64_t)(64 >> sh);
     const bool has = m < nmem;
     Slot* q = has ? &slots[sp->jmem[m]] : nullptr;
     const int k = has ? q->k : 0;
     const uint32_t off = has ? (uint32_t)q->lab + (uint32_t)(hs * Lp) : 0u;
     uint32_t star = kInf;
-    for (uint64_t vb = v0; vb < v1; vb += kStarBatch) {
+    for (uint64_t vb = v0 + kStarBatch * sub; vb < v1; vb += kStarBatch * ngr) {
       uint32_t c[kStarBatch][kMaxK];
 #pragma unroll
       for (int b = 0; b < kStarBatch; b++) {
@@ -1329,9 +1336,12 @@ __device__ __forceinline__ uint32_t lb_full_regs(const uint32_t* c, int k, const
 // (64 members at a time) holds its query's full-set route entries, bound and
 // row offset, and the wave walks the tile's vertices, so the members read
 // their labels side by side from the same row (each row is fetched once for
-// the whole job instead of a line per member and vertex).  A lane gathers the
-// live bits of 32 vertices into a word and stores it into its query's live
-// bits (the tile owns those words; the query's FILL cleared them).  The
+// the whole job instead of a line per member and vertex).  A last group of
+// fewer than 64 members takes the smallest power of two P of lanes that holds
+// it, and the 64 / P groups of P lanes walk different 32-vertex words of the
+// tile.  A lane gathers the live bits of 32 vertices into a word and stores it
+// into its query's live bits (the tile owns those words; the query's FILL
+// cleared them).  The
 // query's LIVE then lists its live vertices and initialises their T segments
 // (item_live2).  Progressive members also get the minima prog_bounds keeps.
 constexpr uint32_t kJLiveTV = 256;
@@ -1343,7 +1353,11 @@ __device__ void item_jlive(Slot* slots, Slot* sp, uint32_t tile, uint64_t V) {
   const uint64_t v1 = v0 + kJLiveTV < V ? v0 + kJLiveTV : V;
   const int nmem = sp->nmem;
   for (int mg = 0; mg < nmem; mg += 64) {
-    const int m = mg + lane;
+    const int rem = nmem - mg < 64 ? nmem - mg : 64;
+    int sh = 6;  // log2 of the lanes per member group
+    while (sh > 0 && (1 << (sh - 1)) >= rem) sh--;
+    const int m = mg + (lane & ((1 << sh) - 1));
+    const uint64_t sub = (uint64_t)(lane >> sh), ngr = (uint64_t)(64 >> sh);
     const bool has = m < nmem;
     Slot* q = has ? &slots[sp->jmem[m]] : nullptr;
     const int k = has ? q->k : 0;
@@ -1368,7 +1382,7 @@ __device__ void item_jlive(Slot* slots, Slot* sp, uint32_t tile, uint64_t V) {
     const uint32_t so = has ? (uint32_t)q->lab + (uint32_t)(sp->D >= 0 ? (uint64_t)(sp->D / 2) * Lp : 0) : 0u;
     uint32_t lbmn = kInf, smin = kInf;
     uint64_t pkmn = ~0ull;
-    for (uint64_t w0 = v0; w0 < v1; w0 += 32) {
+    for (uint64_t w0 = v0 + 32u * sub; w0 < v1; w0 += 32u * ngr) {
       const uint64_t w1 = w0 + 32 < v1 ? w0 + 32 : v1;
       uint32_t word = 0;
       for (uint64_t vb = w0; vb < w1; vb += kJLiveBatch) {
