@@ -82,12 +82,16 @@ def test_rmat_costs_with_and_without_backbone(rmat):
         assert v["ok"], v
 
 
-def test_rmat_hub_bound_fills_agree(rmat):
+@pytest.mark.parametrize("nq", [28, 32, 38])
+def test_rmat_hub_bound_fills_agree(rmat, nq):
     """Unbounded jobs start their rows at hub bounds, filled in packed 16-bit
     arithmetic (this graph's hub distances allow it), in 32-bit arithmetic,
-    or not at all (rows at infinity): the same costs, equal to the oracle."""
+    or not at all (rows at infinity): the same costs, equal to the oracle.
+    28, 32 and 38 queries of k = 5 make one job of 144, 160 and 192 labels,
+    whose last group of 16, 32 and 64 labels the 16-bit fill takes with 8,
+    16 and 32 lanes per vertex."""
     inst, g = rmat
-    qs = data.make_queries(32, 5, inst.meta["eligible"], 93)
+    qs = data.make_queries(nq, 5, inst.meta["eligible"], 93)
     costs = {}
     for hb, h16 in ((1, 1), (1, 0), (0, 0)):
         g.set_option("hub_bounds", hb)
