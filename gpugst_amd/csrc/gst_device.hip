@@ -1089,8 +1089,14 @@ __device__ void item_jstar(Slot* slots, Slot* sp, uint32_t tile, uint64_t V) {
     // shortest paths from one vertex to every group form a connected cover,
     // so their sum bounds the optimum from above (for a hop query, paths of
     // at most floor(D/2) hops: every two of them are within D hops of each
-    // other)
-    if (has && star < kInf) atomicMin(&q->best, star);
+    // other); one atomic per member and tile, only when it lowers best
+    for (int o = 1 << sh; o < 64; o <<= 1) {
+      const uint32_t a = __shfl_xor(star, o, 64);
+      star = a < star ? a : star;
+    }
+    if (has && sub == 0 && star < kInf &&
+        star < __hip_atomic_load(&q->best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      atomicMin(&q->best, star);
   }
 }
 
@@ -1142,7 +1148,9 @@ __device__ void item_dist(const GraphDev& g, Slot* sp, uint32_t tile) {
     star = s < star ? s : star;
   }
   star = wave_min_u32(star);
-  if (lane == 0 && star < kInf) atomicMin(&sp->best, star);
+  // one word for every tile of the query: lower it only when the tile beats it
+  if (lane == 0 && star < kInf && star < __hip_atomic_load(&sp->best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+    atomicMin(&sp->best, star);
 }
 
 __device__ void item_vvr(const GraphDev& g, Slot* sp, WaveLDS* L) {
@@ -1420,8 +1428,15 @@ __device__ void item_jlive(Slot* slots, Slot* sp, uint32_t tile, uint64_t V) {
       if (word) q->lvb[w0 >> 5] = word;
     }
     // the star is the cost of a real walk-tree: shortest paths from one
-    // vertex to every group
-    if (has && smin < kInf) atomicMin(&q->best, smin);
+    // vertex to every group.  A member's lane groups combine their stars,
+    // and its first lane lowers best only when the tile's star beats it:
+    // every tile of every member on the same word would otherwise queue at
+    // its L2 channel (progressive members, rare, keep one atomic per lane)
+    for (uint32_t o = 64u / (uint32_t)ngr; o < 64u; o <<= 1) {
+      const uint32_t a = __shfl_xor(smin, (int)o, 64);
+      smin = a < smin ? a : smin;
+    }
+    if (has && sub == 0 && smin <= lim) atomicMin(&q->best, smin);  // lim: best when the tile began
     if (prog) {
       if (lbmn < kInf) atomicMin(&q->lbmin, lbmn);
       if (pkmn != ~0ull) atomicMin((unsigned long long*)&q->star_pk, (unsigned long long)pkmn);
@@ -1645,7 +1660,8 @@ __device__ void item_merge(const GraphDev& g, Slot* sp, uint32_t tile, WaveLDS* 
   ubmin = wave_min_u32(ubmin);
   n_eval = wave_sum_u64(n_eval);
   if (lane == 0) {
-    if (ubmin < kInf) atomicMin(&sp->best, ubmin);
+    if (ubmin < kInf && ubmin < __hip_atomic_load(&sp->best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      atomicMin(&sp->best, ubmin);
     atomicAdd(&sp->ctr[5], (unsigned long long)n_eval);
   }
 }

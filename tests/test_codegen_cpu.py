@@ -103,14 +103,15 @@ def _kernel_meta(text, sym):
     return {k: int(v) for k, v in re.findall(r"\.(vgpr_count|vgpr_spill_count|sgpr_spill_count):\s+(\d+)", block)}
 
 
-@pytest.mark.parametrize("cls,max_vgpr,max_vspill", [("lean", 128, 4), ("heavy", 168, 16)])
+@pytest.mark.parametrize("cls,max_vgpr,max_vspill", [("lean", 128, 4), ("heavy", 168, 17)])
 def test_step_kernel_register_budget(device_asm, cls, max_vgpr, max_vspill):
     """Register budgets of the step kernels: the lean kernel (job rounds, whose
     64-byte chunks take the registers) at four waves per SIMD with at most
-    four VGPR spills, the heavy kernel at three with at most sixteen
+    four VGPR spills, the heavy kernel at three with at most seventeen
     (loop-invariant values stored once when the kernel starts and reloaded
-    at most once per work item: 13 spilled VGPRs in 8 reload sites in the
-    current build).  Code added to rare items has pushed both further before
+    at most once per work item: 17 spilled VGPRs in 10 reload sites in the
+    current build; the job live test's combined star update took it from 13,
+    and saved a third of that pass).  Code added to rare items has pushed both further before
     (a statistics pass is a kernel of its own for that reason, and the
     progressive bounds carry nothing across LIVE's loop)."""
     m = _kernel_meta(device_asm, KERNELS[cls][0])
