@@ -2320,7 +2320,10 @@ __device__ void step_body(const GraphDev& g, Slot* slots, Ctrl* ctrl, LDS* L) {
   // waves beyond the item count have nothing to claim: leave before touching
   // the shared counter (a small step then costs one atomic per item, not one
   // per wave of the grid)
-  if (blockIdx.x * (uint32_t)WPB + (threadIdx.x >> 6) >= total) return;
+  // (the same for the whole wave: read into a scalar register, so that the
+  // loop below stays uniform)
+  const uint32_t wid = __builtin_amdgcn_readfirstlane(blockIdx.x * (uint32_t)WPB + (threadIdx.x >> 6));
+  if (wid >= total) return;
   const uint32_t nact = ctrl->n_active[CLS];
   const uint32_t* prefix = ctrl->prefix[CLS];
   const uint32_t* active = ctrl->active[CLS];
@@ -2331,7 +2334,11 @@ __device__ void step_body(const GraphDev& g, Slot* slots, Ctrl* ctrl, LDS* L) {
   const uint32_t nwaves = gridDim.x * (uint32_t)WPB;
   uint32_t run = total / (8u * nwaves);
   run = run < 1u ? 1u : (run > 16u ? 16u : run);
-  for (;;) {
+  // A wave's first run of items is its own (items wid * run ...), so a wave
+  // starts without waiting on the counter; the counter hands out the runs
+  // after the first nwaves ones.
+  uint32_t it0 = wid * run;
+  for (bool first = true;; first = false) {
     // Claim items for the whole wave.  Every lane executes the atomic (only
     // lane 0 adds) and the index is read from lane 0 into a scalar register,
     // so the claim sits under no lane-dependent branch and the loop test is
@@ -2339,8 +2346,10 @@ __device__ void step_body(const GraphDev& g, Slot* slots, Ctrl* ctrl, LDS* L) {
     // the lane != 0 edge of the `if (lane == 0)` blocks that end several items
     // straight to the broadcast: the loop splits in two and lanes 1..63 re-run
     // the same item without lane 0, never finishing.
-    const uint32_t mine = atomicAdd(&ctrl->item_next[CLS], lane == 0 ? run : 0u);
-    const uint32_t it0 = __builtin_amdgcn_readfirstlane(mine);
+    if (!first) {
+      const uint32_t mine = atomicAdd(&ctrl->item_next[CLS], lane == 0 ? run : 0u);
+      it0 = nwaves * run + __builtin_amdgcn_readfirstlane(mine);
+    }
     if (it0 >= total) break;
     const uint32_t it1 = it0 + run < total ? it0 + run : total;
     for (uint32_t it = it0; it < it1; it++) {
