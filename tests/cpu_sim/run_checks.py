@@ -104,6 +104,10 @@ def check_jobs(V, nq, seed):
     g.set_groups(*groups_csr(groups))
     rs = g.solve_batch(qs)
     ctr = {x: g.get_option("batch_job_" + x) for x in ("entries", "arcs", "relaxations", "src_chunks")}
+    ctr["hub_bytes"] = g.get_option("hub_bytes")
+    g.set_option("hub16", 0)
+    ctr["hub_bytes_32"] = g.get_option("hub_bytes")
+    g.set_option("hub16", 1)
     g.set_option("backbone", 0)  # the same batch on every arc
     rf = g.solve_batch(qs)
     g.set_option("backbone", 1)

@@ -58,6 +58,9 @@ def test_group_distance_jobs_on_cpu_build(cpu_results):
     c = j["counters"]
     assert 0 < 16 * c["src_chunks"] <= c["relaxations"], c
     assert c["src_chunks"] <= 16 * c["entries"], c
+    # this graph's hub distances fit 16 bits: the fill reads 2 bytes per hub,
+    # 4 with the packed fill switched off
+    assert (c["hub_bytes"], c["hub_bytes_32"]) == (2, 4), c
 
 
 def test_hop_queries_on_cpu_build(cpu_results):
