@@ -96,6 +96,7 @@ def test_rmat_hub_bound_fills_agree(rmat, nq):
     for hb, h16 in ((1, 1), (1, 0), (0, 0)):
         g.set_option("hub_bounds", hb)
         g.set_option("hub16", h16)
+        assert g.get_option("hub_bytes") == (2 if h16 else 4)
         costs[(hb, h16)] = [r.cost for r in g.solve_batch(list(qs))]
         _say("hub_bounds", hb, "hub16", h16, "costs", costs[(hb, h16)][:6])
     g.set_option("hub_bounds", 1)
